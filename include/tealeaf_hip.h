@@ -1,0 +1,219 @@
+/*
+ * tealeaf_hip.h -- C ABI of libtealeaf_hip, the MI355X (gfx950) linear-solve
+ * hot path of the TeaLeaf 2-D heat-conduction mini-app.
+ *
+ * The ABI stands in for the solver plug-in interface of Laura7089/TeaLeaf.jl:
+ * a solver is a module whose `solve!(chunk::Chunk, set::Settings, rx, ry)` is
+ * called once per timestep (src/TeaLeaf.jl:74) and selected by `use_*` deck
+ * lines (src/settings.jl:103-111).  Every entry point returns an int status
+ * (TL_OK = 0); no C++ exception crosses the ABI.  A context owns one HIP
+ * stream on one device and is not thread-safe.
+ *
+ * Host arrays are Float64, column-major with x fastest, of size
+ * (nx + 2*halo_depth) * (ny + 2*halo_depth) -- the layout of the reference's
+ * Chunk fields (src/chunk.jl:19-38, 68-70).
+ */
+#ifndef TEALEAF_HIP_H
+#define TEALEAF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the reference throws strings instead (CG.jl:49, kernels.jl:42,80) */
+#define TL_OK 0
+#define TL_E_INVALID 1      /* invalid argument or deck value, e.g. coefficient (CG.jl:48-50) */
+#define TL_E_HIP 2          /* HIP runtime failure */
+#define TL_E_NOMEM 3        /* device allocation failed */
+#define TL_E_BREAKDOWN 4    /* p.Ap <= 0 or a non-finite residual */
+#define TL_E_EIGEN 5        /* negative eigenvalue or eigen-solve failure (kernels.jl:41-43, 80) */
+#define TL_E_PARSE 6        /* malformed deck */
+#define TL_E_STATE 7        /* call out of order (e.g. a learned warm start without a network) */
+
+/* solvers, settings.jl:103-111 */
+#define TL_CG 0
+#define TL_CHEBY 1
+#define TL_PPCG 2
+#define TL_JACOBI 3
+
+/* arithmetic of the solve */
+#define TL_FP64 0           /* everything in FP64 (the reference's Float64) */
+#define TL_FP32_IR 1        /* FP32 inner CG; FP64 residual, solution and reduction accumulators */
+
+/* stop rules */
+#define TL_STOP_REFERENCE 0 /* each solver's own rule: CG sqrt|rr| < eps (CG.jl:24); others |err| < eps */
+#define TL_STOP_RELATIVE 1  /* CG: ||u0 - A u||_2 <= rel_tol * ||u0||_2 (true FP64 residual at exit) */
+
+/* initial guess of the solve */
+#define TL_WARM_NONE 0      /* u = energy*density, the reference's start (CG.jl:52) */
+#define TL_WARM_LEARNED 1   /* U-Net prediction; falls back to TL_WARM_NONE when its residual is worse */
+#define TL_WARM_GUESS 2     /* caller-provided guess set with tl_set_guess, same fallback */
+
+/* field ids, in the order of the Chunk struct (chunk.jl:25-38) */
+#define TL_F_DENSITY0 0
+#define TL_F_DENSITY 1
+#define TL_F_ENERGY0 2
+#define TL_F_ENERGY 3
+#define TL_F_U 4
+#define TL_F_U0 5
+#define TL_F_P 6
+#define TL_F_R 7
+#define TL_F_MI 8
+#define TL_F_W 9
+#define TL_F_KX 10
+#define TL_F_KY 11
+#define TL_F_SD 12
+#define TL_NFIELDS 13
+
+/* halo exchange mask, EXCHANGE_FIELDS of chunk.jl:12 */
+#define TL_X_DENSITY (1u << 0)
+#define TL_X_P (1u << 1)
+#define TL_X_ENERGY0 (1u << 2)
+#define TL_X_ENERGY (1u << 3)
+#define TL_X_U (1u << 4)
+#define TL_X_SD (1u << 5)
+
+typedef struct tl_ctx tl_ctx;
+
+/* solver settings of Settings (settings.jl:39-78) plus the build's additions */
+typedef struct {
+    int solver;           /* TL_CG ... TL_JACOBI            (Settings.solver) */
+    int coefficient;      /* 1 conductivity, 2 reciprocal    (Settings.coefficient) */
+    int max_iters;        /*                                 (Settings.maxiters) */
+    int presteps;         /* CG steps before a Cheby/PPCG switch (Settings.presteps) */
+    int ppcg_inner_steps; /*                                 (Settings.ppcginnersteps) */
+    int error_switch;     /*                                 (Settings.errorswitch) */
+    double eps;           /*                                 (Settings.eps) */
+    double epslim;        /*                                 (Settings.epslim) */
+    int precision;        /* TL_FP64 | TL_FP32_IR */
+    int stop_rule;        /* TL_STOP_REFERENCE | TL_STOP_RELATIVE */
+    double rel_tol;       /* for TL_STOP_RELATIVE */
+    int warm_start;       /* TL_WARM_* */
+    int check_result;     /* Settings.checkresult: residual! in tl_solve_finished */
+    double ir_delta;      /* TL_FP32_IR: refresh the FP64 residual when the FP32 one falls by this factor */
+    int reserved[8];
+} tl_solve_params;
+
+/* replaces the reference's @info lines (CG.jl:27, Cheby.jl:60, PPCG.jl:54, Jacobi.jl:29) */
+typedef struct {
+    int iterations;       /* outer iterations (tt at exit) */
+    int cg_iters;         /* CG iterations (before the switch for Cheby/PPCG) */
+    int cheby_iters;      /* Chebyshev / PPCG iterations after the switch */
+    int est_iters;        /* Chebyshev iteration estimate (Cheby.calciter) */
+    int inner_iters;      /* TL_FP32_IR: FP32 CG iterations */
+    int refreshes;        /* TL_FP32_IR: FP64 residual recomputations */
+    int converged;        /* stop rule met (0: ran max_iters) */
+    int fallback;         /* warm start rejected in favour of u = energy*density */
+    double error;         /* final error as the solver defines it */
+    double eigmin, eigmax;
+    double rel_resid;     /* ||u0 - A u|| / ||u0|| of the returned u, FP64 */
+    double init_rel_resid;/* same for the initial guess that was used */
+    double solve_ms;      /* wall time of tl_solve */
+    double reserved[4];
+} tl_solve_stats;
+
+/* a parsed tea.in deck (settings.jl:90-173); states keep the deck's coordinates */
+#define TL_MAX_STATES 128
+typedef struct {
+    double density, energy, xmin, ymin, xmax, ymax, radius;
+    int geometry;         /* 0 rectangle, 1 circular, 2 point */
+    int defined;
+} tl_state;
+
+typedef struct {
+    int x_cells, y_cells, halo_depth;
+    double xmin, ymin, xmax, ymax, dx, dy;
+    long long end_step;
+    int summary_frequency;
+    double dt_init, end_time;
+    int check_result;
+    tl_solve_params params;
+    int nstates;
+    tl_state states[TL_MAX_STATES];
+    int n_warnings;       /* unknown keys, as settings.jl:109,127 warn */
+} tl_deck;
+
+/* ------------------------------------------------------------ lifecycle */
+void tl_default_params(tl_solve_params *p);
+/* Chunk(settings), chunk.jl:68-89: nx*ny interior cells, domain origin and cell size */
+int tl_create(tl_ctx **ctx, int nx, int ny, int halo_depth, double xmin, double ymin, double dx, double dy,
+              int device);
+void tl_destroy(tl_ctx *ctx);
+const char *tl_last_error(const tl_ctx *ctx);
+int tl_dims(const tl_ctx *ctx, int *X, int *Y);
+
+/* ---------------------------------------------------------------- fields */
+/* H2D / D2H of one reference field in the host layout described above */
+int tl_set_field(tl_ctx *ctx, int field_id, const double *host);
+int tl_get_field(tl_ctx *ctx, int field_id, double *host);
+/* density and energy as the Julia driver holds them before solve! (TeaLeaf.jl:71-74) */
+int tl_set_fields(tl_ctx *ctx, const double *density, const double *energy);
+/* caller's initial guess (host, full layout) for TL_WARM_GUESS */
+int tl_set_guess(tl_ctx *ctx, const double *host);
+
+/* ----------------------------------------------------- per-step operations */
+/* setchunkstate! + initialiseapp!, chunk.jl:122-151 and TeaLeaf.jl:35-44, on the device */
+int tl_initialise(tl_ctx *ctx, const tl_state *states, int nstates);
+/* haloupdate!, kernels.jl:146-159 (reflective, depth layers, interior span) */
+int tl_halo_update(tl_ctx *ctx, unsigned field_mask, int depth);
+/* set.solver.solve!(chunk, set, rx, ry), TeaLeaf.jl:74 */
+int tl_solve(tl_ctx *ctx, const tl_solve_params *p, double rx, double ry, tl_solve_stats *out);
+/* solvefinished!, kernels.jl:166-170: residual! (if check_result), finalise!, halo(energy) */
+int tl_solve_finished(tl_ctx *ctx, int check_result);
+/* fieldsummary, kernels.jl:119-133: out = {vol, mass, ie, temp} */
+int tl_field_summary(tl_ctx *ctx, double out[4]);
+/* one timestep of diffuse!, TeaLeaf.jl:67-81 (dt = Settings.dtinit) */
+int tl_step(tl_ctx *ctx, const tl_solve_params *p, double dt, tl_solve_stats *out);
+
+/* -------------------------------------------------------------- deck / run */
+int tl_deck_parse(const char *text, tl_deck *out);
+/* Chunk(settings) + initialiseapp! for a parsed deck */
+int tl_deck_create(tl_ctx **ctx, const tl_deck *deck, int device);
+/* whole diffuse! loop (TeaLeaf.jl:62-83) on the device; cb (may be NULL) sees every step */
+typedef void (*tl_step_cb)(int step, const double summary[4], const tl_solve_stats *stats, void *user);
+int tl_run(tl_ctx *ctx, const tl_deck *deck, const tl_solve_params *p, int nsteps, tl_step_cb cb, void *user);
+
+/* ------------------------------------------------------ host-side scalars */
+/* eigenvalues! (kernels.jl:19-51): extrema of the Lanczos tridiagonal of n CG
+ * steps (alpha, beta 0-based, n entries), scaled by 0.95 / 1.05 */
+int tl_eigen_bounds(const double *alpha, const double *beta, int n, double *eigmin, double *eigmax);
+/* Cheby.coef!, Cheby.jl:121-135 (alpha, beta 0-based, n entries) */
+void tl_cheby_coef(double eigmin, double eigmax, int n, double *alpha, double *beta, double *theta);
+/* Cheby.calciter, Cheby.jl:109-118 */
+int tl_cheby_calciter(double eigmin, double eigmax, double error, double bb);
+
+/* ------------------------------------------------- device-side diagnostics */
+/* w = A*field over the interior (smvp, kernels.jl:178-184), result left in TL_F_W */
+int tl_apply_operator(tl_ctx *ctx, int field_id, double rx, double ry, int coefficient);
+/* CG history (cgα, cgβ of chunk.jl:56-57) of the last solve, n entries */
+int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
+
+/* --------------------------------------------------- learned warm start */
+/* BF16 U-Net (3 levels, 16/32/64 channels, 3x3 convs, FP32 accumulation)
+ * predicting the timestep's change of u from (u_prev, density, energy, Kx, Ky).
+ * No counterpart in the reference. */
+typedef struct {
+    int steps;            /* optimiser steps */
+    int batch;            /* crops per step */
+    int crop;             /* crop edge in cells (multiple of 8) */
+    float lr;             /* Adam learning rate */
+    int seed;
+    int reserved[8];
+} tl_train_params;
+
+/* number of FP32 parameters of the network */
+int tl_unet_param_count(void);
+/* load / read the network's FP32 master weights (host, tl_unet_param_count entries) */
+int tl_unet_set_weights(tl_ctx *ctx, const float *w);
+int tl_unet_get_weights(tl_ctx *ctx, float *w);
+/* in-device dataset of training samples recorded from this context's own solves */
+int tl_unet_record(tl_ctx *ctx, int enable);
+int tl_unet_dataset_size(tl_ctx *ctx);
+int tl_unet_train(tl_ctx *ctx, const tl_train_params *tp, float *loss_out);
+/* predict the next u into TL_F_MI (host view) from the current fields */
+int tl_unet_predict(tl_ctx *ctx, double rx, double ry, int coefficient);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,866 @@
+// tl_api.hip -- C ABI and host-side solver drivers of libtealeaf_hip.
+//
+// The drivers follow the solver modules of Laura7089/TeaLeaf.jl
+// (src/solvers/{CG,Cheby,PPCG,Jacobi}.jl) in their intended form.  The host
+// only enqueues kernels: every scalar recurrence (alpha, beta, stop tests)
+// runs on the device (tl_kernels.hip), and the host polls a snapshot of the
+// device control block once per batch of iterations, one batch ahead, so
+// the GPU never idles waiting for the host inside a solve.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tl_internal.hpp"
+
+using namespace tl;
+
+namespace tl {
+hipError_t launch_copy_full(tl_ctx *c, const double *a, double *b);
+hipError_t launch_resid_norm_slot(tl_ctx *c, const double *b, const double *x, int slot);
+hipError_t launch_residual_u(tl_ctx *c, const double *u, int with_norm, int finalize_kind);
+hipError_t launch_cheby_iter2(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int it);
+hipError_t launch_jacobi2(tl_ctx *c, const double *u_in, double *u_out, int tt, int defer);
+int unet_predict_guess(tl_ctx *c, double rx, double ry, int coefficient, double *guess_dev);
+void unet_destroy(tl_ctx *c);
+int unet_record_step(tl_ctx *c, double rx, double ry, int coefficient);
+}  // namespace tl
+
+#define HIPCHK(c, expr)                                                                \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            (c)->err = std::string(#expr) + ": " + hipGetErrorString(e_);             \
+            return TL_E_HIP;                                                           \
+        }                                                                              \
+    } while (0)
+
+#define RETIF(expr)                  \
+    do {                             \
+        int rc_ = (expr);            \
+        if (rc_ != TL_OK) return rc_; \
+    } while (0)
+
+static thread_local std::string g_create_err;
+
+extern "C" void tl_default_params(tl_solve_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    // Settings defaults, settings.jl:41-59
+    p->solver = TL_CG;
+    p->coefficient = 1;
+    p->max_iters = 10000;
+    p->presteps = 30;
+    p->ppcg_inner_steps = 10;
+    p->error_switch = 0;
+    p->eps = 1e-15;
+    p->epslim = 1e-5;
+    p->precision = TL_FP64;
+    p->stop_rule = TL_STOP_REFERENCE;
+    p->rel_tol = 1e-10;
+    p->warm_start = TL_WARM_NONE;
+    p->check_result = 1;
+    p->ir_delta = 0.1;
+}
+
+// ------------------------------------------------------------- lifecycle
+
+static int alloc_d(tl_ctx *c, double **p, size_t n)
+{
+    if (hipMalloc((void **)p, n * sizeof(double)) != hipSuccess) {
+        c->err = "device allocation failed";
+        *p = nullptr;
+        return TL_E_NOMEM;
+    }
+    if (hipMemset(*p, 0, n * sizeof(double)) != hipSuccess) return TL_E_HIP;
+    return TL_OK;
+}
+
+static int alloc_f(tl_ctx *c, float **p, size_t n)
+{
+    if (hipMalloc((void **)p, n * sizeof(float)) != hipSuccess) {
+        c->err = "device allocation failed";
+        *p = nullptr;
+        return TL_E_NOMEM;
+    }
+    if (hipMemset(*p, 0, n * sizeof(float)) != hipSuccess) return TL_E_HIP;
+    return TL_OK;
+}
+
+static hipEvent_t *events(tl_ctx *c);
+
+extern "C" int tl_create(tl_ctx **out, int nx, int ny, int halo_depth, double xmin, double ymin, double dx,
+                         double dy, int device)
+{
+    *out = nullptr;
+    if (nx < 2 || ny < 2 || halo_depth < 1 || halo_depth > 16 || nx < halo_depth || ny < halo_depth ||
+        !(dx > 0) || !(dy > 0)) {
+        g_create_err = "tl_create: invalid grid";
+        return TL_E_INVALID;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_err = "tl_create: no HIP device";
+        return TL_E_HIP;
+    }
+    if (device < 0 || device >= ndev) {
+        g_create_err = "tl_create: invalid device";
+        return TL_E_INVALID;
+    }
+    tl_ctx *c = new (std::nothrow) tl_ctx();
+    if (!c) return TL_E_NOMEM;
+    c->device = device;
+    c->xmin = xmin;
+    c->ymin = ymin;
+    c->dx = dx;
+    c->dy = dy;
+    c->g = make_geo(nx, ny, halo_depth);
+    int rc = TL_OK;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = TL_E_HIP;
+    for (int i = 0; i < TL_NFIELDS && rc == TL_OK; ++i) rc = alloc_d(c, &c->f[i], c->g.elems);
+    if (rc == TL_OK) rc = alloc_d(c, &c->u_alt, c->g.elems);
+    if (rc == TL_OK) rc = alloc_d(c, &c->sd_alt, c->g.elems);
+    c->partials_len = (size_t)4 * c->g.gx * c->g.gy;
+    if (rc == TL_OK) rc = alloc_d(c, &c->partials, c->partials_len);
+    if (rc == TL_OK && hipMalloc((void **)&c->ctrl, sizeof(Ctrl)) != hipSuccess) rc = TL_E_NOMEM;
+    if (rc == TL_OK && hipMemset(c->ctrl, 0, sizeof(Ctrl)) != hipSuccess) rc = TL_E_HIP;
+    if (rc == TL_OK && hipHostMalloc((void **)&c->ctrl_h, 2 * sizeof(Ctrl), hipHostMallocDefault) != hipSuccess)
+        rc = TL_E_NOMEM;
+    if (rc == TL_OK && !events(c)) rc = TL_E_HIP;
+    if (rc != TL_OK) {
+        g_create_err = c->err.empty() ? "tl_create: HIP setup failed" : c->err;
+        tl_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return TL_OK;
+}
+
+static hipEvent_t *events(tl_ctx *c)
+{
+    if (!c->ev[0] && hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (!c->ev[1] && hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming) != hipSuccess) return nullptr;
+    return c->ev;
+}
+
+extern "C" void tl_destroy(tl_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    unet_destroy(c);
+    void *bufs[] = {c->u_alt, c->sd_alt, c->guess, c->kx32, c->ky32, c->p32, c->r32, c->w32, c->x32,
+                    c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
+    for (int i = 0; i < TL_NFIELDS; ++i)
+        if (c->f[i]) (void)hipFree(c->f[i]);
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->ctrl_h) (void)hipHostFree(c->ctrl_h);
+    if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
+    if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char *tl_last_error(const tl_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+extern "C" int tl_dims(const tl_ctx *c, int *X, int *Y)
+{
+    if (!c) return TL_E_INVALID;
+    *X = c->g.X;
+    *Y = c->g.Y;
+    return TL_OK;
+}
+
+// ---------------------------------------------------------------- fields
+
+static double *field_ptr(tl_ctx *c, int id) { return (id >= 0 && id < TL_NFIELDS) ? c->f[id] : nullptr; }
+
+static int copy_in(tl_ctx *c, double *dev, const double *host)
+{
+    const Geo &g = c->g;
+    HIPCHK(c, hipMemcpy2DAsync(dev + (XCOL - g.hd), (size_t)g.pitch * sizeof(double), host,
+                               (size_t)g.X * sizeof(double), (size_t)g.X * sizeof(double), (size_t)g.Y,
+                               hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TL_OK;
+}
+
+static int copy_out(tl_ctx *c, const double *dev, double *host)
+{
+    const Geo &g = c->g;
+    HIPCHK(c, hipMemcpy2DAsync(host, (size_t)g.X * sizeof(double), dev + (XCOL - g.hd),
+                               (size_t)g.pitch * sizeof(double), (size_t)g.X * sizeof(double), (size_t)g.Y,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TL_OK;
+}
+
+extern "C" int tl_set_field(tl_ctx *c, int id, const double *host)
+{
+    if (!c || !host) return TL_E_INVALID;
+    double *d = field_ptr(c, id);
+    if (!d) {
+        c->err = "tl_set_field: invalid field id";
+        return TL_E_INVALID;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    return copy_in(c, d, host);
+}
+
+extern "C" int tl_get_field(tl_ctx *c, int id, double *host)
+{
+    if (!c || !host) return TL_E_INVALID;
+    double *d = field_ptr(c, id);
+    if (!d) {
+        c->err = "tl_get_field: invalid field id";
+        return TL_E_INVALID;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    return copy_out(c, d, host);
+}
+
+extern "C" int tl_set_fields(tl_ctx *c, const double *density, const double *energy)
+{
+    if (!c) return TL_E_INVALID;
+    RETIF(tl_set_field(c, TL_F_DENSITY, density));
+    return tl_set_field(c, TL_F_ENERGY, energy);
+}
+
+extern "C" int tl_set_guess(tl_ctx *c, const double *host)
+{
+    if (!c || !host) return TL_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->guess) RETIF(alloc_d(c, &c->guess, c->g.elems));
+    return copy_in(c, c->guess, host);
+}
+
+extern "C" int tl_halo_update(tl_ctx *c, unsigned mask, int depth)
+{
+    if (!c || depth < 0 || depth > c->g.hd) return TL_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_halo(c, mask, depth, nullptr, 0));
+    return TL_OK;
+}
+
+extern "C" int tl_initialise(tl_ctx *c, const tl_state *states, int nstates)
+{
+    if (!c || !states || nstates < 1 || nstates > TL_MAX_STATES) return TL_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->states_d) HIPCHK(c, hipMalloc((void **)&c->states_d, sizeof(tl_state) * TL_MAX_STATES));
+    HIPCHK(c, hipMemcpyAsync(c->states_d, states, sizeof(tl_state) * (size_t)nstates, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, launch_set_state(c, c->states_d, nstates));
+    // initialiseapp!, TeaLeaf.jl:38-41
+    HIPCHK(c, launch_halo(c, TL_X_DENSITY | TL_X_ENERGY0 | TL_X_ENERGY, 1, nullptr, 0));
+    HIPCHK(c, launch_copy_energy(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TL_OK;
+}
+
+// ------------------------------------------------------- batch pipeline
+
+// Enqueues `body` (one iteration's kernels) in batches, one batch ahead of
+// the host's check, until the device control block reports F.  Returns with
+// the stream synchronised and *out holding the final control block.
+template <class Body>
+static int run_until_stop(tl_ctx *c, int kmax, long max_batches, Body &&body, Ctrl *out)
+{
+    hipEvent_t *ev = events(c);
+    Ctrl *snap = c->ctrl_h;
+    int k = 4;
+    for (long b = 0;; ++b) {
+        for (int i = 0; i < k; ++i) HIPCHK(c, body());
+        HIPCHK(c, hipMemcpyAsync(&snap[b & 1], c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipEventRecord(ev[b & 1], c->stream));
+        if (k < kmax) k *= 2;
+        if (b > 0) {
+            HIPCHK(c, hipEventSynchronize(ev[(b - 1) & 1]));
+            if (snap[(b - 1) & 1].F || b > max_batches) {
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                *out = snap[b & 1];
+                if (!out->F) {
+                    c->err = "solve did not stop within its iteration budget";
+                    return TL_E_HIP;
+                }
+                return TL_OK;
+            }
+        }
+    }
+}
+
+static int read_ctrl(tl_ctx *c, Ctrl *out)
+{
+    HIPCHK(c, hipMemcpyAsync(c->ctrl_h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = c->ctrl_h[0];
+    return TL_OK;
+}
+
+static int ensure_history(tl_ctx *c, int n)
+{
+    if (c->hist_len >= n + 2) return TL_OK;
+    if (c->hist_a) (void)hipFree(c->hist_a);
+    if (c->hist_b) (void)hipFree(c->hist_b);
+    c->hist_a = c->hist_b = nullptr;
+    RETIF(alloc_d(c, &c->hist_a, (size_t)n + 2));
+    RETIF(alloc_d(c, &c->hist_b, (size_t)n + 2));
+    c->hist_len = n + 2;
+    return TL_OK;
+}
+
+static int ensure_fp32(tl_ctx *c)
+{
+    if (c->kx32) return TL_OK;
+    const size_t n = c->g.elems;
+    RETIF(alloc_f(c, &c->kx32, n));
+    RETIF(alloc_f(c, &c->ky32, n));
+    RETIF(alloc_f(c, &c->p32, n));
+    RETIF(alloc_f(c, &c->r32, n));
+    RETIF(alloc_f(c, &c->w32, n));
+    RETIF(alloc_f(c, &c->x32, n));
+    return TL_OK;
+}
+
+static int status_of(tl_ctx *c, const Ctrl &k)
+{
+    if (k.status == TL_E_BREAKDOWN || k.why == WHY_BREAKDOWN) {
+        c->err = "solver breakdown: non-positive p.Ap or non-finite residual";
+        return TL_E_BREAKDOWN;
+    }
+    return TL_OK;
+}
+
+// ------------------------------------------------------ host scalar work
+
+// Extrema of the eigenvalues of the symmetric tridiagonal (d, e) by Sturm
+// sequence bisection (count of eigenvalues below x from the signs of the
+// LDL^T pivots), to full double precision.
+static int sturm_count(const std::vector<double> &d, const std::vector<double> &e, double x)
+{
+    int cnt = 0;
+    double q = 1.0;
+    const size_t n = d.size();
+    for (size_t i = 0; i < n; ++i) {
+        const double e2 = i ? e[i - 1] * e[i - 1] : 0.0;
+        q = d[i] - x - (i ? e2 / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0) ++cnt;
+    }
+    return cnt;
+}
+
+static double bisect_kth(const std::vector<double> &d, const std::vector<double> &e, int kth, double lo, double hi)
+{
+    // smallest x with count(x) > kth  (0-based kth eigenvalue)
+    for (int it = 0; it < 200 && hi - lo > 0; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        if (sturm_count(d, e, mid) > kth)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+extern "C" int tl_eigen_bounds(const double *alpha, const double *beta, int n, double *eigmin, double *eigmax)
+{
+    // eigenvalues!, kernels.jl:19-51: Lanczos tridiagonal of n CG steps
+    if (n < 1) return TL_E_EIGEN;
+    std::vector<double> d(n), e(n > 1 ? n - 1 : 0);
+    for (int i = 0; i < n; ++i) {
+        d[i] = 1.0 / alpha[i];
+        if (i > 0) d[i] += beta[i - 1] / alpha[i - 1];
+        if (i < n - 1) e[i] = std::sqrt(beta[i]) / alpha[i];
+    }
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(d[i])) return TL_E_EIGEN;
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; ++i) {
+        const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0);
+        lo = std::min(lo, d[i] - r);
+        hi = std::max(hi, d[i] + r);
+    }
+    const double pad = 1e-12 * std::max(std::fabs(lo), std::fabs(hi)) + 1e-300;
+    lo -= pad;
+    hi += pad;
+    const double mn = bisect_kth(d, e, 0, lo, hi);
+    const double mx = bisect_kth(d, e, n - 1, lo, hi);
+    if (mn < 0 || mx < 0) return TL_E_EIGEN;  // kernels.jl:41-43
+    *eigmin = mn * 0.95;
+    *eigmax = mx * 1.05;
+    return TL_OK;
+}
+
+extern "C" void tl_cheby_coef(double eigmin, double eigmax, int n, double *alpha, double *beta, double *theta)
+{
+    // Cheby.coef!, Cheby.jl:121-135
+    const double th = (eigmax + eigmin) / 2.0;
+    const double delta = (eigmax - eigmin) / 2.0;
+    const double sigma = th / delta;
+    double rho_old = 1.0 / sigma;
+    for (int i = 0; i < n; ++i) {
+        const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+        alpha[i] = rho_new * rho_old;
+        beta[i] = 2.0 * rho_new / delta;
+        rho_old = rho_new;
+    }
+    *theta = th;
+}
+
+extern "C" int tl_cheby_calciter(double eigmin, double eigmax, double error, double bb)
+{
+    // Cheby.calciter, Cheby.jl:109-118 (round to nearest, ties to even)
+    const double connum = eigmax / eigmin;
+    const double ita = 2.220446049250313e-16 * bb / (4.0 * error);
+    const double gamma = (std::sqrt(connum) - 1.0) / (std::sqrt(connum) + 1.0);
+    const double v = (std::log(ita) / std::log(10.0)) / (2.0 * (std::log(gamma) / std::log(10.0)));
+    if (!std::isfinite(v)) return -1;
+    return (int)std::nearbyint(v);
+}
+
+// --------------------------------------------------------------- solvers
+
+static int finish_stats(tl_ctx *c, tl_solve_stats *st)
+{
+    // true relative residual of the returned u (FP64)
+    HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U0], c->f[TL_F_U], 2));
+    HIPCHK(c, launch_sumsq(c, c->f[TL_F_U0], FIN_NONE));
+    Ctrl k;
+    RETIF(read_ctrl(c, &k));
+    st->rel_resid = k.sums[0] > 0 ? std::sqrt(k.sums[2] / k.sums[0]) : std::sqrt(k.sums[2]);
+    return TL_OK;
+}
+
+// Warm start: put the candidate guess in c->guess and keep it only if its
+// residual is smaller than that of u = energy*density (the fallback).
+static int choose_guess(tl_ctx *c, const tl_solve_params *p, double rx, double ry, const double **guess,
+                        tl_solve_stats *st)
+{
+    *guess = nullptr;
+    if (p->warm_start == TL_WARM_NONE) return TL_OK;
+    if (p->warm_start == TL_WARM_LEARNED) {
+        if (!c->guess) RETIF(alloc_d(c, &c->guess, c->g.elems));
+        RETIF(unet_predict_guess(c, rx, ry, p->coefficient, c->guess));
+    } else if (p->warm_start == TL_WARM_GUESS) {
+        if (!c->guess) {
+            c->err = "TL_WARM_GUESS without tl_set_guess";
+            return TL_E_STATE;
+        }
+    } else {
+        return TL_E_INVALID;
+    }
+    HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U], c->guess, 0));
+    HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U], c->f[TL_F_U], 1));
+    Ctrl k;
+    RETIF(read_ctrl(c, &k));
+    if (std::isfinite(k.sums[0]) && k.sums[0] < k.sums[1]) {
+        *guess = c->guess;
+    } else {
+        st->fallback = 1;
+    }
+    return TL_OK;
+}
+
+static void fill_stats(const Ctrl &k, tl_solve_stats *st)
+{
+    st->error = k.err;
+    st->converged = k.why == WHY_CONVERGED;
+}
+
+// CG.solve!, CG.jl:7-29 (FP64)
+static int solve_cg64(tl_ctx *c, const tl_solve_params *p, double rx, double ry, tl_solve_stats *st)
+{
+    RETIF(ensure_history(c, p->max_iters));
+    HIPCHK(c, launch_init_coef(c, rx, ry, p->coefficient, INIT_CG));
+    const int mode = p->stop_rule == TL_STOP_RELATIVE ? MODE_CG_REL : MODE_CG_REF;
+    HIPCHK(c, launch_init_ctrl(c, mode, p->max_iters, p->eps, p->rel_tol, 0.0, p->presteps, p->error_switch,
+                               p->epslim));
+    const double *guess = nullptr;
+    RETIF(choose_guess(c, p, rx, ry, &guess, st));
+    HIPCHK(c, launch_cg_init_resid(c, guess));
+    Ctrl k;
+    RETIF(run_until_stop(c, 64, (long)p->max_iters + 16,
+                         [&]() -> hipError_t {
+                             hipError_t e = launch_cg_w64(c);
+                             if (e == hipSuccess) e = launch_cg_ur64(c);
+                             if (e == hipSuccess) e = launch_cg_p64(c);
+                             return e;
+                         },
+                         &k));
+    st->init_rel_resid = k.bb > 0 ? std::sqrt(k.sums[0] / k.bb) : 0.0;
+    st->iterations = st->cg_iters = k.iters;
+    c->last_cg_iters = k.iters;
+    fill_stats(k, st);
+    HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P, 1, nullptr, 0));
+    return status_of(c, k);
+}
+
+// the CG phase shared by Cheby and PPCG: CG-init, then CG steps until the
+// switch test of Cheby.jl:25-29 / PPCG.jl:25-30 (or convergence / max_iters)
+static int cg_phase(tl_ctx *c, const tl_solve_params *p, double rx, double ry, Ctrl *k, tl_solve_stats *st)
+{
+    RETIF(ensure_history(c, p->max_iters));
+    HIPCHK(c, launch_init_coef(c, rx, ry, p->coefficient, INIT_CG));
+    HIPCHK(c, launch_init_ctrl(c, MODE_SWITCH, p->max_iters, p->eps, p->rel_tol, 0.0, p->presteps,
+                               p->error_switch, p->epslim));
+    HIPCHK(c, launch_cg_init_resid(c, nullptr));
+    RETIF(run_until_stop(c, 32, (long)p->max_iters + 16,
+                         [&]() -> hipError_t {
+                             hipError_t e = launch_cg_w64(c);
+                             if (e == hipSuccess) e = launch_cg_ur64(c);
+                             if (e == hipSuccess) e = launch_cg_p64(c);
+                             return e;
+                         },
+                         k));
+    st->init_rel_resid = k->bb > 0 ? std::sqrt(k->sums[0] / k->bb) : 0.0;
+    st->cg_iters = k->iters;
+    st->iterations = k->iters;
+    c->last_cg_iters = k->iters;
+    fill_stats(*k, st);
+    return status_of(c, *k);
+}
+
+static int eigen_from_history(tl_ctx *c, int cgiters, double *eigmin, double *eigmax)
+{
+    std::vector<double> a(cgiters), b(cgiters);
+    HIPCHK(c, hipMemcpyAsync(a.data(), c->hist_a, sizeof(double) * cgiters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.data(), c->hist_b, sizeof(double) * cgiters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int rc = tl_eigen_bounds(a.data(), b.data(), cgiters, eigmin, eigmax);
+    if (rc != TL_OK) c->err = "eigenvalue estimate failed (negative or non-finite)";
+    return rc;
+}
+
+// Cheby.solve!, Cheby.jl:10-61 (intended form)
+static int solve_cheby(tl_ctx *c, const tl_solve_params *p, double rx, double ry, tl_solve_stats *st)
+{
+    Ctrl k;
+    RETIF(cg_phase(c, p, rx, ry, &k, st));
+    if (k.why != WHY_SWITCH) {
+        HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P, 1, nullptr, 0));
+        return TL_OK;
+    }
+    const int cgiters = k.iters;
+    const int maxiters = p->max_iters;
+    double eigmin, eigmax, theta;
+    RETIF(eigen_from_history(c, cgiters, &eigmin, &eigmax));
+    st->eigmin = eigmin;
+    st->eigmax = eigmax;
+    const int ncoef = maxiters - cgiters;
+    std::vector<double> ca(maxiters + 2, 0.0), cb(maxiters + 2, 0.0);  // 1-based, zero past ncoef
+    {
+        std::vector<double> ta(std::max(ncoef, 1)), tb(std::max(ncoef, 1));
+        tl_cheby_coef(eigmin, eigmax, ncoef, ta.data(), tb.data(), &theta);
+        for (int i = 0; i < ncoef; ++i) {
+            ca[i + 1] = ta[i];
+            cb[i + 1] = tb[i];
+        }
+    }
+    double *ucur = c->f[TL_F_U], *unext = c->u_alt;
+    // Cheby.init! and the first main step (Cheby.jl:40-45)
+    HIPCHK(c, launch_cheby_init(c, theta, ucur, unext));
+    std::swap(ucur, unext);
+    HIPCHK(c, launch_set_mode(c, MODE_JACOBI /* |err| < eps */, maxiters));
+    HIPCHK(c, launch_cheby_iter2(c, ca[2], cb[2], ucur, unext, 1, 1));
+    std::swap(ucur, unext);
+    RETIF(read_ctrl(c, &k));
+    int chebyiters = 1;
+    const int estiter = tl_cheby_calciter(eigmin, eigmax, k.err, k.bb);
+    st->est_iters = estiter;
+    int tt = cgiters + 1;
+    if (!(std::fabs(k.err) < p->eps)) {
+        // main loop (Cheby.jl:46-53); each launch swaps u buffers on the host
+        std::vector<double *> cur_after;  // u buffer after each chebyiter index
+        cur_after.reserve(64);
+        const int first = chebyiters;
+        long launched = first;
+        int done_at = -1;
+        while (tt < maxiters && done_at < 0) {
+            // one batch: up to the next norm iteration or 64 launches
+            int nb = 0;
+            while (tt < maxiters && nb < 64) {
+                ++tt;
+                ++chebyiters;
+                const int norm = (chebyiters >= estiter) && ((tt + 1) % 10 == 0);
+                HIPCHK(c, launch_cheby_iter2(c, ca[chebyiters + 1], cb[chebyiters + 1], ucur, unext, norm,
+                                             chebyiters));
+                std::swap(ucur, unext);
+                ++launched;
+                ++nb;
+                if (norm) break;
+            }
+            Ctrl kk;
+            RETIF(read_ctrl(c, &kk));
+            if (kk.F) done_at = kk.iters;
+        }
+        if (done_at > 0 && ((chebyiters - done_at) & 1)) std::swap(ucur, unext);
+        if (done_at > 0) chebyiters = done_at;
+        tt = cgiters + chebyiters;
+        (void)launched;
+    }
+    if (ucur != c->f[TL_F_U]) std::swap(c->f[TL_F_U], c->u_alt);
+    RETIF(read_ctrl(c, &k));
+    st->iterations = tt;
+    st->cheby_iters = chebyiters;
+    st->error = k.err;
+    st->converged = std::fabs(k.err) < p->eps;
+    HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P, 1, nullptr, 0));
+    return TL_OK;
+}
+
+// PPCG.solve!, PPCG.jl:9-55 (intended form)
+static int solve_ppcg(tl_ctx *c, const tl_solve_params *p, double rx, double ry, tl_solve_stats *st)
+{
+    Ctrl k;
+    RETIF(cg_phase(c, p, rx, ry, &k, st));
+    if (k.why != WHY_SWITCH) {
+        HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P, 1, nullptr, 0));
+        return TL_OK;
+    }
+    const int cgiters = k.iters;
+    const int inner = p->ppcg_inner_steps;
+    double eigmin, eigmax, theta;
+    RETIF(eigen_from_history(c, cgiters, &eigmin, &eigmax));
+    st->eigmin = eigmin;
+    st->eigmax = eigmax;
+    std::vector<double> ca(std::max(inner, 1)), cb(std::max(inner, 1));
+    tl_cheby_coef(eigmin, eigmax, inner, ca.data(), cb.data(), &theta);
+    // PPCG.init!(chunk, settings), PPCG.jl:58-61: r = u0 - A u
+    HIPCHK(c, launch_residual(c, 0, FIN_NONE));
+    HIPCHK(c, launch_set_mode(c, MODE_PPCG, p->max_iters - cgiters));
+    double *sd0 = c->f[TL_F_SD], *sd1 = c->sd_alt;
+    RETIF(run_until_stop(c, 16, (long)p->max_iters + 16,
+                         [&]() -> hipError_t {
+                             hipError_t e = launch_cg_w64(c);
+                             if (e == hipSuccess) e = launch_ppcg_ur_sd(c, theta);
+                             double *a = sd0, *b = sd1;
+                             for (int pp = 0; pp < inner && e == hipSuccess; ++pp) {
+                                 e = launch_ppcg_inner(c, ca[pp], cb[pp], a, b, pp == inner - 1);
+                                 std::swap(a, b);
+                             }
+                             if (inner == 0 && e == hipSuccess) e = launch_sumsq(c, c->f[TL_F_R], FIN_PPCG);
+                             if (e == hipSuccess) e = launch_cg_p64(c);
+                             return e;
+                         },
+                         &k));
+    if (inner & 1) std::swap(c->f[TL_F_SD], c->sd_alt);
+    st->cheby_iters = k.iters;
+    st->iterations = cgiters + k.iters;
+    fill_stats(k, st);
+    HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P | TL_X_SD, 1, nullptr, 0));
+    return status_of(c, k);
+}
+
+// Jacobi.driver!, Jacobi.jl:7-31 (intended form)
+static int solve_jacobi(tl_ctx *c, const tl_solve_params *p, double rx, double ry, tl_solve_stats *st)
+{
+    HIPCHK(c, launch_init_coef(c, rx, ry, p->coefficient, INIT_JACOBI));
+    HIPCHK(c, launch_init_ctrl(c, MODE_JACOBI, p->max_iters, p->eps, p->rel_tol, 0.0, 0, 0, p->epslim));
+    double *ucur = c->f[TL_F_U], *unext = c->u_alt;
+    int tt = 0;
+    Ctrl k;
+    RETIF(run_until_stop(c, 50, (long)p->max_iters + 16,
+                         [&]() -> hipError_t {
+                             if (tt >= p->max_iters) return hipSuccess;
+                             ++tt;
+                             const int every = (tt % 50) == 0;
+                             hipError_t e = launch_jacobi2(c, ucur, unext, tt, every);
+                             if (every && e == hipSuccess) e = launch_residual_u(c, unext, 1, FIN_JACOBI_ADD);
+                             std::swap(ucur, unext);
+                             return e;
+                         },
+                         &k));
+    // tt launches were made; the last one that ran was k.iters
+    if ((tt - k.iters) & 1) std::swap(ucur, unext);
+    if (ucur != c->f[TL_F_U]) std::swap(c->f[TL_F_U], c->u_alt);
+    st->iterations = k.iters;
+    fill_stats(k, st);
+    HIPCHK(c, launch_halo(c, TL_X_U, 1, nullptr, 0));
+    return status_of(c, k);
+}
+
+// FP32 inner CG with FP64 residual replacement ("reliable updates"): the
+// FP32 recurrence runs on the correction x; when its residual has fallen by
+// ir_delta (or reaches the tolerance) the FP64 solution is accumulated,
+// u += x, and the FP64 residual r = u0 - A u replaces the FP32 one, keeping
+// the search direction.  Converged when the FP64 residual meets rel_tol.
+static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry, tl_solve_stats *st)
+{
+    RETIF(ensure_fp32(c));
+    HIPCHK(c, launch_init_coef(c, rx, ry, p->coefficient, INIT_CG));
+    HIPCHK(c, launch_ir_prepare(c));
+    const double delta = p->ir_delta > 0 && p->ir_delta < 1 ? p->ir_delta : 0.1;
+    HIPCHK(c, launch_init_ctrl(c, MODE_IR, p->max_iters, p->eps, p->rel_tol, delta, 0, 0, p->epslim));
+    const double *guess = nullptr;
+    RETIF(choose_guess(c, p, rx, ry, &guess, st));
+    // u0 = b = energy*density; u = guess or b; bb, tol_abs
+    HIPCHK(c, launch_cg_init_resid(c, guess));
+    HIPCHK(c, hipMemsetAsync(c->x32, 0, c->g.elems * sizeof(float), c->stream));
+    HIPCHK(c, launch_ir_residual(c, 1));
+    Ctrl k;
+    RETIF(read_ctrl(c, &k));
+    st->init_rel_resid = k.bb > 0 ? std::sqrt(k.sums[0] / k.bb) : 0.0;
+    int refreshes = 0;
+    while (!(k.F && k.why != WHY_REFRESH)) {
+        if (k.F && k.why == WHY_REFRESH) {
+            HIPCHK(c, launch_ir_accumulate(c));
+            HIPCHK(c, launch_ir_residual(c, 0));
+            ++refreshes;
+            RETIF(read_ctrl(c, &k));
+            continue;
+        }
+        RETIF(run_until_stop(c, 64, (long)p->max_iters + 16,
+                             [&]() -> hipError_t {
+                                 hipError_t e = launch_cg_w32(c);
+                                 if (e == hipSuccess) e = launch_cg_ur32(c);
+                                 if (e == hipSuccess) e = launch_cg_p32(c);
+                                 return e;
+                             },
+                             &k));
+    }
+    if (k.why == WHY_MAXITERS) HIPCHK(c, launch_ir_accumulate(c));  // keep the partial correction
+    st->inner_iters = k.iters;
+    st->iterations = k.iters;
+    st->cg_iters = k.iters;
+    st->refreshes = refreshes;
+    fill_stats(k, st);
+    HIPCHK(c, launch_halo(c, TL_X_U, 1, nullptr, 0));
+    return status_of(c, k);
+}
+
+extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double ry, tl_solve_stats *st)
+{
+    if (!c || !p || !st) return TL_E_INVALID;
+    std::memset(st, 0, sizeof(*st));
+    if (p->coefficient != 1 && p->coefficient != 2) {
+        c->err = "Coefficient " + std::to_string(p->coefficient) + " is not valid";  // CG.jl:48-50
+        return TL_E_INVALID;
+    }
+    if (p->max_iters < 1 || p->solver < TL_CG || p->solver > TL_JACOBI || p->ppcg_inner_steps < 0) {
+        c->err = "tl_solve: invalid parameters";
+        return TL_E_INVALID;
+    }
+    if (p->precision == TL_FP32_IR && (p->solver != TL_CG || p->stop_rule != TL_STOP_RELATIVE)) {
+        c->err = "TL_FP32_IR needs the CG solver and the relative stop rule";
+        return TL_E_INVALID;
+    }
+    if (p->warm_start != TL_WARM_NONE && p->solver != TL_CG) {
+        c->err = "warm starts apply to the CG solver";
+        return TL_E_INVALID;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    if (p->precision == TL_FP32_IR) {
+        rc = solve_cg_ir(c, p, rx, ry, st);
+    } else {
+        switch (p->solver) {
+        case TL_CG: rc = solve_cg64(c, p, rx, ry, st); break;
+        case TL_CHEBY: rc = solve_cheby(c, p, rx, ry, st); break;
+        case TL_PPCG: rc = solve_ppcg(c, p, rx, ry, st); break;
+        default: rc = solve_jacobi(c, p, rx, ry, st); break;
+        }
+    }
+    if (rc != TL_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    st->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    RETIF(finish_stats(c, st));
+    // training data for the learned warm start: (inputs of this step, solution)
+    if (c->recording) RETIF(unet_record_step(c, rx, ry, p->coefficient));
+    return TL_OK;
+}
+
+extern "C" int tl_solve_finished(tl_ctx *c, int check_result)
+{
+    if (!c) return TL_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    // solvefinished!, kernels.jl:166-170
+    if (check_result) HIPCHK(c, launch_residual(c, 0, FIN_NONE));
+    HIPCHK(c, launch_finalise(c));
+    HIPCHK(c, launch_halo(c, TL_X_ENERGY, 1, nullptr, 0));
+    return TL_OK;
+}
+
+extern "C" int tl_field_summary(tl_ctx *c, double out[4])
+{
+    if (!c || !out) return TL_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_field_summary(c));
+    Ctrl k;
+    RETIF(read_ctrl(c, &k));
+    for (int i = 0; i < 4; ++i) out[i] = k.sums[i];
+    return TL_OK;
+}
+
+extern "C" int tl_step(tl_ctx *c, const tl_solve_params *p, double dt, tl_solve_stats *st)
+{
+    if (!c || !p) return TL_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    // diffuse!, TeaLeaf.jl:69-77
+    const double rx = dt / (c->dx * c->dx), ry = dt / (c->dy * c->dy);
+    HIPCHK(c, launch_halo(c, TL_X_ENERGY | TL_X_DENSITY, 1, nullptr, 0));
+    RETIF(tl_solve(c, p, rx, ry, st));
+    return tl_solve_finished(c, p->check_result);
+}
+
+extern "C" int tl_apply_operator(tl_ctx *c, int field_id, double rx, double ry, int coefficient)
+{
+    if (!c) return TL_E_INVALID;
+    double *src = field_ptr(c, field_id);
+    if (!src || field_id == TL_F_W || field_id == TL_F_KX || field_id == TL_F_KY) return TL_E_INVALID;
+    if (coefficient != 1 && coefficient != 2) return TL_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    // coefficients from the current density (CG.jl:56-68), fields untouched
+    HIPCHK(c, launch_init_coef(c, rx, ry, coefficient, INIT_COEF_ONLY));
+    HIPCHK(c, launch_apply(c, src, c->f[TL_F_W]));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TL_OK;
+}
+
+extern "C" int tl_get_cg_history(tl_ctx *c, double *alpha, double *beta, int n)
+{
+    if (!c || n < 0 || n > c->hist_len) return TL_E_INVALID;
+    if (n == 0) return TL_OK;
+    HIPCHK(c, hipMemcpyAsync(alpha, c->hist_a, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(beta, c->hist_b, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TL_OK;
+}
+
+// ------------------------------------------------------------ deck / run
+
+extern "C" int tl_deck_create(tl_ctx **out, const tl_deck *d, int device)
+{
+    if (!out || !d) return TL_E_INVALID;
+    RETIF(tl_create(out, d->x_cells, d->y_cells, d->halo_depth, d->xmin, d->ymin, d->dx, d->dy, device));
+    const int rc = tl_initialise(*out, d->states, d->nstates);
+    if (rc != TL_OK) {
+        g_create_err = (*out)->err;
+        tl_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+extern "C" int tl_run(tl_ctx *c, const tl_deck *d, const tl_solve_params *p, int nsteps, tl_step_cb cb, void *user)
+{
+    if (!c || !d) return TL_E_INVALID;
+    const tl_solve_params *pp = p ? p : &d->params;
+    const long long n = nsteps > 0 ? nsteps : d->end_step;
+    for (long long t = 1; t <= n; ++t) {
+        tl_solve_stats st;
+        RETIF(tl_step(c, pp, d->dt_init, &st));
+        double s[4];
+        RETIF(tl_field_summary(c, s));
+        if (cb) cb((int)t, s, &st, user);
+    }
+    return TL_OK;
+}

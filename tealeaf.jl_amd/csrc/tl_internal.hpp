@@ -1,0 +1,142 @@
+// tl_internal.hpp -- shared definitions of libtealeaf_hip (not part of the ABI).
+//
+// Device layout.  Each field is one pitched allocation.  The reference's
+// 0-based element (k, j) of an X*Y Chunk array (chunk.jl:19-38) lives at
+//     j * pitch + k + (32 - hd)
+// so the first interior cell of every row (k = hd) sits at column 32, i.e.
+// 256-byte aligned for FP64 and 128-byte aligned for FP32, and
+// pitch = roundup(nx + hd + 32, 64) elements.  Interior coordinates (x, y),
+// 0 <= x < nx, 0 <= y < ny, address  off + y * pitch + x  with
+// off = hd * pitch + 32.  The face coefficient kx(x, y) couples cells x-1 and
+// x (the reference's kx[k] with k = x + hd); ky likewise in y.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/tealeaf_hip.h"
+
+namespace tl {
+
+constexpr int TB = 256;        // threads per block (one row segment of 256 cells)
+constexpr int XCOL = 32;       // storage column of the first interior cell
+
+struct Geo {
+    int nx, ny, hd;
+    int X, Y;                  // nx + 2hd, ny + 2hd
+    long pitch;                // elements per stored row
+    long off;                  // element offset of interior (0, 0)
+    int gx, gy, ty;            // stencil launch: gx*gy blocks of TB columns x ty rows
+    size_t elems;              // elements per allocation
+};
+
+// Device-resident solver scalars.  Kernels read them at start and the last
+// block of a reduction kernel writes them, so a timestep never round-trips
+// through the host between iterations.
+enum StopMode : int {
+    MODE_CG_REF = 0,    // sqrt|rr| < eps                       (CG.jl:24)
+    MODE_CG_REL = 1,    // sqrt|rr| <= tol_abs                  (relative rule)
+    MODE_SWITCH = 2,    // CG phase of Cheby/PPCG: |rr| < eps, or the switch test of Cheby.jl:25-29
+    MODE_IR = 3,        // FP32 CG: refresh trigger or candidate convergence
+    MODE_PPCG = 4,      // PPCG outer iteration: |rr| < eps     (PPCG.jl:50)
+    MODE_JACOBI = 5,    // |err| < eps                          (Jacobi.jl:26)
+};
+enum Why : int { WHY_NONE = 0, WHY_CONVERGED = 1, WHY_SWITCH = 2, WHY_MAXITERS = 3, WHY_REFRESH = 4,
+                 WHY_BREAKDOWN = 5 };
+
+struct Ctrl {
+    double rro, pw, alpha, beta, err;
+    double bb;          // sum u0^2
+    double tol_abs;     // absolute stop threshold for MODE_CG_REL / MODE_IR
+    double refresh_abs; // MODE_IR: refresh when sqrt(rr) drops below this
+    double eps, epslim;
+    double rel_tol;     // MODE_CG_REL / MODE_IR: tol_abs = rel_tol * sqrt(bb)
+    double ir_delta;    // MODE_IR: refresh_abs = ir_delta * sqrt(rr_true)
+    double sums[4];     // last multi-value reduction (field summary, residual norms)
+    int iters;          // completed iterations of the current phase
+    int maxiters;
+    int mode;
+    int presteps, errorswitch;
+    int F;              // stop: set by the deciding reduction
+    int G;              // set by the first kernel of an iteration that saw F
+    int why;
+    int status;
+    unsigned ticket;
+    int pad[2];
+};
+
+struct Unet;  // learned warm start (tl_unet.hip)
+
+}  // namespace tl
+
+struct tl_ctx {
+    tl::Geo g;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double xmin = 0, ymin = 0, dx = 1, dy = 1;
+    // FP64 fields (reference Chunk fields; u and sd have ping-pong twins)
+    double *f[TL_NFIELDS] = {};
+    double *u_alt = nullptr, *sd_alt = nullptr, *guess = nullptr;
+    // FP32 fields of the mixed-precision solve
+    float *kx32 = nullptr, *ky32 = nullptr, *p32 = nullptr, *r32 = nullptr, *w32 = nullptr, *x32 = nullptr;
+    // reductions
+    double *partials = nullptr;
+    size_t partials_len = 0;
+    tl::Ctrl *ctrl = nullptr;        // device
+    tl::Ctrl *ctrl_h = nullptr;      // pinned host mirror
+    double *hist_a = nullptr, *hist_b = nullptr;  // device CG alpha/beta history
+    int hist_len = 0;
+    int last_cg_iters = 0;
+    // rasterisation
+    tl_state *states_d = nullptr;
+    std::string err;
+    hipEvent_t ev[2] = {nullptr, nullptr};   // batch pipeline
+    tl::Unet *unet = nullptr;
+    bool recording = false;
+};
+
+namespace tl {
+
+// ------------------------------------------------------------ launchers
+// All launchers enqueue on ctx->stream and return hipError_t.
+// what k_init_coef writes besides kx, ky
+enum { INIT_CG = 0, INIT_JACOBI = 1, INIT_COEF_ONLY = 2 };
+hipError_t launch_init_coef(tl_ctx *c, double rx, double ry, int coefficient, int what);
+hipError_t launch_set_state(tl_ctx *c, const tl_state *states_d, int nstates);
+hipError_t launch_u_from_energy(tl_ctx *c);
+hipError_t launch_copy_energy(tl_ctx *c);
+hipError_t launch_halo(tl_ctx *c, unsigned mask, int depth, double *const *extra, int nextra);
+hipError_t launch_cg_init_resid(tl_ctx *c, const double *guess);
+hipError_t launch_resid_norm(tl_ctx *c, const double *x, double *out_dev_sum);
+hipError_t launch_cg_w64(tl_ctx *c);
+hipError_t launch_cg_ur64(tl_ctx *c);
+hipError_t launch_cg_p64(tl_ctx *c);
+hipError_t launch_cg_w32(tl_ctx *c);
+hipError_t launch_cg_ur32(tl_ctx *c);
+hipError_t launch_cg_p32(tl_ctx *c);
+hipError_t launch_residual(tl_ctx *c, int with_norm, int finalize_kind);
+hipError_t launch_finalise(tl_ctx *c);
+hipError_t launch_field_summary(tl_ctx *c);
+hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double *u_out);
+hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm);
+hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta);
+hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last);
+hipError_t launch_jacobi_iter(tl_ctx *c, const double *u_in, double *u_out, int defer);
+hipError_t launch_ir_prepare(tl_ctx *c);
+hipError_t launch_ir_accumulate(tl_ctx *c);
+hipError_t launch_ir_residual(tl_ctx *c, int reset_p);
+hipError_t launch_apply(tl_ctx *c, const double *src, double *dst);
+hipError_t launch_init_ctrl(tl_ctx *c, int mode, int maxiters, double eps, double rel_tol, double ir_delta,
+                            int presteps, int errorswitch, double epslim);
+hipError_t launch_set_mode(tl_ctx *c, int mode, int maxiters);
+hipError_t launch_sumsq(tl_ctx *c, const double *v, int finalize_kind);
+
+// finalizer kinds of launch_residual
+enum { FIN_NONE = 0, FIN_JACOBI_ADD = 1, FIN_IR = 2, FIN_SUMS = 3, FIN_PPCG = 4 };
+
+Geo make_geo(int nx, int ny, int hd);
+
+}  // namespace tl

@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark: wall seconds per timestep at 8192^2, relative residual 1e-10.
+
+Flagship: FP32 inner CG with FP64 residual replacement, warm-started from
+the learned U-Net when trained weights are present (falling back per step
+to the previous-step guess when the prediction's residual is worse).
+Baseline: plain FP64 CG from the previous-step guess (the reference's
+u = energy*density start).  Both run on the same seeded random-rectangle
+problem (background rho=100, e=1e-4 plus 32 rectangles; dt 0.004) and must
+reach ||u0 - A u|| <= 1e-10 ||u0|| in FP64 every step.
+
+A step is one timestep of diffuse!: halo(energy, density), the solve,
+solvefinished! and the field summary, all on the device, with the fields
+resident in HBM.  W untimed warmup steps, then K timed steps per method.
+
+Prints one JSON line (rank 0).  `roofline` is for the flagship's dominant
+kernel (largest total time among the FP32 CG kernels), timed per launch with
+HIP events on the library's stream during the timed steps; `cpu_baseline`
+times the CPU oracle's FP64 CG on the host for a bounded number of
+iterations and extrapolates to the GPU's iteration count.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tealeaf.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+import tealeaf_hip as T  # noqa: E402
+from tealeaf_hip import _lib, decks  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+# algorithmic bytes per interior cell per launch (arrays read once + written once)
+KERNEL_BYTES = {
+    _lib.TL_K_CG_W64: 32, _lib.TL_K_CG_UR64: 48, _lib.TL_K_CG_P64: 24,
+    _lib.TL_K_CG_W32: 16, _lib.TL_K_CG_UR32: 24, _lib.TL_K_CG_P32: 12,
+}
+WEIGHTS = os.path.join(ROOT, "tealeaf.jl_amd", "weights", "unet_warmstart.npy")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_settings(n, seed, precision, warm):
+    text = decks.random_rect_deck(seed, n=n, steps=1000, precision=precision, warm_start=warm)
+    return T.Settings.from_text(text)
+
+
+def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0):
+    s = make_settings(n, seed, precision, warm)
+    chunk = T.initialiseapp(s)
+    if warm == "learned":
+        from tealeaf_hip import unet
+        unet.load_weights(chunk, WEIGHTS)
+    stats = []
+    for w in range(warmup):
+        st = T.step(chunk, s)
+        T.fieldsummary(chunk)
+        log(f"[{name}] warmup step {w + 1}: {st['iterations']} it, {st['solve_ms']:.0f} ms, "
+            f"rel_resid {st['rel_resid']:.2e}")
+    L = _lib.load()
+    if profile_mask:
+        _lib.check(L.tl_profile_begin(chunk._ctx, profile_mask), chunk._ctx, "profile")
+    times = []
+    summaries = []
+    t_all0 = time.perf_counter()
+    for k in range(steps):
+        t0 = time.perf_counter()
+        st = T.step(chunk, s)
+        summ = T.fieldsummary(chunk)
+        t1 = time.perf_counter()
+        times.append(t1 - t0)
+        stats.append(st)
+        summaries.append(summ)
+        log(f"[{name}] step {warmup + k + 1}: {t1 - t0:.3f} s, {st['iterations']} it, "
+            f"refreshes {st['refreshes']}, fallback {st['fallback']}, rel_resid {st['rel_resid']:.2e}")
+    t_all1 = time.perf_counter()
+    prof = {}
+    if profile_mask:
+        for kid in range(6):
+            if profile_mask & (1 << kid):
+                cnt = C.c_int()
+                _lib.check(L.tl_profile_read(chunk._ctx, kid, None, 0, C.byref(cnt)), chunk._ctx, "profile")
+                buf = np.zeros(cnt.value)
+                _lib.check(L.tl_profile_read(chunk._ctx, kid, buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                             cnt.value, C.byref(cnt)), chunk._ctx, "profile")
+                prof[kid] = buf
+        L.tl_profile_end(chunk._ctx)
+    chunk.close()
+    return {"times": times, "total": t_all1 - t_all0, "stats": stats, "summaries": summaries, "prof": prof}
+
+
+def roofline_from(prof, n):
+    """Dominant kernel by total time; average over launches that ran (launches
+    made after a solve's stop flag exit at once and are excluded)."""
+    best = None
+    for kid, d in prof.items():
+        if len(d) == 0:
+            continue
+        med = float(np.median(d))
+        live = d[d > 0.25 * med]
+        tot = float(live.sum())
+        if best is None or tot > best[1]:
+            best = (kid, tot, live, len(d) - len(live))
+    if best is None:
+        return None
+    kid, tot, live, skipped = best
+    avg_ms = float(live.mean())
+    algo_bytes = KERNEL_BYTES[kid] * n * n
+    achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
+    return {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+        "kernel": _lib.KERNEL_NAMES[kid], "avg_ms": round(avg_ms, 5), "launches": int(len(live)),
+        "stopped_launches_excluded": int(skipped), "algorithmic_bytes_per_launch": algo_bytes,
+        "bytes_per_cell": KERNEL_BYTES[kid],
+    }
+
+
+def cpu_baseline(n, seed, gpu_iters, budget_s):
+    """The CPU oracle's FP64 CG on the host: time a bounded number of CG
+    iterations of the first step at n^2, then extrapolate s/timestep with the
+    GPU's FP64 iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    text = decks.random_rect_deck(seed, n=n, steps=1)
+    d = O.parse_deck(text)
+    d.stop_rule = O.STOP_RELATIVE
+    d.rel_tol = 1e-10
+
+    d.maxiters = 2000
+    o = O.Oracle(d, threads=threads)
+    o.initialise()
+
+    def timed(iters):
+        o.set_maxiters(iters)
+        t0 = time.perf_counter()
+        st = o.step()
+        return time.perf_counter() - t0, st
+
+    t2, _ = timed(2)
+    t4, _ = timed(4)
+    per_iter = max((t4 - t2) / 2, 1e-9)
+    fixed = max(t2 - 2 * per_iter, 0.0)
+    iters = int(max(4, min(2000, (budget_s - fixed) / per_iter)))
+    tn, st = timed(iters)
+    per_iter = (tn - fixed) / iters
+    return {
+        "value": round(fixed + per_iter * gpu_iters, 3), "unit": "s/timestep", "cores": threads, "kind": "port",
+        "sample": (f"CPU oracle FP64 CG at {n}^2 (seed {seed}): {iters} iterations timed ({tn:.1f} s, "
+                   f"{per_iter * 1e3:.1f} ms/iteration, step overhead {fixed:.2f} s), extrapolated to the GPU "
+                   f"FP64 CG's {gpu_iters:.0f} iterations/step"),
+    }
+
+
+def kernel_repeat(n, seed, kid, reps):
+    """Back-to-back launches of one kernel on the 8192^2 problem (for
+    counter runs); kid -1 is a same-size copy kernel (16 B/cell)."""
+    s = make_settings(n, seed, "fp64", "none")
+    chunk = T.initialiseapp(s)
+    _lib.check(_lib.load().tl_kernel_repeat(chunk._ctx, kid, reps), chunk._ctx, "kernel_repeat")
+    chunk.close()
+    print(json.dumps({"kernel": _lib.KERNEL_NAMES[kid], "reps": reps, "n": n}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--seed", type=int, default=1001)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--kernel-repeat", type=int, nargs=2, metavar=("KERNEL_ID", "REPS"))
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the solve shards by 2-D domain decomposition "
+              "(see DESIGN.md) but no multi-GPU path is built, so --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    if a.kernel_repeat:
+        kernel_repeat(a.n, a.seed, a.kernel_repeat[0], a.kernel_repeat[1])
+        return
+    learned = os.path.exists(WEIGHTS)
+    warm = "learned" if learned else "none"
+    mask = (1 << _lib.TL_K_CG_W32) | (1 << _lib.TL_K_CG_UR32) | (1 << _lib.TL_K_CG_P32)
+    flag = run_method("fp32-ir" + ("+learned" if learned else ""), a.n, a.seed, "fp32_ir", warm, a.steps,
+                      a.warmup, mask)
+    base = None
+    if not a.no_baseline:
+        base = run_method("fp64-cg", a.n, a.seed, "fp64", "none", a.steps, a.warmup)
+    value = float(np.mean(flag["times"]))
+    out = {
+        "metric": "s/timestep at 8192², rel. resid 1e-10: learned+FP32-refined vs plain FP64 CG",
+        "value": round(value, 4), "unit": "s/timestep", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(value * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 inner / fp64 residual+solution+reductions", "data": "synthetic",
+        "config": {
+            "workload": f"TeaLeaf {a.n}x{a.n} random rectangles (32, seed {a.seed}), dt 0.004, rel. resid 1e-10",
+            "grid": a.n, "seed": a.seed, "warm_start": warm, "parallelism": "single GPU",
+        },
+        "flagship": {
+            "s_per_step": [round(t, 4) for t in flag["times"]],
+            "iterations": [st["iterations"] for st in flag["stats"]],
+            "refreshes": [st["refreshes"] for st in flag["stats"]],
+            "fallback": [st["fallback"] for st in flag["stats"]],
+            "rel_resid": [st["rel_resid"] for st in flag["stats"]],
+        },
+    }
+    if base is not None:
+        bval = float(np.mean(base["times"]))
+        out["baseline_fp64_cg"] = {
+            "s_per_step": [round(t, 4) for t in base["times"]], "mean": round(bval, 4),
+            "iterations": [st["iterations"] for st in base["stats"]],
+            "rel_resid": [st["rel_resid"] for st in base["stats"]],
+        }
+        out["speedup_vs_fp64_cg"] = round(bval / value, 3)
+        diffs = [float(np.max(np.abs(f - b) / np.abs(b))) for f, b in zip(flag["summaries"], base["summaries"])]
+        out["field_summary_max_rel_diff_vs_fp64"] = max(diffs)
+    out["roofline"] = roofline_from(flag["prof"], a.n)
+    if not a.no_cpu:
+        it = (np.mean([st["iterations"] for st in base["stats"]]) if base is not None
+              else np.mean([st["iterations"] for st in flag["stats"]]))
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(a.n, a.seed, it, a.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

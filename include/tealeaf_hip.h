@@ -188,6 +188,26 @@ int tl_apply_operator(tl_ctx *ctx, int field_id, double rx, double ry, int coeff
 /* CG history (cgα, cgβ of chunk.jl:56-57) of the last solve, n entries */
 int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 
+/* ------------------------------------------------------------ profiling */
+/* kernel ids for tl_profile_begin */
+#define TL_K_CG_W64 0
+#define TL_K_CG_UR64 1
+#define TL_K_CG_P64 2
+#define TL_K_CG_W32 3
+#define TL_K_CG_UR32 4
+#define TL_K_CG_P32 5
+/* Bracket every launch of the kernels in kernel_mask (bit = 1 << TL_K_*)
+ * with HIP events on the context's stream; tl_profile_read returns one
+ * kernel's per-launch durations (ms, launch order; launches made after a
+ * solve's stop flag exit at once and are near-empty).  No counterpart in
+ * the reference (it has no timers). */
+int tl_profile_begin(tl_ctx *ctx, unsigned kernel_mask);
+int tl_profile_read(tl_ctx *ctx, int kernel_id, double *durations_ms, int max, int *count);
+int tl_profile_end(tl_ctx *ctx);
+/* `reps` back-to-back launches of one kernel on the current fields (a copy
+ * kernel of the same size when kernel_id = -1), for counter collection */
+int tl_kernel_repeat(tl_ctx *ctx, int kernel_id, int reps);
+
 /* --------------------------------------------------- learned warm start */
 /* BF16 U-Net (3 levels, 16/32/64 channels, 3x3 convs, FP32 accumulation)
  * predicting the timestep's change of u from (u_prev, density, energy, Kx, Ky).

@@ -85,6 +85,7 @@ def lib():
         L.or_create.argtypes = [C.POINTER(_Settings)]
         L.or_destroy.argtypes = [vp]
         L.or_dims.argtypes = [vp, ip, ip]
+        L.or_set_maxiters.argtypes = [vp, C.c_int]
         L.or_field.restype = dp
         L.or_field.argtypes = [vp, C.c_int]
         L.or_history.restype = dp
@@ -319,6 +320,10 @@ class Oracle:
     def history(self, which: int, n: int) -> np.ndarray:
         p = lib().or_history(self._c, which)
         return np.ctypeslib.as_array(p, shape=(n + 2,)).copy()
+
+    def set_maxiters(self, n: int):
+        if lib().or_set_maxiters(self._c, n):
+            raise ValueError("maxiters above the allocated history")
 
     def initialise(self):
         rc = lib().or_initialise(self._c)

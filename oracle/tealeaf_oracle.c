@@ -34,6 +34,7 @@ struct or_chunk {
     double theta, eigmin, eigmax;       /* chunk.jl:52-54 */
     double *rowsum;
     unsigned toex;                      /* settings.toexchange, settings.jl:46-47 */
+    int maxalloc;                       /* length of the histories */
 };
 
 #define AT(c, k, j) ((size_t)(j) * (size_t)(c)->X + (size_t)(k))
@@ -78,6 +79,7 @@ or_chunk *or_create(const or_settings *s)
     c->cy = (double *)calloc((size_t)c->Y, sizeof(double));
     c->rowsum = (double *)calloc((size_t)c->Y, sizeof(double));
     for (int i = 0; i < 4; i++) c->hist[i] = (double *)calloc((size_t)s->maxiters + 2, sizeof(double));
+    c->maxalloc = s->maxiters;
     /* vertexx = xmin + dx*((1:x+1) - hd - 1), chunk.jl:76-77 */
     for (int i = 0; i <= c->X; i++) c->vx[i] = s->xmin + s->dx * (double)(i - c->hd);
     for (int i = 0; i <= c->Y; i++) c->vy[i] = s->ymin + s->dy * (double)(i - c->hd);
@@ -97,6 +99,15 @@ void or_destroy(or_chunk *c)
 }
 
 int or_dims(const or_chunk *c, int *X, int *Y) { *X = c->X; *Y = c->Y; return c->hd; }
+
+/* lower the iteration cap (the CG history was sized by the settings' maxiters) */
+int or_set_maxiters(or_chunk *c, int n)
+{
+    if (n < 1) return 1;
+    if (n > c->maxalloc) return 2;
+    c->s.maxiters = n;
+    return 0;
+}
 double *or_field(or_chunk *c, int id) { return (id >= 0 && id < OR_NFIELDS) ? c->f[id] : NULL; }
 double *or_history(or_chunk *c, int which) { return (which >= 0 && which < 4) ? c->hist[which] : NULL; }
 
@@ -219,8 +230,10 @@ static int set_chunk_state(or_chunk *c)
     const or_settings *s = &c->s;
     if (s->nstates < 1) return 1;
     for (size_t i = 0; i < n; i++) { e0[i] = s->states[0].energy; rho[i] = s->states[0].density; }
+    int bad = 0;
     for (int si = 0; si < s->nstates; si++) {
         const or_state *st = &s->states[si];
+#pragma omp parallel for schedule(static) reduction(| : bad)
         for (int j = 0; j < Y; j++) {
             for (int k = 0; k < X; k++) {
                 int apply = 0;
@@ -237,7 +250,7 @@ static int set_chunk_state(or_chunk *c)
                 size_t idx;
                 if (s->literal & OR_LIT_D4) {
                     /* ch.energy0[j, k]: x-index j, y-index k */
-                    if (j >= X || k >= Y) return 2; /* BoundsError in the reference */
+                    if (j >= X || k >= Y) { bad = 1; continue; } /* BoundsError in the reference */
                     idx = (size_t)k * (size_t)X + (size_t)j;
                 } else {
                     idx = AT(c, k, j);
@@ -247,6 +260,7 @@ static int set_chunk_state(or_chunk *c)
             }
         }
     }
+    if (bad) return 2;
     /* u = energy0*density on halo(chunk, 1), chunk.jl:149-150 */
     double *u = c->f[OR_F_U];
     for (int j = 1; j < Y - 1; j++)

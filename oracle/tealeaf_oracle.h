@@ -77,6 +77,7 @@ typedef struct or_chunk or_chunk;
 or_chunk *or_create(const or_settings *s);
 void or_destroy(or_chunk *c);
 int or_dims(const or_chunk *c, int *X, int *Y);
+int or_set_maxiters(or_chunk *c, int n);
 double *or_field(or_chunk *c, int id);
 double *or_history(or_chunk *c, int which); /* 0 cgalpha 1 cgbeta 2 chebyalpha 3 chebybeta */
 void or_set_threads(int n);

@@ -7,6 +7,7 @@
 // device control block once per batch of iterations, one batch ahead, so
 // the GPU never idles waiting for the host inside a solve.
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -20,11 +21,6 @@
 using namespace tl;
 
 namespace tl {
-hipError_t launch_copy_full(tl_ctx *c, const double *a, double *b);
-hipError_t launch_resid_norm_slot(tl_ctx *c, const double *b, const double *x, int slot);
-hipError_t launch_residual_u(tl_ctx *c, const double *u, int with_norm, int finalize_kind);
-hipError_t launch_cheby_iter2(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int it);
-hipError_t launch_jacobi2(tl_ctx *c, const double *u_in, double *u_out, int tt, int defer);
 int unet_predict_guess(tl_ctx *c, double rx, double ry, int coefficient, double *guess_dev);
 void unet_destroy(tl_ctx *c);
 int unet_record_step(tl_ctx *c, double rx, double ry, int coefficient);
@@ -125,7 +121,7 @@ extern "C" int tl_create(tl_ctx **out, int nx, int ny, int halo_depth, double xm
     for (int i = 0; i < TL_NFIELDS && rc == TL_OK; ++i) rc = alloc_d(c, &c->f[i], c->g.elems);
     if (rc == TL_OK) rc = alloc_d(c, &c->u_alt, c->g.elems);
     if (rc == TL_OK) rc = alloc_d(c, &c->sd_alt, c->g.elems);
-    c->partials_len = (size_t)4 * c->g.gx * c->g.gy;
+    c->partials_len = (size_t)7 * c->g.gx * c->g.gy;  // pw | pr0 | pr1 | 4 x misc
     if (rc == TL_OK) rc = alloc_d(c, &c->partials, c->partials_len);
     if (rc == TL_OK && hipMalloc((void **)&c->ctrl, sizeof(Ctrl)) != hipSuccess) rc = TL_E_NOMEM;
     if (rc == TL_OK && hipMemset(c->ctrl, 0, sizeof(Ctrl)) != hipSuccess) rc = TL_E_HIP;
@@ -161,6 +157,7 @@ extern "C" void tl_destroy(tl_ctx *c)
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ctrl_h) (void)hipHostFree(c->ctrl_h);
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
     if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -266,8 +263,8 @@ extern "C" int tl_initialise(tl_ctx *c, const tl_state *states, int nstates)
 // ------------------------------------------------------- batch pipeline
 
 // Enqueues `body` (one iteration's kernels) in batches, one batch ahead of
-// the host's check, until the device control block reports F.  Returns with
-// the stream synchronised and *out holding the final control block.
+// the host's check, until the device control block reports a stop.  Returns
+// with the stream synchronised and *out holding the final control block.
 template <class Body>
 static int run_until_stop(tl_ctx *c, int kmax, long max_batches, Body &&body, Ctrl *out)
 {
@@ -281,10 +278,10 @@ static int run_until_stop(tl_ctx *c, int kmax, long max_batches, Body &&body, Ct
         if (k < kmax) k *= 2;
         if (b > 0) {
             HIPCHK(c, hipEventSynchronize(ev[(b - 1) & 1]));
-            if (snap[(b - 1) & 1].F || b > max_batches) {
+            if (snap[(b - 1) & 1].stop_iter != INT_MAX || b > max_batches) {
                 HIPCHK(c, hipStreamSynchronize(c->stream));
                 *out = snap[b & 1];
-                if (!out->F) {
+                if (out->stop_iter == INT_MAX) {
                     c->err = "solve did not stop within its iteration budget";
                     return TL_E_HIP;
                 }
@@ -431,7 +428,7 @@ static int finish_stats(tl_ctx *c, tl_solve_stats *st)
 {
     // true relative residual of the returned u (FP64)
     HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U0], c->f[TL_F_U], 2));
-    HIPCHK(c, launch_sumsq(c, c->f[TL_F_U0], FIN_NONE));
+    HIPCHK(c, launch_sumsq_slot(c, c->f[TL_F_U0], 0));
     Ctrl k;
     RETIF(read_ctrl(c, &k));
     st->rel_resid = k.sums[0] > 0 ? std::sqrt(k.sums[2] / k.sums[0]) : std::sqrt(k.sums[2]);
@@ -474,6 +471,14 @@ static void fill_stats(const Ctrl &k, tl_solve_stats *st)
     st->converged = k.why == WHY_CONVERGED;
 }
 
+static hipError_t cg64_iteration(tl_ctx *c, int t)
+{
+    hipError_t e = launch_cg_w64(c, t);
+    if (e == hipSuccess) e = launch_cg_ur64(c, t);
+    if (e == hipSuccess) e = launch_cg_p64(c, t);
+    return e;
+}
+
 // CG.solve!, CG.jl:7-29 (FP64)
 static int solve_cg64(tl_ctx *c, const tl_solve_params *p, double rx, double ry, tl_solve_stats *st)
 {
@@ -486,17 +491,11 @@ static int solve_cg64(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
     RETIF(choose_guess(c, p, rx, ry, &guess, st));
     HIPCHK(c, launch_cg_init_resid(c, guess));
     Ctrl k;
-    RETIF(run_until_stop(c, 64, (long)p->max_iters + 16,
-                         [&]() -> hipError_t {
-                             hipError_t e = launch_cg_w64(c);
-                             if (e == hipSuccess) e = launch_cg_ur64(c);
-                             if (e == hipSuccess) e = launch_cg_p64(c);
-                             return e;
-                         },
-                         &k));
+    int t = 0;
+    RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return cg64_iteration(c, ++t); }, &k));
     st->init_rel_resid = k.bb > 0 ? std::sqrt(k.sums[0] / k.bb) : 0.0;
-    st->iterations = st->cg_iters = k.iters;
-    c->last_cg_iters = k.iters;
+    st->iterations = st->cg_iters = k.stop_iter;
+    c->last_cg_iters = k.stop_iter;
     fill_stats(k, st);
     HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P, 1, nullptr, 0));
     return status_of(c, k);
@@ -511,18 +510,12 @@ static int cg_phase(tl_ctx *c, const tl_solve_params *p, double rx, double ry, C
     HIPCHK(c, launch_init_ctrl(c, MODE_SWITCH, p->max_iters, p->eps, p->rel_tol, 0.0, p->presteps,
                                p->error_switch, p->epslim));
     HIPCHK(c, launch_cg_init_resid(c, nullptr));
-    RETIF(run_until_stop(c, 32, (long)p->max_iters + 16,
-                         [&]() -> hipError_t {
-                             hipError_t e = launch_cg_w64(c);
-                             if (e == hipSuccess) e = launch_cg_ur64(c);
-                             if (e == hipSuccess) e = launch_cg_p64(c);
-                             return e;
-                         },
-                         k));
+    int t = 0;
+    RETIF(run_until_stop(c, 32, (long)p->max_iters + 16, [&]() { return cg64_iteration(c, ++t); }, k));
     st->init_rel_resid = k->bb > 0 ? std::sqrt(k->sums[0] / k->bb) : 0.0;
-    st->cg_iters = k->iters;
-    st->iterations = k->iters;
-    c->last_cg_iters = k->iters;
+    st->cg_iters = k->stop_iter;
+    st->iterations = k->stop_iter;
+    c->last_cg_iters = k->stop_iter;
     fill_stats(*k, st);
     return status_of(c, *k);
 }
@@ -547,7 +540,7 @@ static int solve_cheby(tl_ctx *c, const tl_solve_params *p, double rx, double ry
         HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P, 1, nullptr, 0));
         return TL_OK;
     }
-    const int cgiters = k.iters;
+    const int cgiters = k.stop_iter;
     const int maxiters = p->max_iters;
     double eigmin, eigmax, theta;
     RETIF(eigen_from_history(c, cgiters, &eigmin, &eigmax));
@@ -563,50 +556,45 @@ static int solve_cheby(tl_ctx *c, const tl_solve_params *p, double rx, double ry
             cb[i + 1] = tb[i];
         }
     }
+    HIPCHK(c, launch_set_mode(c, MODE_JACOBI, maxiters));
     double *ucur = c->f[TL_F_U], *unext = c->u_alt;
-    // Cheby.init! and the first main step (Cheby.jl:40-45)
+    // Cheby.init! and the first main step (Cheby.jl:40-45); iteration t = tt
+    int tt = cgiters + 1;
+    int chebyiters = 1;
     HIPCHK(c, launch_cheby_init(c, theta, ucur, unext));
     std::swap(ucur, unext);
-    HIPCHK(c, launch_set_mode(c, MODE_JACOBI /* |err| < eps */, maxiters));
-    HIPCHK(c, launch_cheby_iter2(c, ca[2], cb[2], ucur, unext, 1, 1));
+    HIPCHK(c, launch_cheby_iter(c, ca[2], cb[2], ucur, unext, 1, tt));
     std::swap(ucur, unext);
     RETIF(read_ctrl(c, &k));
-    int chebyiters = 1;
     const int estiter = tl_cheby_calciter(eigmin, eigmax, k.err, k.bb);
     st->est_iters = estiter;
-    int tt = cgiters + 1;
-    if (!(std::fabs(k.err) < p->eps)) {
-        // main loop (Cheby.jl:46-53); each launch swaps u buffers on the host
-        std::vector<double *> cur_after;  // u buffer after each chebyiter index
-        cur_after.reserve(64);
-        const int first = chebyiters;
-        long launched = first;
-        int done_at = -1;
-        while (tt < maxiters && done_at < 0) {
-            // one batch: up to the next norm iteration or 64 launches
+    if (k.stop_iter == INT_MAX) {
+        // main loop (Cheby.jl:46-53); every launch swaps the u buffers
+        while (tt < maxiters) {
             int nb = 0;
+            int norm = 0;
             while (tt < maxiters && nb < 64) {
                 ++tt;
                 ++chebyiters;
-                const int norm = (chebyiters >= estiter) && ((tt + 1) % 10 == 0);
-                HIPCHK(c, launch_cheby_iter2(c, ca[chebyiters + 1], cb[chebyiters + 1], ucur, unext, norm,
-                                             chebyiters));
+                norm = (chebyiters >= estiter) && ((tt + 1) % 10 == 0);
+                HIPCHK(c, launch_cheby_iter(c, ca[chebyiters + 1], cb[chebyiters + 1], ucur, unext, norm, tt));
                 std::swap(ucur, unext);
-                ++launched;
                 ++nb;
                 if (norm) break;
             }
-            Ctrl kk;
-            RETIF(read_ctrl(c, &kk));
-            if (kk.F) done_at = kk.iters;
+            if (norm) {
+                RETIF(read_ctrl(c, &k));
+                if (k.stop_iter != INT_MAX) break;
+            }
         }
-        if (done_at > 0 && ((chebyiters - done_at) & 1)) std::swap(ucur, unext);
-        if (done_at > 0) chebyiters = done_at;
-        tt = cgiters + chebyiters;
-        (void)launched;
+        RETIF(read_ctrl(c, &k));
+        if (k.stop_iter != INT_MAX && ((tt - k.stop_iter) & 1)) std::swap(ucur, unext);
+        if (k.stop_iter != INT_MAX) {
+            chebyiters -= tt - k.stop_iter;
+            tt = k.stop_iter;
+        }
     }
     if (ucur != c->f[TL_F_U]) std::swap(c->f[TL_F_U], c->u_alt);
-    RETIF(read_ctrl(c, &k));
     st->iterations = tt;
     st->cheby_iters = chebyiters;
     st->error = k.err;
@@ -624,7 +612,7 @@ static int solve_ppcg(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
         HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P, 1, nullptr, 0));
         return TL_OK;
     }
-    const int cgiters = k.iters;
+    const int cgiters = k.stop_iter;
     const int inner = p->ppcg_inner_steps;
     double eigmin, eigmax, theta;
     RETIF(eigen_from_history(c, cgiters, &eigmin, &eigmax));
@@ -633,26 +621,28 @@ static int solve_ppcg(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
     std::vector<double> ca(std::max(inner, 1)), cb(std::max(inner, 1));
     tl_cheby_coef(eigmin, eigmax, inner, ca.data(), cb.data(), &theta);
     // PPCG.init!(chunk, settings), PPCG.jl:58-61: r = u0 - A u
-    HIPCHK(c, launch_residual(c, 0, FIN_NONE));
-    HIPCHK(c, launch_set_mode(c, MODE_PPCG, p->max_iters - cgiters));
+    HIPCHK(c, launch_residual(c));
+    HIPCHK(c, launch_set_mode(c, MODE_PPCG, p->max_iters));
     double *sd0 = c->f[TL_F_SD], *sd1 = c->sd_alt;
+    int t = cgiters;
     RETIF(run_until_stop(c, 16, (long)p->max_iters + 16,
                          [&]() -> hipError_t {
-                             hipError_t e = launch_cg_w64(c);
-                             if (e == hipSuccess) e = launch_ppcg_ur_sd(c, theta);
+                             ++t;
+                             hipError_t e = launch_cg_w64(c, t);
+                             if (e == hipSuccess) e = launch_ppcg_ur_sd(c, theta, t);
                              double *a = sd0, *b = sd1;
                              for (int pp = 0; pp < inner && e == hipSuccess; ++pp) {
-                                 e = launch_ppcg_inner(c, ca[pp], cb[pp], a, b, pp == inner - 1);
+                                 e = launch_ppcg_inner(c, ca[pp], cb[pp], a, b, pp == inner - 1, t);
                                  std::swap(a, b);
                              }
-                             if (inner == 0 && e == hipSuccess) e = launch_sumsq(c, c->f[TL_F_R], FIN_PPCG);
-                             if (e == hipSuccess) e = launch_cg_p64(c);
+                             if (inner == 0 && e == hipSuccess) e = launch_sumsq_rr(c, c->f[TL_F_R], t);
+                             if (e == hipSuccess) e = launch_cg_p64(c, t);
                              return e;
                          },
                          &k));
     if (inner & 1) std::swap(c->f[TL_F_SD], c->sd_alt);
-    st->cheby_iters = k.iters;
-    st->iterations = cgiters + k.iters;
+    st->cheby_iters = k.stop_iter - cgiters;
+    st->iterations = k.stop_iter;
     fill_stats(k, st);
     HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P | TL_X_SD, 1, nullptr, 0));
     return status_of(c, k);
@@ -671,19 +661,27 @@ static int solve_jacobi(tl_ctx *c, const tl_solve_params *p, double rx, double r
                              if (tt >= p->max_iters) return hipSuccess;
                              ++tt;
                              const int every = (tt % 50) == 0;
-                             hipError_t e = launch_jacobi2(c, ucur, unext, tt, every);
-                             if (every && e == hipSuccess) e = launch_residual_u(c, unext, 1, FIN_JACOBI_ADD);
+                             hipError_t e = launch_jacobi(c, ucur, unext, tt, every);
+                             if (every && e == hipSuccess) e = launch_jacobi_residual(c, unext, tt);
                              std::swap(ucur, unext);
                              return e;
                          },
                          &k));
-    // tt launches were made; the last one that ran was k.iters
-    if ((tt - k.iters) & 1) std::swap(ucur, unext);
+    // tt launches were made; the last one that ran was k.stop_iter
+    if ((tt - k.stop_iter) & 1) std::swap(ucur, unext);
     if (ucur != c->f[TL_F_U]) std::swap(c->f[TL_F_U], c->u_alt);
-    st->iterations = k.iters;
+    st->iterations = k.stop_iter;
     fill_stats(k, st);
     HIPCHK(c, launch_halo(c, TL_X_U, 1, nullptr, 0));
     return status_of(c, k);
+}
+
+static hipError_t cg32_iteration(tl_ctx *c, int t)
+{
+    hipError_t e = launch_cg_w32(c, t);
+    if (e == hipSuccess) e = launch_cg_ur32(c, t);
+    if (e == hipSuccess) e = launch_cg_p32(c, t);
+    return e;
 }
 
 // FP32 inner CG with FP64 residual replacement ("reliable updates"): the
@@ -703,27 +701,22 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     // u0 = b = energy*density; u = guess or b; bb, tol_abs
     HIPCHK(c, launch_cg_init_resid(c, guess));
     HIPCHK(c, hipMemsetAsync(c->x32, 0, c->g.elems * sizeof(float), c->stream));
-    HIPCHK(c, launch_ir_residual(c, 1));
+    int t = 0;  // last iteration enqueued
+    HIPCHK(c, launch_ir_residual(c, 1, t + 1));
     Ctrl k;
     RETIF(read_ctrl(c, &k));
     st->init_rel_resid = k.bb > 0 ? std::sqrt(k.sums[0] / k.bb) : 0.0;
     int refreshes = 0;
-    while (!(k.F && k.why != WHY_REFRESH)) {
-        if (k.F && k.why == WHY_REFRESH) {
+    while (k.stop_iter == INT_MAX || k.why == WHY_REFRESH) {
+        if (k.stop_iter != INT_MAX && k.why == WHY_REFRESH) {
+            t = k.stop_iter;
             HIPCHK(c, launch_ir_accumulate(c));
-            HIPCHK(c, launch_ir_residual(c, 0));
+            HIPCHK(c, launch_ir_residual(c, 0, t + 1));
             ++refreshes;
             RETIF(read_ctrl(c, &k));
             continue;
         }
-        RETIF(run_until_stop(c, 64, (long)p->max_iters + 16,
-                             [&]() -> hipError_t {
-                                 hipError_t e = launch_cg_w32(c);
-                                 if (e == hipSuccess) e = launch_cg_ur32(c);
-                                 if (e == hipSuccess) e = launch_cg_p32(c);
-                                 return e;
-                             },
-                             &k));
+        RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return cg32_iteration(c, ++t); }, &k));
     }
     if (k.why == WHY_MAXITERS) HIPCHK(c, launch_ir_accumulate(c));  // keep the partial correction
     st->inner_iters = k.iters;
@@ -733,6 +726,87 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     fill_stats(k, st);
     HIPCHK(c, launch_halo(c, TL_X_U, 1, nullptr, 0));
     return status_of(c, k);
+}
+
+// collect the durations of the profiled launches recorded so far
+static int prof_harvest(tl_ctx *c)
+{
+    if (c->prof_used == 0) return TL_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i + 1 < c->prof_used; i += 2) {
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->prof_ev[i], c->prof_ev[i + 1]));
+        c->prof_ms[c->prof_tag[i / 2] & 7].push_back(ms);
+    }
+    c->prof_used = 0;
+    return TL_OK;
+}
+
+extern "C" int tl_profile_begin(tl_ctx *c, unsigned kernel_mask)
+{
+    if (!c || (kernel_mask >> (TL_K_CG_P32 + 1))) return TL_E_INVALID;
+    RETIF(prof_harvest(c));
+    c->prof_mask = kernel_mask;
+    for (auto &v : c->prof_ms) v.clear();
+    return TL_OK;
+}
+
+extern "C" int tl_profile_read(tl_ctx *c, int kernel_id, double *durations_ms, int max, int *count)
+{
+    if (!c || !count || kernel_id < 0 || kernel_id > TL_K_CG_P32) return TL_E_INVALID;
+    RETIF(prof_harvest(c));
+    const std::vector<double> &v = c->prof_ms[kernel_id];
+    *count = (int)v.size();
+    if (durations_ms)
+        for (int i = 0; i < *count && i < max; ++i) durations_ms[i] = v[i];
+    return TL_OK;
+}
+
+extern "C" int tl_profile_end(tl_ctx *c)
+{
+    if (!c) return TL_E_INVALID;
+    RETIF(prof_harvest(c));
+    c->prof_mask = 0;
+    for (auto &v : c->prof_ms) v.clear();
+    return TL_OK;
+}
+
+extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
+{
+    if (!c || reps < 1 || kernel_id < -1 || kernel_id > TL_K_CG_P32) return TL_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (kernel_id == -1) {
+        for (int i = 0; i < reps; ++i) HIPCHK(c, launch_copy_full(c, c->f[TL_F_U], c->u_alt));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return TL_OK;
+    }
+    // a live CG state whose stop test never fires: p = r != 0, eps = 0
+    RETIF(ensure_history(c, 1 << 20));
+    HIPCHK(c, launch_init_coef(c, 1.0, 1.0, 1, INIT_CG));
+    HIPCHK(c, launch_init_ctrl(c, MODE_CG_REF, 1 << 30, 0.0, 0.0, 0.1, 0, 0, 0.0));
+    HIPCHK(c, launch_cg_init_resid(c, nullptr));
+    if (kernel_id >= TL_K_CG_W32) {
+        RETIF(ensure_fp32(c));
+        HIPCHK(c, launch_ir_prepare(c));
+        HIPCHK(c, launch_ir_residual(c, 1, 1));
+    }
+    // one live iteration so the W and UR partials exist, then repeats of one
+    // kernel with a fixed iteration number (its inputs stay valid)
+    HIPCHK(c, kernel_id >= TL_K_CG_W32 ? cg32_iteration(c, 1) : cg64_iteration(c, 1));
+    for (int i = 0; i < reps; ++i) {
+        hipError_t e = hipSuccess;
+        switch (kernel_id) {
+        case TL_K_CG_W64: e = launch_cg_w64(c, 2); break;
+        case TL_K_CG_UR64: e = launch_cg_ur64(c, 2); break;
+        case TL_K_CG_P64: e = launch_cg_p64(c, 2); break;
+        case TL_K_CG_W32: e = launch_cg_w32(c, 2); break;
+        case TL_K_CG_UR32: e = launch_cg_ur32(c, 2); break;
+        default: e = launch_cg_p32(c, 2); break;
+        }
+        HIPCHK(c, e);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TL_OK;
 }
 
 extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double ry, tl_solve_stats *st)
@@ -773,6 +847,7 @@ extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double r
     const auto t1 = std::chrono::steady_clock::now();
     st->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     RETIF(finish_stats(c, st));
+    RETIF(prof_harvest(c));
     // training data for the learned warm start: (inputs of this step, solution)
     if (c->recording) RETIF(unet_record_step(c, rx, ry, p->coefficient));
     return TL_OK;
@@ -783,7 +858,7 @@ extern "C" int tl_solve_finished(tl_ctx *c, int check_result)
     if (!c) return TL_E_INVALID;
     HIPCHK(c, hipSetDevice(c->device));
     // solvefinished!, kernels.jl:166-170
-    if (check_result) HIPCHK(c, launch_residual(c, 0, FIN_NONE));
+    if (check_result) HIPCHK(c, launch_residual(c));
     HIPCHK(c, launch_finalise(c));
     HIPCHK(c, launch_halo(c, TL_X_ENERGY, 1, nullptr, 0));
     return TL_OK;

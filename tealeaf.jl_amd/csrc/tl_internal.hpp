@@ -16,6 +16,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/tealeaf_hip.h"
 
@@ -55,17 +56,14 @@ struct Ctrl {
     double eps, epslim;
     double rel_tol;     // MODE_CG_REL / MODE_IR: tol_abs = rel_tol * sqrt(bb)
     double ir_delta;    // MODE_IR: refresh_abs = ir_delta * sqrt(rr_true)
-    double sums[4];     // last multi-value reduction (field summary, residual norms)
-    int iters;          // completed iterations of the current phase
+    double sums[4];     // results of single-block finalizers (field summary, norms)
+    int iters;          // last completed iteration
     int maxiters;
     int mode;
     int presteps, errorswitch;
-    int F;              // stop: set by the deciding reduction
-    int G;              // set by the first kernel of an iteration that saw F
+    int stop_iter;      // iterations t > stop_iter do not run (INT_MAX while running)
     int why;
     int status;
-    unsigned ticket;
-    int pad[2];
 };
 
 struct Unet;  // learned warm start (tl_unet.hip)
@@ -94,6 +92,12 @@ struct tl_ctx {
     tl_state *states_d = nullptr;
     std::string err;
     hipEvent_t ev[2] = {nullptr, nullptr};   // batch pipeline
+    // per-launch event timing of selected kernels (tl_profile_begin)
+    unsigned prof_mask = 0;
+    std::vector<hipEvent_t> prof_ev;          // pairs, reused
+    std::vector<int> prof_tag;                // kernel id of each recorded pair
+    size_t prof_used = 0;                     // events recorded since the last harvest
+    std::vector<double> prof_ms[8];
     tl::Unet *unet = nullptr;
     bool recording = false;
 };
@@ -101,41 +105,42 @@ struct tl_ctx {
 namespace tl {
 
 // ------------------------------------------------------------ launchers
-// All launchers enqueue on ctx->stream and return hipError_t.
-// what k_init_coef writes besides kx, ky
-enum { INIT_CG = 0, INIT_JACOBI = 1, INIT_COEF_ONLY = 2 };
-hipError_t launch_init_coef(tl_ctx *c, double rx, double ry, int coefficient, int what);
-hipError_t launch_set_state(tl_ctx *c, const tl_state *states_d, int nstates);
-hipError_t launch_u_from_energy(tl_ctx *c);
-hipError_t launch_copy_energy(tl_ctx *c);
-hipError_t launch_halo(tl_ctx *c, unsigned mask, int depth, double *const *extra, int nextra);
-hipError_t launch_cg_init_resid(tl_ctx *c, const double *guess);
-hipError_t launch_resid_norm(tl_ctx *c, const double *x, double *out_dev_sum);
-hipError_t launch_cg_w64(tl_ctx *c);
-hipError_t launch_cg_ur64(tl_ctx *c);
-hipError_t launch_cg_p64(tl_ctx *c);
-hipError_t launch_cg_w32(tl_ctx *c);
-hipError_t launch_cg_ur32(tl_ctx *c);
-hipError_t launch_cg_p32(tl_ctx *c);
-hipError_t launch_residual(tl_ctx *c, int with_norm, int finalize_kind);
-hipError_t launch_finalise(tl_ctx *c);
-hipError_t launch_field_summary(tl_ctx *c);
-hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double *u_out);
-hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm);
-hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta);
-hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last);
-hipError_t launch_jacobi_iter(tl_ctx *c, const double *u_in, double *u_out, int defer);
-hipError_t launch_ir_prepare(tl_ctx *c);
-hipError_t launch_ir_accumulate(tl_ctx *c);
-hipError_t launch_ir_residual(tl_ctx *c, int reset_p);
-hipError_t launch_apply(tl_ctx *c, const double *src, double *dst);
+// All launchers enqueue on ctx->stream and return hipError_t.  Iteration
+// kernels take the 1-based iteration number t; they do nothing once
+// t > Ctrl::stop_iter.
 hipError_t launch_init_ctrl(tl_ctx *c, int mode, int maxiters, double eps, double rel_tol, double ir_delta,
                             int presteps, int errorswitch, double epslim);
 hipError_t launch_set_mode(tl_ctx *c, int mode, int maxiters);
-hipError_t launch_sumsq(tl_ctx *c, const double *v, int finalize_kind);
-
-// finalizer kinds of launch_residual
-enum { FIN_NONE = 0, FIN_JACOBI_ADD = 1, FIN_IR = 2, FIN_SUMS = 3, FIN_PPCG = 4 };
+hipError_t launch_set_state(tl_ctx *c, const tl_state *states_d, int nstates);
+hipError_t launch_copy_full(tl_ctx *c, const double *a, double *b);
+hipError_t launch_copy_energy(tl_ctx *c);
+// what k_init_coef writes besides kx, ky
+enum { INIT_CG = 0, INIT_JACOBI = 1, INIT_COEF_ONLY = 2 };
+hipError_t launch_init_coef(tl_ctx *c, double rx, double ry, int coefficient, int what);
+hipError_t launch_halo(tl_ctx *c, unsigned mask, int depth, double *const *extra, int nextra);
+hipError_t launch_cg_init_resid(tl_ctx *c, const double *guess);
+hipError_t launch_resid_norm_slot(tl_ctx *c, const double *b, const double *x, int slot);
+hipError_t launch_cg_w64(tl_ctx *c, int t);
+hipError_t launch_cg_ur64(tl_ctx *c, int t);
+hipError_t launch_cg_p64(tl_ctx *c, int t);
+hipError_t launch_cg_w32(tl_ctx *c, int t);
+hipError_t launch_cg_ur32(tl_ctx *c, int t);
+hipError_t launch_cg_p32(tl_ctx *c, int t);
+hipError_t launch_residual(tl_ctx *c);
+hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t);
+hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next);
+hipError_t launch_apply(tl_ctx *c, const double *src, double *dst);
+hipError_t launch_finalise(tl_ctx *c);
+hipError_t launch_field_summary(tl_ctx *c);
+hipError_t launch_sumsq_slot(tl_ctx *c, const double *v, int slot);
+hipError_t launch_sumsq_rr(tl_ctx *c, const double *v, int t);
+hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double *u_out);
+hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t);
+hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t);
+hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t);
+hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer);
+hipError_t launch_ir_prepare(tl_ctx *c);
+hipError_t launch_ir_accumulate(tl_ctx *c);
 
 Geo make_geo(int nx, int ny, int hd);
 

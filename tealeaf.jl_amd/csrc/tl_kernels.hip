@@ -1,19 +1,29 @@
 // tl_kernels.hip -- gfx950 kernels of the TeaLeaf linear-solve hot path.
 //
-// All stencil kernels share one launch shape: a block of 256 threads owns a
-// 256-cell row segment and sweeps ty consecutive rows, keeping the cells
-// above and below in registers (a vertical sliding window) so every field is
+// Launch shape.  A block of 256 threads owns a 256-cell row segment and
+// sweeps ty consecutive rows of it; each thread owns one column.  Rows are
+// processed in chunks of R = 8 whose loads are all issued before any
+// arithmetic (R rows of loads in flight per wave), and the cells above and
+// below are kept in registers (a vertical sliding window), so each field is
 // fetched from HBM once per sweep.  Left/right neighbours are re-read through
-// L1/L2.  The depth-1 reflective halo of kernels.jl:191-210 (with the index
+// L1/L2.
+//
+// Halo.  The depth-1 reflective halo of kernels.jl:191-210 (with the index
 // correction of the intended algorithm) is applied by clamping neighbour
 // indices to the interior: a reflected ghost cell holds exactly the value of
-// the interior cell it mirrors, so the arithmetic is bit-identical to
-// reading an updated halo, and no halo exchange is needed inside the solve.
+// the interior cell it mirrors, so the arithmetic is bit-identical to reading
+// an updated halo and no halo exchange is needed inside a solve.
 //
-// Reductions are single-pass and deterministic: wave64 butterfly sums, a
-// fixed-order block sum, one partial per block, and the last block (atomic
-// ticket) sums the partials in block order and runs the scalar update of the
-// solver (alpha, beta, stop test) on the device.
+// Reductions need no atomics and no fences.  A producing kernel writes one
+// partial sum per block (wave64 butterfly, then a fixed-order block sum).
+// The consuming kernel -- the next one on the stream, so the kernel boundary
+// makes the partials visible -- has every block sum the ~2k partials in the
+// same fixed order, so all blocks derive the identical scalar (alpha, beta,
+// the stop test) and results are bit-reproducible run to run.  Block 0 also
+// records the scalars in the device control block for the host.  A stopped
+// solve is expressed by Ctrl::stop_iter: launches for later iterations exit
+// at once.
+#include <climits>
 #include <cmath>
 
 #include "tl_internal.hpp"
@@ -32,10 +42,13 @@ Geo make_geo(int nx, int ny, int hd)
     g.off = (long)hd * g.pitch + XCOL;
     g.elems = (size_t)g.pitch * (size_t)g.Y + 64;
     g.gx = (nx + TB - 1) / TB;
-    long rows = (long)ny * g.gx;
-    int ty = (int)((rows + 4095) / 4096);
-    if (ty < 2) ty = 2;
-    if (ty > 128) ty = 128;
+    // about 2048 blocks on large grids (8 per CU): enough waves in flight,
+    // few enough partials for every consumer block to sum them
+    const long rows = (long)ny * g.gx;
+    int ty = (int)((rows + 2047) / 2048);
+    ty = (ty + 7) / 8 * 8;  // a multiple of the row chunk
+    if (ty < 8) ty = 8;
+    if (ty > 1024) ty = 1024;
     g.ty = ty;
     g.gy = (ny + ty - 1) / ty;
     return g;
@@ -50,17 +63,14 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Sums v[] over the whole grid.  Returns true in the last block to finish,
-// where every thread then holds the grid totals in tot[].
+__device__ __forceinline__ unsigned block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
+
+// one partial per block: part[a * stride + block] = sum over the block of v[a]
 template <int N>
-__device__ __forceinline__ bool grid_reduce(const double (&v)[N], double *__restrict__ part, unsigned *ticket,
-                                            double (&tot)[N])
+__device__ __forceinline__ void write_partials(const double (&v)[N], double *__restrict__ part, size_t stride)
 {
     __shared__ double sh[N][TB / 64];
-    __shared__ int am_last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const unsigned nb = gridDim.x * gridDim.y;
-    const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         const double s = wave_sum(v[a]);
@@ -68,26 +78,38 @@ __device__ __forceinline__ bool grid_reduce(const double (&v)[N], double *__rest
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        const unsigned b = block_id();
 #pragma unroll
         for (int a = 0; a < N; ++a) {
             double s = sh[a][0];
 #pragma unroll
             for (int q = 1; q < TB / 64; ++q) s += sh[a][q];
-            part[(size_t)a * nb + bid] = s;
+            part[(size_t)a * stride + b] = s;
         }
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        am_last = (t == nb - 1);
     }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
+}
+
+// every thread of the block gets the fixed-order sum of nb partials
+template <int N>
+__device__ __forceinline__ void sum_partials(const double *__restrict__ part, unsigned nb, size_t stride,
+                                             double (&tot)[N])
+{
+    __shared__ double sh[N][TB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         double s = 0.0;
-        for (unsigned q = threadIdx.x; q < nb; q += TB) s += part[(size_t)a * nb + q];
+        for (unsigned q0 = threadIdx.x; q0 < nb; q0 += 8 * TB) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const unsigned q = q0 + (unsigned)k * TB;
+                v[k] = q < nb ? part[(size_t)a * stride + q] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s += v[k];
+        }
         s = wave_sum(s);
-        __syncthreads();
         if (lane == 0) sh[a][wid] = s;
     }
     __syncthreads();
@@ -98,9 +120,10 @@ __device__ __forceinline__ bool grid_reduce(const double (&v)[N], double *__rest
         for (int q = 1; q < TB / 64; ++q) s += sh[a][q];
         tot[a] = s;
     }
-    if (threadIdx.x == 0) *ticket = 0u;
-    return true;
+    __syncthreads();  // sh may be reused by the next call
 }
+
+__device__ __forceinline__ unsigned nblocks(const Geo &g) { return (unsigned)(g.gx * g.gy); }
 
 // ------------------------------------------------------------- operator
 
@@ -115,9 +138,14 @@ __device__ __forceinline__ T smvp5(T c, T l, T r, T d, T u, T kxl, T kxr, T kyd,
     return (consum * c - (kxr * r + kxl * l)) - (kyu * u + kyd * d);
 }
 
-// Iterates the rows of this thread's column: calls body(i, y) with i the
-// element index of interior cell (x, y).  xl/xr are the clamped horizontal
-// neighbour offsets.
+// One thread owns one column of a tile: rows [y0, y1) of column x.  Rows are
+// processed in chunks of R whose loads are all issued before any arithmetic,
+// so every wave keeps R rows of loads in flight.  Row indices are clamped
+// into [0, ny-1]: the clamped read of row -1 or ny IS the reflective ghost
+// value, and rows past the tile end are loaded (valid addresses) but never
+// stored.
+constexpr int R = 8;
+
 struct Col {
     int x, y0, y1;
     long xl, xr;
@@ -136,41 +164,85 @@ __device__ __forceinline__ Col column(const Geo &g)
     return c;
 }
 
-// Vertical sliding window over field a: cur/dn/up values with the
-// reflective clamp at y = 0 and y = ny-1.
+// element index of interior cell (x, y)
+__device__ __forceinline__ long at(const Geo &g, int x, int y) { return g.off + (long)y * g.pitch + x; }
+// same with y clamped into the interior
+__device__ __forceinline__ long atc(const Geo &g, int x, int y) { return at(g, x, min(y, g.ny - 1)); }
+
+// stencil inputs of R rows of field a: the row above, left, right, and the
+// face coefficients kx(x), kx(x+1), ky(y+1); ky(y) is carried from the row below
 template <typename T>
-struct Window {
-    T dn, cur;
-    __device__ __forceinline__ void start(const T *__restrict__ a, long i, int y0, long pitch)
-    {
-        cur = a[i];
-        dn = y0 > 0 ? a[i - pitch] : cur;
+struct Stn {
+    T up[R], lf[R], rt[R], kxl[R], kxr[R], kyu[R];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_stn(Stn<T> &s, const Geo &g, const Col &c, int yb, const T *__restrict__ a,
+                                         const T *__restrict__ kx, const T *__restrict__ ky)
+{
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int y = yb + q;
+        const long ic = atc(g, c.x, y);
+        s.up[q] = a[atc(g, c.x, y + 1)];
+        s.lf[q] = a[ic + c.xl];
+        s.rt[q] = a[ic + c.xr];
+        s.kxl[q] = kx[ic];
+        s.kxr[q] = kx[ic + 1];
+        s.kyu[q] = ky[at(g, c.x, min(y + 1, g.ny))];
     }
-    __device__ __forceinline__ T up(const T *__restrict__ a, long i, int y, int ny, long pitch) const
+}
+
+// vertical window: value at the current row and the row below, ky below
+template <typename T>
+struct Win {
+    T dn, cur, kyd;
+    __device__ __forceinline__ void start(const Geo &g, const Col &c, const T *__restrict__ a,
+                                          const T *__restrict__ ky)
     {
-        return y + 1 < ny ? a[i + pitch] : cur;
+        cur = a[at(g, c.x, c.y0)];
+        dn = a[at(g, c.x, max(c.y0 - 1, 0))];
+        kyd = ky[at(g, c.x, c.y0)];
+    }
+    __device__ __forceinline__ T apply(const Stn<T> &s, int q) const
+    {
+        return smvp5(cur, s.lf[q], s.rt[q], dn, s.up[q], s.kxl[q], s.kxr[q], kyd, s.kyu[q]);
+    }
+    __device__ __forceinline__ void advance(const Stn<T> &s, int q)
+    {
+        dn = cur;
+        cur = s.up[q];
+        kyd = s.kyu[q];
     }
 };
 
+// R values of field a at rows yb..yb+R-1 (clamped)
+template <typename T>
+__device__ __forceinline__ void load_rows(T (&v)[R], const Geo &g, const Col &c, int yb, const T *__restrict__ a)
+{
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = a[atc(g, c.x, yb + q)];
+}
+
 // ------------------------------------------------------- scalar updates
 
-__device__ void decide(Ctrl *c, double rr)
+// stop test after iteration t with squared residual rr (written by block 0)
+__device__ int decide_why(const Ctrl *c, int t, double rr)
 {
-    const int it = c->iters;
     const double nr = sqrt(fabs(rr));
     int why = WHY_NONE;
     switch (c->mode) {
     case MODE_CG_REF:
-        if (nr < c->eps) why = WHY_CONVERGED;
+        if (nr < c->eps) why = WHY_CONVERGED;  // CG.jl:24
         break;
     case MODE_CG_REL:
         if (nr <= c->tol_abs) why = WHY_CONVERGED;
         break;
     case MODE_SWITCH: {
-        if (fabs(rr) < c->eps) {
+        if (fabs(rr) < c->eps) {  // Cheby.jl:57, PPCG.jl:50
             why = WHY_CONVERGED;
         } else {
-            const int tt = it + 1;
+            const int tt = t + 1;  // the switch test at the top of the next iteration
             const bool sw = c->errorswitch ? (rr < c->epslim && tt > 20) : (tt > c->presteps && rr < 1.0);
             if (sw) why = WHY_SWITCH;
         }
@@ -184,15 +256,17 @@ __device__ void decide(Ctrl *c, double rr)
         if (fabs(rr) < c->eps) why = WHY_CONVERGED;
         break;
     }
-    if (why == WHY_NONE && it >= c->maxiters) why = WHY_MAXITERS;
-    if (!isfinite(rr)) {
-        why = WHY_BREAKDOWN;
-        c->status = TL_E_BREAKDOWN;
-    }
-    if (why != WHY_NONE) {
-        c->why = why;
-        c->F = 1;
-    }
+    if (why == WHY_NONE && t >= c->maxiters) why = WHY_MAXITERS;
+    if (!isfinite(rr)) why = WHY_BREAKDOWN;
+    return why;
+}
+
+__device__ void record_stop(Ctrl *c, int t, int why)
+{
+    if (why == WHY_NONE) return;
+    c->why = why;
+    if (why == WHY_BREAKDOWN) c->status = TL_E_BREAKDOWN;
+    c->stop_iter = t;
 }
 
 // -------------------------------------------------------- init kernels
@@ -203,6 +277,7 @@ __global__ __launch_bounds__(TB) void k_init_ctrl(Ctrl *c, int mode, int maxiter
     if (threadIdx.x != 0) return;
     c->rro = c->pw = c->alpha = c->beta = 0.0;
     c->err = 1e10; /* ERROR_START, kernels.jl:7 */
+    c->bb = 0.0;
     c->eps = eps;
     c->rel_tol = rel_tol;
     c->ir_delta = ir_delta;
@@ -214,23 +289,96 @@ __global__ __launch_bounds__(TB) void k_init_ctrl(Ctrl *c, int mode, int maxiter
     c->mode = mode;
     c->presteps = presteps;
     c->errorswitch = errorswitch;
-    c->F = 0;
-    c->G = 0;
+    c->stop_iter = INT_MAX;
     c->why = WHY_NONE;
     c->status = 0;
-    c->ticket = 0u;
 }
 
-// switches the stop rule between phases; clears the stop flags
+// switches the stop rule between phases and lets iterations run again
 __global__ __launch_bounds__(TB) void k_set_mode(Ctrl *c, int mode, int maxiters)
 {
     if (threadIdx.x != 0) return;
     c->mode = mode;
     c->maxiters = maxiters;
-    c->iters = 0;
-    c->F = 0;
-    c->G = 0;
+    c->stop_iter = INT_MAX;
     c->why = WHY_NONE;
+}
+
+// Single-block finalizers: sum n partial arrays (each nb long, stride nb)
+// into ctrl->sums[slot ...] and act on them.
+enum FinKind : int { FK_SUMS = 0, FK_INIT = 1, FK_BB = 2, FK_CHEBY = 3, FK_JACOBI = 4, FK_IR = 5 };
+
+__global__ __launch_bounds__(TB) void k_fin(Ctrl *__restrict__ c, const double *__restrict__ part, unsigned nb,
+                                            int n, int slot, int kind, int t)
+{
+    if ((kind == FK_CHEBY || kind == FK_JACOBI) && t > c->stop_iter) return;
+    double tot[2] = {0.0, 0.0};
+    if (n >= 2) {
+        sum_partials<2>(part, nb, nb, tot);
+    } else {
+        double t1[1];
+        sum_partials<1>(part, nb, nb, t1);
+        tot[0] = t1[0];
+    }
+    double t34[2] = {0.0, 0.0};
+    if (n == 4) sum_partials<2>(part + 2 * (size_t)nb, nb, nb, t34);
+    if (threadIdx.x != 0) return;
+    const double v[4] = {tot[0], tot[1], t34[0], t34[1]};
+    for (int a = 0; a < n && slot + a < 4; ++a) c->sums[slot + a] = v[a];
+    switch (kind) {
+    case FK_INIT:  // after CG.init!: rro = sum r^2, bb = sum u0^2
+        c->rro = v[0];
+        c->err = v[0];
+        c->bb = v[1];
+        c->tol_abs = c->rel_tol * sqrt(v[1]);
+        c->refresh_abs = c->ir_delta * sqrt(v[0]);
+        if (v[0] == 0.0) {  // exact solution already: nothing to iterate
+            c->why = WHY_CONVERGED;
+            c->stop_iter = 0;
+        }
+        break;
+    case FK_BB:
+        c->bb = v[0];
+        break;
+    case FK_CHEBY:  // calc2norm iteration t of Cheby.mainstep! (Cheby.jl:102-104, 57)
+        c->err = v[0];
+        c->iters = t;
+        if (fabs(v[0]) < c->eps) {
+            c->why = WHY_CONVERGED;
+            c->stop_iter = t;
+        }
+        break;
+    case FK_JACOBI: {  // Jacobi.jl:15-26: err = sum |u'-u| (+ sum r^2 every 50)
+        const double err = n >= 2 ? v[0] + v[1] : v[0];
+        c->err = err;
+        c->iters = t;
+        record_stop(c, t, decide_why(c, t, err));
+        break;
+    }
+    case FK_IR: {  // FP64 refresh of the mixed-precision CG: converged or resume
+        const double rr = v[0];
+        c->rro = rr;
+        c->err = rr;
+        c->refresh_abs = c->ir_delta * sqrt(rr);
+        if (!isfinite(rr)) {
+            c->why = WHY_BREAKDOWN;
+            c->status = TL_E_BREAKDOWN;
+            c->stop_iter = c->iters;
+        } else if (sqrt(rr) <= c->tol_abs) {
+            c->why = WHY_CONVERGED;
+            c->stop_iter = c->iters;
+        } else if (c->iters >= c->maxiters) {
+            c->why = WHY_MAXITERS;
+            c->stop_iter = c->iters;
+        } else {
+            c->why = WHY_NONE;
+            c->stop_iter = INT_MAX;
+        }
+        break;
+    }
+    default:
+        break;
+    }
 }
 
 // setchunkstate!, chunk.jl:122-151: rasterise the states in order over every
@@ -311,239 +459,250 @@ __global__ __launch_bounds__(TB) void k_init_coef(Geo g, const double *__restric
     }
 }
 
+// ------------------------------------------------------------ CG family
+
 // CG.init! lines 70-78 + copyu! (kernels.jl:217-220), with an optional
 // initial guess: b = u (energy*density); x = guess or b;
-// w = A x; r = b - w; p = r; u0 = b; u = x.   rro = sum r^2, bb = sum b^2.
+// w = A x; r = b - w; p = r; u0 = b; u = x.  Partials of sum r^2 go to
+// pr0 (the "previous rr" of iteration 1) and pm[0]; sum b^2 to pm[1].
 __global__ __launch_bounds__(TB) void k_cg_init_resid(Geo g, double *__restrict__ u,
                                                       const double *__restrict__ guess, double *__restrict__ u0,
                                                       double *__restrict__ p, double *__restrict__ r,
                                                       double *__restrict__ w, const double *__restrict__ kx,
-                                                      const double *__restrict__ ky, Ctrl *__restrict__ ctrl,
-                                                      double *__restrict__ part)
+                                                      const double *__restrict__ ky, double *__restrict__ pr0,
+                                                      double *__restrict__ pm)
 {
     const Col c = column(g);
     double acc[2] = {0.0, 0.0};
     const double *xs = guess ? guess : u;
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        Window<double> win;
-        win.start(xs, i, c.y0, g.pitch);
-        double kyd = ky[i];
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const double xu = win.up(xs, i, y, g.ny, g.pitch);
-            const double kyu = ky[i + g.pitch];
-            const double b = u[i];
-            const double wv = smvp5(win.cur, xs[i + c.xl], xs[i + c.xr], win.dn, xu, kx[i], kx[i + 1], kyd, kyu);
-            const double rv = b - wv;
-            w[i] = wv;
-            r[i] = rv;
-            p[i] = rv;
-            u0[i] = b;
-            if (guess) u[i] = win.cur;
-            acc[0] += rv * rv;
-            acc[1] += b * b;
-            win.dn = win.cur;
-            win.cur = xu;
-            kyd = kyu;
+        Win<double> win;
+        win.start(g, c, xs, ky);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            Stn<double> st;
+            double b[R];
+            load_stn(st, g, c, yb, xs, kx, ky);
+            load_rows(b, g, c, yb, u);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const double wv = win.apply(st, q);
+                const double rv = b[q] - wv;
+                if (yb + q < c.y1) {
+                    const long i = at(g, c.x, yb + q);
+                    w[i] = wv;
+                    r[i] = rv;
+                    p[i] = rv;
+                    u0[i] = b[q];
+                    if (guess) u[i] = win.cur;
+                    acc[0] += rv * rv;
+                    acc[1] += b[q] * b[q];
+                }
+                win.advance(st, q);
+            }
         }
     }
-    double tot[2];
-    if (grid_reduce<2>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) {
-        ctrl->rro = tot[0];
-        ctrl->err = tot[0];
-        ctrl->bb = tot[1];
-        ctrl->tol_abs = ctrl->rel_tol * sqrt(tot[1]);
-        ctrl->refresh_abs = ctrl->ir_delta * sqrt(tot[0]);
-        ctrl->sums[0] = tot[0];
-        ctrl->sums[1] = tot[1];
-        if (tot[0] == 0.0) {  // exact solution already: nothing to iterate
-            ctrl->F = 1;
-            ctrl->G = 1;
-            ctrl->why = WHY_CONVERGED;
-        }
-    }
+    const unsigned nb = nblocks(g);
+    write_partials<2>(acc, pm, nb);
+    if (threadIdx.x == 0) pr0[block_id()] = pm[block_id()];
 }
 
-// sum over the interior of (b - A x)^2 with b = u, into ctrl->sums[slot]
-__global__ __launch_bounds__(TB) void k_resid_norm(Geo g, const double *__restrict__ b,
-                                                   const double *__restrict__ xs, const double *__restrict__ kx,
-                                                   const double *__restrict__ ky, Ctrl *__restrict__ ctrl,
-                                                   double *__restrict__ part, int slot)
-{
-    const Col c = column(g);
-    double acc[1] = {0.0};
-    if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        Window<double> win;
-        win.start(xs, i, c.y0, g.pitch);
-        double kyd = ky[i];
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const double xu = win.up(xs, i, y, g.ny, g.pitch);
-            const double kyu = ky[i + g.pitch];
-            const double rv = b[i] - smvp5(win.cur, xs[i + c.xl], xs[i + c.xr], win.dn, xu, kx[i], kx[i + 1], kyd, kyu);
-            acc[0] += rv * rv;
-            win.dn = win.cur;
-            win.cur = xu;
-            kyd = kyu;
-        }
-    }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) ctrl->sums[slot] = tot[0];
-}
-
-// ------------------------------------------------------------ CG kernels
-
-// CG.w!, CG.jl:82-90: w = A p; pw = sum p*w; then alpha = rro/pw (CG.jl:35-36)
+// CG.w!, CG.jl:82-90: w = A p; partial sums of p*w
 template <typename T>
-__global__ __launch_bounds__(TB) void k_cg_w(Geo g, const T *__restrict__ p, const T *__restrict__ kx,
-                                             const T *__restrict__ ky, T *__restrict__ w, Ctrl *__restrict__ ctrl,
-                                             double *__restrict__ part, double *__restrict__ hist_a)
+__global__ __launch_bounds__(TB) void k_cg_w(Geo g, int t, const T *__restrict__ p, const T *__restrict__ kx,
+                                             const T *__restrict__ ky, T *__restrict__ w,
+                                             const Ctrl *__restrict__ ctrl, double *__restrict__ pw)
 {
-    if (ctrl->F) {
-        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) ctrl->G = 1;
-        return;
-    }
+    if (t > ctrl->stop_iter) return;
     const Col c = column(g);
     double acc[1] = {0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        Window<T> win;
-        win.start(p, i, c.y0, g.pitch);
-        T kyd = ky[i];
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const T pu = win.up(p, i, y, g.ny, g.pitch);
-            const T kyu = ky[i + g.pitch];
-            const T wv = smvp5(win.cur, p[i + c.xl], p[i + c.xr], win.dn, pu, kx[i], kx[i + 1], kyd, kyu);
-            w[i] = wv;
-            acc[0] += (double)wv * (double)win.cur;
-            win.dn = win.cur;
-            win.cur = pu;
-            kyd = kyu;
+        Win<T> win;
+        win.start(g, c, p, ky);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            Stn<T> st;
+            load_stn(st, g, c, yb, p, kx, ky);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const T wv = win.apply(st, q);
+                if (yb + q < c.y1) {
+                    w[at(g, c.x, yb + q)] = wv;
+                    acc[0] += (double)wv * (double)win.cur;
+                }
+                win.advance(st, q);
+            }
         }
     }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) {
-        const double pw = tot[0];
-        const double alpha = ctrl->rro / pw;
-        ctrl->pw = pw;
-        ctrl->alpha = alpha;
-        if (hist_a) hist_a[ctrl->iters] = alpha;
-        if (!(pw > 0.0) || !isfinite(alpha)) {
+    write_partials<1>(acc, pw, nblocks(g));
+}
+
+// alpha = rro / pw (CG.jl:35), derived identically by every block
+__device__ __forceinline__ bool cg_alpha(const Geo &g, int t, Ctrl *__restrict__ ctrl,
+                                         const double *__restrict__ pw, const double *__restrict__ prprev,
+                                         double *__restrict__ hist_a, double *alpha)
+{
+    double a1[1], a2[1];
+    sum_partials<1>(pw, nblocks(g), nblocks(g), a1);
+    sum_partials<1>(prprev, nblocks(g), nblocks(g), a2);
+    const double pwv = a1[0], rro = a2[0];
+    const double a = rro / pwv;
+    const bool ok = pwv > 0.0 && isfinite(a);
+    if (block_id() == 0 && threadIdx.x == 0) {
+        ctrl->pw = pwv;
+        ctrl->alpha = a;
+        if (hist_a) hist_a[t - 1] = a;
+        if (!ok) {  // non-positive curvature: stop before this iteration's update
             ctrl->status = TL_E_BREAKDOWN;
             ctrl->why = WHY_BREAKDOWN;
-            ctrl->F = 1;
-            ctrl->G = 1;
+            ctrl->stop_iter = t - 1;
         }
     }
+    *alpha = a;
+    return ok;
 }
 
-// CG.ur!, CG.jl:93-98: u += alpha p; r -= alpha w; rr = sum r^2; then
-// beta = rrn/rro (CG.jl:39-40) and the stop test
+// CG.ur!, CG.jl:93-98: u += alpha p; r -= alpha w; partial sums of r^2
 template <typename T>
-__global__ __launch_bounds__(TB) void k_cg_ur(Geo g, T *__restrict__ u, const T *__restrict__ p,
+__global__ __launch_bounds__(TB) void k_cg_ur(Geo g, int t, T *__restrict__ u, const T *__restrict__ p,
                                               T *__restrict__ r, const T *__restrict__ w, Ctrl *__restrict__ ctrl,
-                                              double *__restrict__ part, double *__restrict__ hist_b)
+                                              const double *__restrict__ pw, const double *__restrict__ prprev,
+                                              double *__restrict__ prcur, double *__restrict__ hist_a)
 {
-    if (ctrl->F) return;
-    const T alpha = (T)ctrl->alpha;
+    if (t > ctrl->stop_iter) return;
+    double ad;
+    if (!cg_alpha(g, t, ctrl, pw, prprev, hist_a, &ad)) return;
+    const T alpha = (T)ad;
     const Col c = column(g);
     double acc[1] = {0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            u[i] = u[i] + alpha * p[i];
-            const T rv = r[i] - alpha * w[i];
-            r[i] = rv;
-            acc[0] += (double)rv * (double)rv;
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            T uv[R], pv[R], rv[R], wv[R];
+            load_rows(uv, g, c, yb, u);
+            load_rows(pv, g, c, yb, p);
+            load_rows(rv, g, c, yb, r);
+            load_rows(wv, g, c, yb, w);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (yb + q < c.y1) {
+                    const long i = at(g, c.x, yb + q);
+                    u[i] = uv[q] + alpha * pv[q];
+                    const T rn = rv[q] - alpha * wv[q];
+                    r[i] = rn;
+                    acc[0] += (double)rn * (double)rn;
+                }
+            }
         }
     }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) {
-        const double rrn = tot[0];
-        const double beta = rrn / ctrl->rro;
+    write_partials<1>(acc, prcur, nblocks(g));
+}
+
+// beta = rrn / rro (CG.jl:39) and the stop test, derived identically by
+// every block; block 0 records them
+__device__ __forceinline__ double cg_beta(const Geo &g, int t, Ctrl *__restrict__ ctrl,
+                                          const double *__restrict__ prprev, const double *__restrict__ prcur,
+                                          double *__restrict__ hist_b)
+{
+    double a1[1], a2[1];
+    sum_partials<1>(prcur, nblocks(g), nblocks(g), a1);
+    sum_partials<1>(prprev, nblocks(g), nblocks(g), a2);
+    const double rrn = a1[0], rro = a2[0];
+    const double beta = rrn / rro;
+    if (block_id() == 0 && threadIdx.x == 0) {
         ctrl->beta = beta;
-        if (hist_b) hist_b[ctrl->iters] = beta;
         ctrl->rro = rrn;
         ctrl->err = rrn;
-        ctrl->iters += 1;
-        decide(ctrl, rrn);
+        ctrl->iters = t;
+        if (hist_b) hist_b[t - 1] = beta;
+        record_stop(ctrl, t, decide_why(ctrl, t, rrn));
+    }
+    return beta;
+}
+
+// CG.p!, CG.jl:101-104: p = beta p + r (runs in the iteration that stops)
+template <typename T>
+__global__ __launch_bounds__(TB) void k_cg_p(Geo g, int t, T *__restrict__ p, const T *__restrict__ r,
+                                             Ctrl *__restrict__ ctrl, const double *__restrict__ prprev,
+                                             const double *__restrict__ prcur, double *__restrict__ hist_b)
+{
+    if (t > ctrl->stop_iter) return;
+    const T beta = (T)cg_beta(g, t, ctrl, prprev, prcur, hist_b);
+    const Col c = column(g);
+    if (!c.active) return;
+    for (int yb = c.y0; yb < c.y1; yb += R) {
+        T pv[R], rv[R];
+        load_rows(pv, g, c, yb, p);
+        load_rows(rv, g, c, yb, r);
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (yb + q < c.y1) p[at(g, c.x, yb + q)] = beta * pv[q] + rv[q];
     }
 }
 
-// CG.p!, CG.jl:101-104: p = beta p + r
-template <typename T>
-__global__ __launch_bounds__(TB) void k_cg_p(Geo g, T *__restrict__ p, const T *__restrict__ r,
-                                             const Ctrl *__restrict__ ctrl)
-{
-    if (ctrl->G) return;
-    const T beta = (T)ctrl->beta;
-    const Col c = column(g);
-    if (!c.active) return;
-    long i = g.off + (long)c.y0 * g.pitch + c.x;
-    for (int y = c.y0; y < c.y1; ++y, i += g.pitch) p[i] = beta * p[i] + r[i];
-}
+// ---------------------------------------------- residuals and summaries
 
-// --------------------------------------------------- residual / finalise
-
-// residual!, kernels.jl:227-232: r = u0 - A u; optional sum r^2 with a finalizer
-__global__ __launch_bounds__(TB) void k_residual(Geo g, const double *__restrict__ u0, const double *__restrict__ u,
-                                                 const double *__restrict__ kx, const double *__restrict__ ky,
-                                                 double *__restrict__ r, float *__restrict__ r32,
-                                                 float *__restrict__ p32, Ctrl *__restrict__ ctrl,
-                                                 double *__restrict__ part, int with_norm, int fin)
+// residual!, kernels.jl:227-232: r = u0 - A u.  Optional outputs: r (FP64),
+// r32 and p32 (FP32 copies for the mixed-precision CG), partial sums of r^2.
+// Skipped when t > stop_iter (t = 0 always runs).
+__global__ __launch_bounds__(TB) void k_residual(Geo g, int t, const double *__restrict__ u0,
+                                                 const double *__restrict__ u, const double *__restrict__ kx,
+                                                 const double *__restrict__ ky, double *__restrict__ r,
+                                                 float *__restrict__ r32, float *__restrict__ p32,
+                                                 const Ctrl *__restrict__ ctrl, double *__restrict__ part)
 {
-    if (fin == FIN_JACOBI_ADD && ctrl->F) return;
+    if (t > ctrl->stop_iter) return;
     const Col c = column(g);
     double acc[1] = {0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        Window<double> win;
-        win.start(u, i, c.y0, g.pitch);
-        double kyd = ky[i];
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const double uu = win.up(u, i, y, g.ny, g.pitch);
-            const double kyu = ky[i + g.pitch];
-            const double rv = u0[i] - smvp5(win.cur, u[i + c.xl], u[i + c.xr], win.dn, uu, kx[i], kx[i + 1], kyd, kyu);
-            if (r) r[i] = rv;
-            if (r32) {
-                const float rf = (float)rv;
-                r32[i] = rf;
-                if (p32) p32[i] = rf;
-            }
-            acc[0] += rv * rv;
-            win.dn = win.cur;
-            win.cur = uu;
-            kyd = kyu;
-        }
-    }
-    if (!with_norm) return;
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) {
-        const double rr = tot[0];
-        ctrl->sums[0] = rr;
-        if (fin == FIN_JACOBI_ADD) {
-            ctrl->err += rr;  // Jacobi.jl:19-20
-            decide(ctrl, ctrl->err);
-        } else if (fin == FIN_IR) {
-            // FP64 refresh of the mixed-precision CG: the true residual
-            // replaces the FP32 recurrence's; converged or resume
-            ctrl->rro = rr;
-            ctrl->err = rr;
-            ctrl->refresh_abs = ctrl->ir_delta * sqrt(rr);
-            ctrl->G = 0;
-            if (sqrt(rr) <= ctrl->tol_abs) {
-                ctrl->why = WHY_CONVERGED;
-                ctrl->F = 1;
-            } else if (ctrl->iters >= ctrl->maxiters || !isfinite(rr)) {
-                ctrl->why = isfinite(rr) ? WHY_MAXITERS : WHY_BREAKDOWN;
-                if (!isfinite(rr)) ctrl->status = TL_E_BREAKDOWN;
-                ctrl->F = 1;
-            } else {
-                ctrl->why = WHY_NONE;
-                ctrl->F = 0;
+        Win<double> win;
+        win.start(g, c, u, ky);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            Stn<double> st;
+            double b[R];
+            load_stn(st, g, c, yb, u, kx, ky);
+            load_rows(b, g, c, yb, u0);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const double rv = b[q] - win.apply(st, q);
+                if (yb + q < c.y1) {
+                    const long i = at(g, c.x, yb + q);
+                    if (r) r[i] = rv;
+                    if (r32) {
+                        const float rf = (float)rv;
+                        r32[i] = rf;
+                        if (p32) p32[i] = rf;
+                    }
+                    acc[0] += rv * rv;
+                }
+                win.advance(st, q);
             }
         }
     }
+    if (part) write_partials<1>(acc, part, nblocks(g));
+}
+
+// partial sums of (b - A x)^2
+__global__ __launch_bounds__(TB) void k_resid_norm(Geo g, const double *__restrict__ b,
+                                                   const double *__restrict__ xs, const double *__restrict__ kx,
+                                                   const double *__restrict__ ky, double *__restrict__ part)
+{
+    const Col c = column(g);
+    double acc[1] = {0.0};
+    if (c.active) {
+        Win<double> win;
+        win.start(g, c, xs, ky);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            Stn<double> st;
+            double bb[R];
+            load_stn(st, g, c, yb, xs, kx, ky);
+            load_rows(bb, g, c, yb, b);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const double rv = bb[q] - win.apply(st, q);
+                if (yb + q < c.y1) acc[0] += rv * rv;
+                win.advance(st, q);
+            }
+        }
+    }
+    write_partials<1>(acc, part, nblocks(g));
 }
 
 // dst = A src over the interior
@@ -552,17 +711,17 @@ __global__ __launch_bounds__(TB) void k_apply(Geo g, const double *__restrict__ 
 {
     const Col c = column(g);
     if (!c.active) return;
-    long i = g.off + (long)c.y0 * g.pitch + c.x;
-    Window<double> win;
-    win.start(a, i, c.y0, g.pitch);
-    double kyd = ky[i];
-    for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-        const double au = win.up(a, i, y, g.ny, g.pitch);
-        const double kyu = ky[i + g.pitch];
-        o[i] = smvp5(win.cur, a[i + c.xl], a[i + c.xr], win.dn, au, kx[i], kx[i + 1], kyd, kyu);
-        win.dn = win.cur;
-        win.cur = au;
-        kyd = kyu;
+    Win<double> win;
+    win.start(g, c, a, ky);
+    for (int yb = c.y0; yb < c.y1; yb += R) {
+        Stn<double> st;
+        load_stn(st, g, c, yb, a, kx, ky);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const double v = win.apply(st, q);
+            if (yb + q < c.y1) o[at(g, c.x, yb + q)] = v;
+            win.advance(st, q);
+        }
     }
 }
 
@@ -572,55 +731,61 @@ __global__ __launch_bounds__(TB) void k_finalise(Geo g, const double *__restrict
 {
     const Col c = column(g);
     if (!c.active) return;
-    long i = g.off + (long)c.y0 * g.pitch + c.x;
-    for (int y = c.y0; y < c.y1; ++y, i += g.pitch) e[i] = u[i] / rho[i];
+    for (int yb = c.y0; yb < c.y1; yb += R) {
+        double uv[R], dv[R];
+        load_rows(uv, g, c, yb, u);
+        load_rows(dv, g, c, yb, rho);
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (yb + q < c.y1) e[at(g, c.x, yb + q)] = uv[q] / dv[q];
+    }
 }
 
-// fieldsummary, kernels.jl:119-133: vol, mass, ie, temp
+// fieldsummary, kernels.jl:119-133: partials of vol, mass, ie, temp
 __global__ __launch_bounds__(TB) void k_field_summary(Geo g, const double *__restrict__ rho,
                                                       const double *__restrict__ e, const double *__restrict__ u,
-                                                      double vol, Ctrl *__restrict__ ctrl, double *__restrict__ part)
+                                                      double vol, double *__restrict__ part)
 {
     const Col c = column(g);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const double m = vol * rho[i];
-            acc[0] += vol;
-            acc[1] += m;
-            acc[2] += m * e[i];
-            acc[3] += m * u[i];
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            double dv[R], ev[R], uv[R];
+            load_rows(dv, g, c, yb, rho);
+            load_rows(ev, g, c, yb, e);
+            load_rows(uv, g, c, yb, u);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (yb + q < c.y1) {
+                    const double m = vol * dv[q];
+                    acc[0] += vol;
+                    acc[1] += m;
+                    acc[2] += m * ev[q];
+                    acc[3] += m * uv[q];
+                }
+            }
         }
     }
-    double tot[4];
-    if (grid_reduce<4>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0)
-        for (int a = 0; a < 4; ++a) ctrl->sums[a] = tot[a];
+    write_partials<4>(acc, part, nblocks(g));
 }
 
-// sum v^2 over the interior with a finalizer (PPCG norm when there are no inner steps)
-__global__ __launch_bounds__(TB) void k_sumsq(Geo g, const double *__restrict__ v, Ctrl *__restrict__ ctrl,
-                                              double *__restrict__ part, int fin)
+// partial sums of v^2 (skipped when t > stop_iter; t = 0 always runs)
+__global__ __launch_bounds__(TB) void k_sumsq(Geo g, int t, const double *__restrict__ v,
+                                              const Ctrl *__restrict__ ctrl, double *__restrict__ part)
 {
-    if (fin == FIN_PPCG && ctrl->F) return;
+    if (t > ctrl->stop_iter) return;
     const Col c = column(g);
     double acc[1] = {0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) acc[0] += v[i] * v[i];
-    }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) {
-        ctrl->sums[0] = tot[0];
-        if (fin == FIN_PPCG) {
-            const double rrn = tot[0];
-            ctrl->beta = rrn / ctrl->rro;
-            ctrl->rro = rrn;
-            ctrl->err = rrn;
-            ctrl->iters += 1;
-            decide(ctrl, rrn);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            double vv[R];
+            load_rows(vv, g, c, yb, v);
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                if (yb + q < c.y1) acc[0] += vv[q] * vv[q];
         }
     }
+    write_partials<1>(acc, part, nblocks(g));
 }
 
 // ---------------------------------------------------------------- halo
@@ -654,192 +819,193 @@ __global__ __launch_bounds__(TB) void k_halo(Geo g, HaloArgs h, int depth)
 // ------------------------------------------------------------ Chebyshev
 
 // Cheby.init!, Cheby.jl:64-81: w = A u; r = u0 - w; p = r/theta; u' = u + p;
-// bb = sum u0^2 (u' is a separate buffer: the stencil reads u's neighbours)
+// partials of bb = sum u0^2 (u' is a separate buffer: the stencil reads u's
+// neighbours)
 __global__ __launch_bounds__(TB) void k_cheby_init(Geo g, const double *__restrict__ uin, double *__restrict__ uout,
                                                    const double *__restrict__ u0, double *__restrict__ p,
                                                    double *__restrict__ r, const double *__restrict__ kx,
                                                    const double *__restrict__ ky, double theta,
-                                                   Ctrl *__restrict__ ctrl, double *__restrict__ part)
+                                                   double *__restrict__ part)
 {
     const Col c = column(g);
     double acc[1] = {0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        Window<double> win;
-        win.start(uin, i, c.y0, g.pitch);
-        double kyd = ky[i];
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const double uu = win.up(uin, i, y, g.ny, g.pitch);
-            const double kyu = ky[i + g.pitch];
-            const double b = u0[i];
-            const double wv = smvp5(win.cur, uin[i + c.xl], uin[i + c.xr], win.dn, uu, kx[i], kx[i + 1], kyd, kyu);
-            const double rv = b - wv;
-            const double pv = rv / theta;
-            r[i] = rv;
-            p[i] = pv;
-            uout[i] = win.cur + pv;
-            acc[0] += b * b;
-            win.dn = win.cur;
-            win.cur = uu;
-            kyd = kyu;
+        Win<double> win;
+        win.start(g, c, uin, ky);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            Stn<double> st;
+            double b[R];
+            load_stn(st, g, c, yb, uin, kx, ky);
+            load_rows(b, g, c, yb, u0);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const double rv = b[q] - win.apply(st, q);
+                const double pv = rv / theta;
+                if (yb + q < c.y1) {
+                    const long i = at(g, c.x, yb + q);
+                    r[i] = rv;
+                    p[i] = pv;
+                    uout[i] = win.cur + pv;
+                    acc[0] += b[q] * b[q];
+                }
+                win.advance(st, q);
+            }
         }
     }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) ctrl->bb = tot[0];
+    write_partials<1>(acc, part, nblocks(g));
 }
 
 // Cheby.mainstep!, Cheby.jl:84-106: w = A u; r = u0 - w; p = a p + b r;
-// u' = u + p; optional err = sum r^2 (calc2norm) with the stop test.
-// `it` is the Chebyshev iteration index, recorded when this launch decides.
-__global__ __launch_bounds__(TB) void k_cheby_iter(Geo g, const double *__restrict__ uin, double *__restrict__ uout,
-                                                   const double *__restrict__ u0, double *__restrict__ p,
-                                                   const double *__restrict__ kx, const double *__restrict__ ky,
-                                                   double ca, double cb, int norm, int it,
-                                                   Ctrl *__restrict__ ctrl, double *__restrict__ part)
+// u' = u + p; with `norm`, partials of sum r^2 (calc2norm)
+__global__ __launch_bounds__(TB) void k_cheby_iter(Geo g, int t, const double *__restrict__ uin,
+                                                   double *__restrict__ uout, const double *__restrict__ u0,
+                                                   double *__restrict__ p, const double *__restrict__ kx,
+                                                   const double *__restrict__ ky, double ca, double cb, int norm,
+                                                   const Ctrl *__restrict__ ctrl, double *__restrict__ part)
 {
-    if (ctrl->F) return;
+    if (t > ctrl->stop_iter) return;
     const Col c = column(g);
     double acc[1] = {0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        Window<double> win;
-        win.start(uin, i, c.y0, g.pitch);
-        double kyd = ky[i];
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const double uu = win.up(uin, i, y, g.ny, g.pitch);
-            const double kyu = ky[i + g.pitch];
-            const double wv = smvp5(win.cur, uin[i + c.xl], uin[i + c.xr], win.dn, uu, kx[i], kx[i + 1], kyd, kyu);
-            const double rv = u0[i] - wv;
-            const double pv = ca * p[i] + cb * rv;
-            p[i] = pv;
-            uout[i] = win.cur + pv;
-            acc[0] += rv * rv;
-            win.dn = win.cur;
-            win.cur = uu;
-            kyd = kyu;
+        Win<double> win;
+        win.start(g, c, uin, ky);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            Stn<double> st;
+            double b[R], pp[R];
+            load_stn(st, g, c, yb, uin, kx, ky);
+            load_rows(b, g, c, yb, u0);
+            load_rows(pp, g, c, yb, p);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const double rv = b[q] - win.apply(st, q);
+                const double pv = ca * pp[q] + cb * rv;
+                if (yb + q < c.y1) {
+                    const long i = at(g, c.x, yb + q);
+                    p[i] = pv;
+                    uout[i] = win.cur + pv;
+                    acc[0] += rv * rv;
+                }
+                win.advance(st, q);
+            }
         }
     }
-    if (!norm) return;
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) {
-        ctrl->err = tot[0];
-        ctrl->iters = it;
-        if (fabs(tot[0]) < ctrl->eps) {  // Cheby.jl:57
-            ctrl->why = WHY_CONVERGED;
-            ctrl->F = 1;
-        }
-    }
+    if (norm) write_partials<1>(acc, part, nblocks(g));
 }
 
 // ----------------------------------------------------------------- PPCG
 
 // CG.ur! without its norm (PPCG.jl:67, value unused) fused with
 // PPCG.init!(chunk, hd) (PPCG.jl:97-100): u += alpha p; r -= alpha w; sd = r/theta
-__global__ __launch_bounds__(TB) void k_ppcg_ur_sd(Geo g, double *__restrict__ u, const double *__restrict__ p,
-                                                   double *__restrict__ r, const double *__restrict__ w,
-                                                   double *__restrict__ sd, double theta,
-                                                   const Ctrl *__restrict__ ctrl)
+__global__ __launch_bounds__(TB) void k_ppcg_ur_sd(Geo g, int t, double *__restrict__ u,
+                                                   const double *__restrict__ p, double *__restrict__ r,
+                                                   const double *__restrict__ w, double *__restrict__ sd,
+                                                   double theta, Ctrl *__restrict__ ctrl,
+                                                   const double *__restrict__ pw, const double *__restrict__ prprev)
 {
-    if (ctrl->F) return;
-    const double alpha = ctrl->alpha;
+    if (t > ctrl->stop_iter) return;
+    double alpha;
+    if (!cg_alpha(g, t, ctrl, pw, prprev, nullptr, &alpha)) return;
     const Col c = column(g);
     if (!c.active) return;
-    long i = g.off + (long)c.y0 * g.pitch + c.x;
-    for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-        u[i] = u[i] + alpha * p[i];
-        const double rv = r[i] - alpha * w[i];
-        r[i] = rv;
-        sd[i] = rv / theta;
+    for (int yb = c.y0; yb < c.y1; yb += R) {
+        double uv[R], pv[R], rv[R], wv[R];
+        load_rows(uv, g, c, yb, u);
+        load_rows(pv, g, c, yb, p);
+        load_rows(rv, g, c, yb, r);
+        load_rows(wv, g, c, yb, w);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if (yb + q < c.y1) {
+                const long i = at(g, c.x, yb + q);
+                u[i] = uv[q] + alpha * pv[q];
+                const double rn = rv[q] - alpha * wv[q];
+                r[i] = rn;
+                sd[i] = rn / theta;
+            }
+        }
     }
 }
 
 // PPCG inner step (PPCG.jl:75-84 in the two-phase form): r -= A sd;
-// u += sd; sd' = a sd + b r.  On the last step: rrn = sum r^2 (PPCG.jl:88),
-// beta = rrn/rro and the stop test.
-__global__ __launch_bounds__(TB) void k_ppcg_inner(Geo g, const double *__restrict__ sdin,
+// u += sd; sd' = a sd + b r.  With `last`, partials of rrn = sum r^2
+// (PPCG.jl:88), consumed by the following CG.p! kernel.
+__global__ __launch_bounds__(TB) void k_ppcg_inner(Geo g, int t, const double *__restrict__ sdin,
                                                    double *__restrict__ sdout, double *__restrict__ r,
                                                    double *__restrict__ u, const double *__restrict__ kx,
                                                    const double *__restrict__ ky, double ca, double cb, int last,
-                                                   Ctrl *__restrict__ ctrl, double *__restrict__ part)
+                                                   const Ctrl *__restrict__ ctrl, double *__restrict__ part)
 {
-    if (ctrl->F) return;
+    if (t > ctrl->stop_iter) return;
     const Col c = column(g);
     double acc[1] = {0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        Window<double> win;
-        win.start(sdin, i, c.y0, g.pitch);
-        double kyd = ky[i];
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const double su = win.up(sdin, i, y, g.ny, g.pitch);
-            const double kyu = ky[i + g.pitch];
-            const double s = win.cur;
-            const double rv = r[i] - smvp5(s, sdin[i + c.xl], sdin[i + c.xr], win.dn, su, kx[i], kx[i + 1], kyd, kyu);
-            r[i] = rv;
-            u[i] = u[i] + s;
-            sdout[i] = ca * s + cb * rv;
-            acc[0] += rv * rv;
-            win.dn = s;
-            win.cur = su;
-            kyd = kyu;
+        Win<double> win;
+        win.start(g, c, sdin, ky);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            Stn<double> st;
+            double rr[R], uu[R];
+            load_stn(st, g, c, yb, sdin, kx, ky);
+            load_rows(rr, g, c, yb, r);
+            load_rows(uu, g, c, yb, u);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const double s = win.cur;
+                const double rv = rr[q] - win.apply(st, q);
+                if (yb + q < c.y1) {
+                    const long i = at(g, c.x, yb + q);
+                    r[i] = rv;
+                    u[i] = uu[q] + s;
+                    sdout[i] = ca * s + cb * rv;
+                    acc[0] += rv * rv;
+                }
+                win.advance(st, q);
+            }
         }
     }
-    if (!last) return;
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) {
-        const double rrn = tot[0];
-        ctrl->beta = rrn / ctrl->rro;
-        ctrl->rro = rrn;
-        ctrl->err = rrn;
-        ctrl->iters += 1;
-        decide(ctrl, rrn);
-    }
+    if (last) write_partials<1>(acc, part, nblocks(g));
 }
 
 // --------------------------------------------------------------- Jacobi
 
 // Jacobi.iterate!, Jacobi.jl:59-80 (u' in a separate buffer, the reference's
-// r = copy of u): u' = (u0 + sum K u_nbr) / (1 + sum K); err = sum |u' - u|.
-// `defer`: the residual norm of Jacobi.jl:16-21 follows and decides.
-__global__ __launch_bounds__(TB) void k_jacobi(Geo g, const double *__restrict__ uin, double *__restrict__ uout,
-                                               const double *__restrict__ u0, const double *__restrict__ kx,
-                                               const double *__restrict__ ky, int tt, int defer,
-                                               Ctrl *__restrict__ ctrl, double *__restrict__ part)
+// r = copy of u): u' = (u0 + sum K u_nbr) / (1 + sum K); partials of
+// err = sum |u' - u|
+__global__ __launch_bounds__(TB) void k_jacobi(Geo g, int t, const double *__restrict__ uin,
+                                               double *__restrict__ uout, const double *__restrict__ u0,
+                                               const double *__restrict__ kx, const double *__restrict__ ky,
+                                               const Ctrl *__restrict__ ctrl, double *__restrict__ part)
 {
-    if (ctrl->F) return;
+    if (t > ctrl->stop_iter) return;
     const Col c = column(g);
     double acc[1] = {0.0};
     if (c.active) {
-        long i = g.off + (long)c.y0 * g.pitch + c.x;
-        Window<double> win;
-        win.start(uin, i, c.y0, g.pitch);
-        double kyd = ky[i];
-        for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-            const double uu = win.up(uin, i, y, g.ny, g.pitch);
-            const double kyu = ky[i + g.pitch];
-            const double kxl = kx[i], kxr = kx[i + 1];
-            double num = u0[i] + kxr * uin[i + c.xr];
-            num = num + kxl * uin[i + c.xl];
-            num = num + kyu * uu;
-            num = num + kyd * win.dn;
-            double den = 1.0 + kxl;
-            den = den + kxr;
-            den = den + kyd;
-            den = den + kyu;
-            const double un = num / den;
-            uout[i] = un;
-            acc[0] += fabs(un - win.cur);
-            win.dn = win.cur;
-            win.cur = uu;
-            kyd = kyu;
+        Win<double> win;
+        win.start(g, c, uin, ky);
+        for (int yb = c.y0; yb < c.y1; yb += R) {
+            Stn<double> st;
+            double b[R];
+            load_stn(st, g, c, yb, uin, kx, ky);
+            load_rows(b, g, c, yb, u0);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double num = b[q] + st.kxr[q] * st.rt[q];
+                num = num + st.kxl[q] * st.lf[q];
+                num = num + st.kyu[q] * st.up[q];
+                num = num + win.kyd * win.dn;
+                double den = 1.0 + st.kxl[q];
+                den = den + st.kxr[q];
+                den = den + win.kyd;
+                den = den + st.kyu[q];
+                const double un = num / den;
+                if (yb + q < c.y1) {
+                    uout[at(g, c.x, yb + q)] = un;
+                    acc[0] += fabs(un - win.cur);
+                }
+                win.advance(st, q);
+            }
         }
     }
-    double tot[1];
-    if (grid_reduce<1>(acc, part, &ctrl->ticket, tot) && threadIdx.x == 0) {
-        ctrl->err = tot[0];
-        ctrl->iters = tt;
-        if (!defer) decide(ctrl, tot[0]);
-    }
+    write_partials<1>(acc, part, nblocks(g));
 }
 
 // ------------------------------------------------------ mixed precision
@@ -856,10 +1022,19 @@ __global__ __launch_bounds__(TB) void k_ir_accumulate(Geo g, double *__restrict_
 {
     const Col c = column(g);
     if (!c.active) return;
-    long i = g.off + (long)c.y0 * g.pitch + c.x;
-    for (int y = c.y0; y < c.y1; ++y, i += g.pitch) {
-        u[i] = u[i] + (double)x[i];
-        x[i] = 0.0f;
+    for (int yb = c.y0; yb < c.y1; yb += R) {
+        double uv[R];
+        float xv[R];
+        load_rows(uv, g, c, yb, u);
+        load_rows(xv, g, c, yb, x);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if (yb + q < c.y1) {
+                const long i = at(g, c.x, yb + q);
+                u[i] = uv[q] + (double)xv[q];
+                x[i] = 0.0f;
+            }
+        }
     }
 }
 
@@ -867,6 +1042,38 @@ __global__ __launch_bounds__(TB) void k_ir_accumulate(Geo g, double *__restrict_
 
 static inline dim3 sgrid(const Geo &g) { return dim3(g.gx, g.gy); }
 static inline dim3 fgrid(const Geo &g) { return dim3((g.X + TB - 1) / TB, g.Y); }
+static inline unsigned nbh(const tl_ctx *c) { return (unsigned)(c->g.gx * c->g.gy); }
+// partial-sum buffers: [pw | pr0 | pr1 | pm0 pm1 pm2 pm3], nb doubles each
+static inline double *part_w(tl_ctx *c) { return c->partials; }
+static inline double *part_r(tl_ctx *c, int t) { return c->partials + (size_t)(1 + (t & 1)) * nbh(c); }
+static inline double *part_m(tl_ctx *c) { return c->partials + (size_t)3 * nbh(c); }
+
+// event pair around one launch when this kernel is being profiled
+static bool prof_start(tl_ctx *c, int kind)
+{
+    if (!(c->prof_mask & (1u << kind))) return false;
+    if (c->prof_used + 2 > c->prof_ev.size()) {
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return false;
+        c->prof_ev.push_back(e0);
+        c->prof_ev.push_back(e1);
+        c->prof_tag.push_back(kind);
+    }
+    c->prof_tag[c->prof_used / 2] = kind;
+    return hipEventRecord(c->prof_ev[c->prof_used], c->stream) == hipSuccess;
+}
+
+static void prof_stop(tl_ctx *c)
+{
+    (void)hipEventRecord(c->prof_ev[c->prof_used + 1], c->stream);
+    c->prof_used += 2;
+}
+
+static hipError_t fin(tl_ctx *c, const double *part, int n, int slot, int kind, int t)
+{
+    k_fin<<<1, TB, 0, c->stream>>>(c->ctrl, part, nbh(c), n, slot, kind, t);
+    return hipGetLastError();
+}
 
 hipError_t launch_init_ctrl(tl_ctx *c, int mode, int maxiters, double eps, double rel_tol, double ir_delta,
                             int presteps, int errorswitch, double epslim)
@@ -889,17 +1096,13 @@ hipError_t launch_set_state(tl_ctx *c, const tl_state *states_d, int nstates)
     return hipGetLastError();
 }
 
-hipError_t launch_copy_energy(tl_ctx *c)
-{
-    k_copy_full<<<fgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_ENERGY0], c->f[TL_F_ENERGY]);
-    return hipGetLastError();
-}
-
 hipError_t launch_copy_full(tl_ctx *c, const double *a, double *b)
 {
     k_copy_full<<<fgrid(c->g), TB, 0, c->stream>>>(c->g, a, b);
     return hipGetLastError();
 }
+
+hipError_t launch_copy_energy(tl_ctx *c) { return launch_copy_full(c, c->f[TL_F_ENERGY0], c->f[TL_F_ENERGY]); }
 
 hipError_t launch_init_coef(tl_ctx *c, double rx, double ry, int coefficient, int what)
 {
@@ -913,79 +1116,96 @@ hipError_t launch_cg_init_resid(tl_ctx *c, const double *guess)
 {
     k_cg_init_resid<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U], guess, c->f[TL_F_U0], c->f[TL_F_P],
                                                        c->f[TL_F_R], c->f[TL_F_W], c->f[TL_F_KX], c->f[TL_F_KY],
-                                                       c->ctrl, c->partials);
-    return hipGetLastError();
+                                                       part_r(c, 0), part_m(c));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? fin(c, part_m(c), 2, 0, FK_INIT, 0) : e;
 }
 
 hipError_t launch_resid_norm_slot(tl_ctx *c, const double *b, const double *x, int slot)
 {
-    k_resid_norm<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, b, x, c->f[TL_F_KX], c->f[TL_F_KY], c->ctrl, c->partials,
-                                                    slot);
+    k_resid_norm<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, b, x, c->f[TL_F_KX], c->f[TL_F_KY], part_m(c));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? fin(c, part_m(c), 1, slot, FK_SUMS, 0) : e;
+}
+
+hipError_t launch_cg_w64(tl_ctx *c, int t)
+{
+    const bool prof = prof_start(c, TL_K_CG_W64);
+    k_cg_w<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_P], c->f[TL_F_KX], c->f[TL_F_KY],
+                                                       c->f[TL_F_W], c->ctrl, part_w(c));
+    if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_w64(tl_ctx *c)
+hipError_t launch_cg_ur64(tl_ctx *c, int t)
 {
-    k_cg_w<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_P], c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_W],
-                                                       c->ctrl, c->partials, c->hist_a);
+    const bool prof = prof_start(c, TL_K_CG_UR64);
+    k_cg_ur<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_U], c->f[TL_F_P], c->f[TL_F_R],
+                                                        c->f[TL_F_W], c->ctrl, part_w(c), part_r(c, t - 1),
+                                                        part_r(c, t), c->hist_a);
+    if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_ur64(tl_ctx *c)
+hipError_t launch_cg_p64(tl_ctx *c, int t)
 {
-    k_cg_ur<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U], c->f[TL_F_P], c->f[TL_F_R], c->f[TL_F_W],
-                                                        c->ctrl, c->partials, c->hist_b);
+    const bool prof = prof_start(c, TL_K_CG_P64);
+    k_cg_p<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_P], c->f[TL_F_R], c->ctrl,
+                                                       part_r(c, t - 1), part_r(c, t), c->hist_b);
+    if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_p64(tl_ctx *c)
+hipError_t launch_cg_w32(tl_ctx *c, int t)
 {
-    k_cg_p<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_P], c->f[TL_F_R], c->ctrl);
+    const bool prof = prof_start(c, TL_K_CG_W32);
+    k_cg_w<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->p32, c->kx32, c->ky32, c->w32, c->ctrl,
+                                                      part_w(c));
+    if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_w32(tl_ctx *c)
+hipError_t launch_cg_ur32(tl_ctx *c, int t)
 {
-    k_cg_w<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->p32, c->kx32, c->ky32, c->w32, c->ctrl, c->partials,
-                                                      nullptr);
+    const bool prof = prof_start(c, TL_K_CG_UR32);
+    k_cg_ur<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->x32, c->p32, c->r32, c->w32, c->ctrl,
+                                                       part_w(c), part_r(c, t - 1), part_r(c, t), nullptr);
+    if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_ur32(tl_ctx *c)
+hipError_t launch_cg_p32(tl_ctx *c, int t)
 {
-    k_cg_ur<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->x32, c->p32, c->r32, c->w32, c->ctrl, c->partials,
-                                                       nullptr);
+    const bool prof = prof_start(c, TL_K_CG_P32);
+    k_cg_p<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->p32, c->r32, c->ctrl, part_r(c, t - 1),
+                                                      part_r(c, t), nullptr);
+    if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_p32(tl_ctx *c)
+hipError_t launch_residual(tl_ctx *c)
 {
-    k_cg_p<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->p32, c->r32, c->ctrl);
+    k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],
+                                                  c->f[TL_F_KY], c->f[TL_F_R], nullptr, nullptr, c->ctrl, nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_residual(tl_ctx *c, int with_norm, int finalize_kind)
+hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t)
 {
-    k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX], c->f[TL_F_KY],
-                                                  c->f[TL_F_R], nullptr, nullptr, c->ctrl, c->partials, with_norm,
-                                                  finalize_kind);
-    return hipGetLastError();
+    k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_U0], u, c->f[TL_F_KX], c->f[TL_F_KY],
+                                                  c->f[TL_F_R], nullptr, nullptr, c->ctrl,
+                                                  part_m(c) + nbh(c));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? fin(c, part_m(c), 2, 0, FK_JACOBI, t) : e;
 }
 
-hipError_t launch_residual_u(tl_ctx *c, const double *u, int with_norm, int finalize_kind)
+hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next)
 {
-    k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U0], u, c->f[TL_F_KX], c->f[TL_F_KY],
-                                                  c->f[TL_F_R], nullptr, nullptr, c->ctrl, c->partials, with_norm,
-                                                  finalize_kind);
-    return hipGetLastError();
-}
-
-hipError_t launch_ir_residual(tl_ctx *c, int reset_p)
-{
-    k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX], c->f[TL_F_KY],
-                                                  nullptr, c->r32, reset_p ? c->p32 : nullptr, c->ctrl, c->partials,
-                                                  1, FIN_IR);
-    return hipGetLastError();
+    k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],
+                                                  c->f[TL_F_KY], nullptr, c->r32, reset_p ? c->p32 : nullptr,
+                                                  c->ctrl, part_r(c, t_next - 1));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? fin(c, part_r(c, t_next - 1), 1, 0, FK_IR, 0) : e;
 }
 
 hipError_t launch_apply(tl_ctx *c, const double *src, double *dst)
@@ -1003,13 +1223,21 @@ hipError_t launch_finalise(tl_ctx *c)
 hipError_t launch_field_summary(tl_ctx *c)
 {
     k_field_summary<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_DENSITY], c->f[TL_F_ENERGY], c->f[TL_F_U],
-                                                       c->dx * c->dy, c->ctrl, c->partials);
-    return hipGetLastError();
+                                                       c->dx * c->dy, part_m(c));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? fin(c, part_m(c), 4, 0, FK_SUMS, 0) : e;
 }
 
-hipError_t launch_sumsq(tl_ctx *c, const double *v, int finalize_kind)
+hipError_t launch_sumsq_slot(tl_ctx *c, const double *v, int slot)
 {
-    k_sumsq<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, v, c->ctrl, c->partials, finalize_kind);
+    k_sumsq<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, v, c->ctrl, part_m(c));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? fin(c, part_m(c), 1, slot, FK_SUMS, 0) : e;
+}
+
+hipError_t launch_sumsq_rr(tl_ctx *c, const double *v, int t)
+{
+    k_sumsq<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, v, c->ctrl, part_r(c, t));
     return hipGetLastError();
 }
 
@@ -1030,36 +1258,41 @@ hipError_t launch_halo(tl_ctx *c, unsigned mask, int depth, double *const *extra
 hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double *u_out)
 {
     k_cheby_init<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_P], c->f[TL_F_R],
-                                                    c->f[TL_F_KX], c->f[TL_F_KY], theta, c->ctrl, c->partials);
+                                                    c->f[TL_F_KX], c->f[TL_F_KY], theta, part_m(c));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? fin(c, part_m(c), 1, 0, FK_BB, 0) : e;
+}
+
+hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t)
+{
+    k_cheby_iter<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_P],
+                                                    c->f[TL_F_KX], c->f[TL_F_KY], a, b, norm, c->ctrl, part_m(c));
+    hipError_t e = hipGetLastError();
+    return (e == hipSuccess && norm) ? fin(c, part_m(c), 1, 0, FK_CHEBY, t) : e;
+}
+
+hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t)
+{
+    k_ppcg_ur_sd<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_U], c->f[TL_F_P], c->f[TL_F_R],
+                                                    c->f[TL_F_W], c->f[TL_F_SD], theta, c->ctrl, part_w(c),
+                                                    part_r(c, t - 1));
     return hipGetLastError();
 }
 
-hipError_t launch_cheby_iter2(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int it)
+hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t)
 {
-    k_cheby_iter<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_P], c->f[TL_F_KX],
-                                                    c->f[TL_F_KY], a, b, norm, it, c->ctrl, c->partials);
+    k_ppcg_inner<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, sd_in, sd_out, c->f[TL_F_R], c->f[TL_F_U],
+                                                    c->f[TL_F_KX], c->f[TL_F_KY], a, b, last, c->ctrl,
+                                                    part_r(c, t));
     return hipGetLastError();
 }
 
-hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta)
+hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer)
 {
-    k_ppcg_ur_sd<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U], c->f[TL_F_P], c->f[TL_F_R], c->f[TL_F_W],
-                                                    c->f[TL_F_SD], theta, c->ctrl);
-    return hipGetLastError();
-}
-
-hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last)
-{
-    k_ppcg_inner<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, sd_in, sd_out, c->f[TL_F_R], c->f[TL_F_U], c->f[TL_F_KX],
-                                                    c->f[TL_F_KY], a, b, last, c->ctrl, c->partials);
-    return hipGetLastError();
-}
-
-hipError_t launch_jacobi2(tl_ctx *c, const double *u_in, double *u_out, int tt, int defer)
-{
-    k_jacobi<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_KX], c->f[TL_F_KY], tt,
-                                                defer, c->ctrl, c->partials);
-    return hipGetLastError();
+    k_jacobi<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_KX], c->f[TL_F_KY],
+                                                c->ctrl, part_m(c));
+    hipError_t e = hipGetLastError();
+    return (e == hipSuccess && !defer) ? fin(c, part_m(c), 1, 0, FK_JACOBI, t) : e;
 }
 
 hipError_t launch_ir_prepare(tl_ctx *c)

@@ -1,0 +1,104 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, its deck parser agrees with the oracle's, and its host-side
+scalar routines (eigenvalue bounds, Chebyshev coefficients, iteration
+estimate) agree with the oracle and with textbook results."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import oracle as O
+import tealeaf_hip as T
+from tealeaf_hip import _lib, decks
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def test_exports_match_header():
+    hdr = open(os.path.join(ROOT, "include", "tealeaf_hip.h")).read()
+    declared = set(re.findall(r"\b(tl_[a-z0-9_]+)\s*\(", hdr)) - {"tl_step_cb"}
+    L = T.load()
+    missing = [n for n in sorted(declared) if not hasattr(L, n)]
+    assert not missing, missing
+    assert declared <= set(_lib.EXPORTS) | {"tl_step_cb"}, declared - set(_lib.EXPORTS)
+
+
+def test_struct_sizes_match_header():
+    # the ctypes mirrors must agree with the C layout the library was built with
+    p = _lib.SolveParams()
+    T.load().tl_default_params(C.byref(p))
+    assert (p.solver, p.coefficient, p.max_iters, p.presteps, p.ppcg_inner_steps) == (0, 1, 10000, 30, 10)
+    assert p.eps == 1e-15 and p.epslim == 1e-5 and p.rel_tol == 1e-10 and p.check_result == 1
+
+
+@pytest.mark.parametrize("text", [
+    decks.reference_deck(512, 10),
+    decks.reference_deck(4096, 20, solver="ppcg", presteps=8),
+    decks.random_rect_deck(1001, n=8192, steps=20),
+    decks.random_rect_deck(3, n=100, steps=2, nrect=5),
+])
+def test_parser_agrees_with_oracle(text):
+    s = T.Settings.from_text(text)
+    d = O.parse_deck(text)
+    assert (s.xcells, s.ycells, s.halodepth) == (d.xcells, d.ycells, d.halodepth)
+    assert s.dx == d.dx and s.dy == d.dy and s.dtinit == d.dtinit and s.eps == d.eps
+    assert s.maxiters == d.maxiters and s.presteps == d.presteps and s.ppcginnersteps == d.ppcginnersteps
+    assert s.solver.name == d.solver and s.endstep == d.endstep
+    assert len(s.states) == len(d.states)
+    for a, b in zip(s.states, d.states):
+        for k in ("density", "energy", "xmin", "ymin", "xmax", "ymax", "radius", "geometry"):
+            assert a[k] == getattr(b, k), (k, a[k], getattr(b, k))
+
+
+def test_parser_build_keys_and_warnings():
+    s = T.Settings.from_text(decks.random_rect_deck(1, n=64, steps=1, precision="fp32_ir", warm_start="learned")
+                             + "bogus_key=1\nuse_chebyshev\n")
+    assert s.precision == T.TL_FP32_IR and s.warm_start == T.TL_WARM_LEARNED
+    assert s.stop_rule == T.TL_STOP_RELATIVE and s.rel_tol == 1e-10
+    assert s.n_warnings == 2  # unknown key and unknown solver: warned, ignored (settings.jl:109,127)
+    with pytest.raises(T.TeaLeafError):
+        T.Settings.from_text("x_cells=4\n")  # no state 1
+
+
+def test_eigen_bounds_match_scipy_and_oracle():
+    rng = np.random.default_rng(2)
+    L = T.load()
+    for n in (1, 2, 7, 30):
+        a = rng.uniform(0.05, 1.0, n)
+        b = rng.uniform(0.05, 0.9, n)
+        mn, mx = C.c_double(), C.c_double()
+        assert L.tl_eigen_bounds(_dp(a), _dp(b), n, C.byref(mn), C.byref(mx)) == 0
+        d = 1 / a + np.r_[0, b[:-1] / a[:-1]]
+        e = np.sqrt(b[:-1]) / a[:-1]
+        ev = sla.eigvalsh_tridiagonal(d, e) if n > 1 else d
+        assert abs(mn.value - 0.95 * ev[0]) <= 1e-12 * ev[0]
+        assert abs(mx.value - 1.05 * ev[-1]) <= 1e-12 * ev[-1]
+        omn, omx = O.eigenvalues(np.r_[0, a], np.r_[0, b], n)
+        assert abs(omn - mn.value) <= 1e-12 * omn and abs(omx - mx.value) <= 1e-12 * omx
+
+
+def test_cheby_coef_and_calciter_match_oracle():
+    L = T.load()
+    n = 40
+    a, b, th = np.zeros(n), np.zeros(n), C.c_double()
+    L.tl_cheby_coef(1.3, 870.0, n, _dp(a), _dp(b), C.byref(th))
+    oa, ob, oth = O.cheby_coef(1.3, 870.0, n)
+    np.testing.assert_array_equal(a, oa[1:n + 1])
+    np.testing.assert_array_equal(b, ob[1:n + 1])
+    assert th.value == oth
+    for args in ((1.2, 880.0, 3.3e-2, 4.1e3), (0.9, 55.0, 1e-3, 1e2), (1.0, 2.0e6, 5.0, 1e9)):
+        assert L.tl_cheby_calciter(*args) == O.calciter(*args)
+
+
+def test_invalid_arguments_return_codes():
+    L = T.load()
+    ctx = C.c_void_p()
+    assert L.tl_create(C.byref(ctx), 0, 10, 2, 0.0, 0.0, 1.0, 1.0, 0) == _lib.TL_E_INVALID
+    assert L.tl_create(C.byref(ctx), 10, 10, 0, 0.0, 0.0, 1.0, 1.0, 0) == _lib.TL_E_INVALID

@@ -221,17 +221,26 @@ typedef struct {
     int reserved[8];
 } tl_train_params;
 
-/* number of FP32 parameters of the network */
+/* number of FP32 parameters of the network (flat layout per layer:
+ * W[tap][cout][cin] with tap = 3*(dy+1)+(dx+1), then bias[cout]) */
 int tl_unet_param_count(void);
 /* load / read the network's FP32 master weights (host, tl_unet_param_count entries) */
 int tl_unet_set_weights(tl_ctx *ctx, const float *w);
 int tl_unet_get_weights(tl_ctx *ctx, float *w);
-/* in-device dataset of training samples recorded from this context's own solves */
-int tl_unet_record(tl_ctx *ctx, int enable);
+/* in-device dataset recorded from this context's own solves: after every
+ * tl_solve, crops_per_step random 128x128 crops of (network input, target
+ * (u_new - u_prev)/s_u), up to `capacity` samples; capacity 0 stops recording */
+int tl_unet_record(tl_ctx *ctx, int capacity, int crops_per_step);
 int tl_unet_dataset_size(tl_ctx *ctx);
+/* Adam steps on minibatches of the dataset; loss_out (may be NULL) gets one
+ * mean squared error per step */
 int tl_unet_train(tl_ctx *ctx, const tl_train_params *tp, float *loss_out);
-/* predict the next u into TL_F_MI (host view) from the current fields */
+/* the warm-start prediction of u for the current fields into TL_F_MI */
 int tl_unet_predict(tl_ctx *ctx, double rx, double ry, int coefficient);
+/* diagnostics: network output (and, with target and grads, the MSE loss and
+ * the gradient w.r.t. the flat parameters) for host inputs x[B][H][W][5] */
+int tl_unet_eval(tl_ctx *ctx, const float *x, const float *target, int B, int H, int W, float *out, float *loss,
+                 float *grads);
 
 #ifdef __cplusplus
 }

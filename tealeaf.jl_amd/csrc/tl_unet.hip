@@ -1,20 +1,1040 @@
-// tl_unet.hip -- learned warm start (placeholder until the network lands).
+// tl_unet.hip -- learned warm start: a BF16 U-Net on MFMA, trained on the
+// device from the solver's own trajectories.  No counterpart in the
+// reference; the network is defined by this file:
+//
+//   input: 16 channels (5 used), per cell of the grid, normalised
+//          c0 = u_prev / s_u   c1 = log10(rho) / 2   c2 = energy / s_e
+//          c3 = log10(1 + Kx) / 6   c4 = log10(1 + Ky) / 6
+//          s_u = max|u_prev|, s_e = max|energy| over the grid
+//   enc1  conv3x3 16->16 ReLU, conv3x3 16->16 ReLU           (skip1, H x W)
+//   avg-pool 2x2
+//   enc2  conv3x3 16->32 ReLU, conv3x3 32->32 ReLU           (skip2, H/2)
+//   avg-pool 2x2
+//   mid   conv3x3 32->64 ReLU, conv3x3 64->64 ReLU           (H/4)
+//   nearest x2, concat [up 64 | skip2 32] -> dec2 conv 96->32 ReLU, 32->32 ReLU
+//   nearest x2, concat [up 32 | skip1 16] -> dec1 conv 48->16 ReLU, 16->16 ReLU
+//   out   conv3x3 16->1 (linear): the predicted (u_new - u_prev) / s_u
+//
+// Activations are BF16 NHWC tensors with a one-cell zero border (the conv's
+// zero padding needs no bounds checks); weights are BF16 copies of FP32
+// master weights; every convolution accumulates in FP32 on the matrix cores
+// (v_mfma_f32_16x16x16_bf16: a wave computes 16 pixels x 16 output channels
+// per instruction, K = 16 input channels of one tap).  The input-gradient of
+// a convolution is the same kernel on the output-gradient with flipped,
+// transposed weights.  Weight gradients are FP32 sums over pixels (per-block
+// partials reduced in fixed order), then Adam on the FP32 master weights.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
 #include "tl_internal.hpp"
 
 namespace tl {
-int unet_predict_guess(tl_ctx *c, double, double, int, double *)
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+constexpr int NL = 11;
+struct LayerDef {
+    int cin, cinp, cout, coutp, relu;
+};
+static const LayerDef LAY[NL] = {
+    {5, 16, 16, 16, 1},  {16, 16, 16, 16, 1}, {16, 16, 32, 32, 1}, {32, 32, 32, 32, 1},
+    {32, 32, 64, 64, 1}, {64, 64, 64, 64, 1}, {96, 96, 32, 32, 1}, {32, 32, 32, 32, 1},
+    {48, 48, 16, 16, 1}, {16, 16, 16, 16, 1}, {16, 16, 1, 16, 0},
+};
+
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ bf16 f2bf(float f)
 {
-    c->err = "learned warm start: no network loaded";
-    return TL_E_STATE;
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16)(u >> 16);
 }
-void unet_destroy(tl_ctx *) {}
-int unet_record_step(tl_ctx *, double, double, int) { return TL_OK; }
+
+// a bordered NHWC tensor view: element (b, y, x, c), -1 <= y <= H, -1 <= x <= W
+struct TV {
+    bf16 *p;
+    int B, H, W, C, off;  // C = stored channels, off = first channel of the view
+    __host__ __device__ long idx(int b, int y, int x) const
+    {
+        return (((long)b * (H + 2) + (y + 1)) * (W + 2) + (x + 1)) * C + off;
+    }
+};
+
+// ------------------------------------------------------------- kernels
+
+// Convolution 3x3, zero padding, as implicit GEMM on MFMA.  Block = 4 waves;
+// wave w computes pixels x0 + 16w .. +15 of row y for all output channels.
+// Lane l: m = l & 15 (pixel / output-channel index), kq = l >> 4 (4-channel
+// group of the 16-wide K chunk).  A (pixels x K): A[m][4kq+i] = in[pix m][ci];
+// B (K x N): B[4kq+i][m] = W[tap][co = m][ci]; D: D[4kq+i][m] -> pixel 4kq+i,
+// output channel m.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv(TV in, TV out, const bf16 *__restrict__ wgt,
+                                              const float *__restrict__ bias, int cinp, int relu)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = lane & 15, kq = lane >> 4;
+    const int y = blockIdx.y, b = blockIdx.z;
+    const int x0 = blockIdx.x * 64 + wave * 16;
+    if (x0 >= in.W) return;
+    const int xa = min(x0 + m, in.W - 1);  // this lane's A-row pixel (clamped: rows past W are never stored)
+    f4 acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
+    const int coutp = NT * 16;
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const bf16 *src = in.p + in.idx(b, y + dy, xa + dx) + kq * 4;
+        const bf16 *wt = wgt + ((long)tap * coutp + m) * cinp + kq * 4;
+        for (int kc = 0; kc < cinp; kc += 16) {
+            const s4 a = *(const s4 *)(src + kc);
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                const s4 bv = *(const s4 *)(wt + (long)n * 16 * cinp + kc);
+                acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[n], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int co = n * 16 + m;
+        const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + 4 * kq + i;
+            if (x < out.W) {
+                float v = acc[n][i] + bb;
+                if (relu) v = fmaxf(v, 0.f);
+                out.p[out.idx(b, y, x) + co] = f2bf(v);
+            }
+        }
+    }
+}
+
+// zero a whole bordered tensor (all channels)
+__global__ void k_zero16(bf16 *p, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// 2x2 average pool: out (H/2) from in (H) view
+__global__ void k_pool(TV in, TV out, int C)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)out.B * out.H * out.W * C;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    long r = t / C;
+    const int x = (int)(r % out.W);
+    r /= out.W;
+    const int y = (int)(r % out.H);
+    const int b = (int)(r / out.H);
+    float s = bf2f(in.p[in.idx(b, 2 * y, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y, 2 * x + 1) + c]) +
+              bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x + 1) + c]);
+    out.p[out.idx(b, y, x) + c] = f2bf(0.25f * s);
+}
+
+// nearest x2 upsample: out (2H) from in (H)
+__global__ void k_up(TV in, TV out, int C)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)out.B * out.H * out.W * C;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    long r = t / C;
+    const int x = (int)(r % out.W);
+    r /= out.W;
+    const int y = (int)(r % out.H);
+    const int b = (int)(r / out.H);
+    out.p[out.idx(b, y, x) + c] = in.p[in.idx(b, y / 2, x / 2) + c];
+}
+
+// backward of the upsample: out (H) = sum of the 2x2 block of in (2H)
+__global__ void k_up_back(TV in, TV out, int C)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)out.B * out.H * out.W * C;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    long r = t / C;
+    const int x = (int)(r % out.W);
+    r /= out.W;
+    const int y = (int)(r % out.H);
+    const int b = (int)(r / out.H);
+    float s = bf2f(in.p[in.idx(b, 2 * y, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y, 2 * x + 1) + c]) +
+              bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x + 1) + c]);
+    out.p[out.idx(b, y, x) + c] = f2bf(s);
+}
+
+// dz = (g [+ pool_back(gp)]) * (y > 0 if relu): the gradient at a conv's
+// pre-activation; gp (H/2) is the gradient of the 2x2 pool that reads y
+__global__ void k_dz(TV g, TV gp, int has_gp, TV y, int relu, TV dz, int C)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)dz.B * dz.H * dz.W * C;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    long r = t / C;
+    const int x = (int)(r % dz.W);
+    r /= dz.W;
+    const int yy = (int)(r % dz.H);
+    const int b = (int)(r / dz.H);
+    float v = bf2f(g.p[g.idx(b, yy, x) + c]);
+    if (has_gp) v += 0.25f * bf2f(gp.p[gp.idx(b, yy / 2, x / 2) + c]);
+    if (relu && !(bf2f(y.p[y.idx(b, yy, x) + c]) > 0.f)) v = 0.f;
+    dz.p[dz.idx(b, yy, x) + c] = f2bf(v);
+}
+
+// weight-gradient partials: block (chunk) sums over its pixels, thread = one
+// (co, ci) pair with its 9 taps: part[chunk][tap][co][ci]
+__global__ __launch_bounds__(256) void k_dw(TV x, TV dz, int cinp, int coutp, int px_per_chunk,
+                                            float *__restrict__ part)
+{
+    const int pair = blockIdx.x * 256 + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int npair = cinp * coutp;
+    const long npx = (long)x.B * x.H * x.W;
+    float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int co = pair / cinp, ci = pair % cinp;
+    if (pair < npair) {
+        const long p0 = (long)chunk * px_per_chunk;
+        const long p1 = min(p0 + px_per_chunk, npx);
+        for (long p = p0; p < p1; ++p) {
+            const int xx = (int)(p % x.W);
+            const long r = p / x.W;
+            const int yy = (int)(r % x.H);
+            const int b = (int)(r / x.H);
+            const float g = bf2f(dz.p[dz.idx(b, yy, xx) + co]);
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) {
+                const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+                acc[tap] += g * bf2f(x.p[x.idx(b, yy + dy, xx + dx) + ci]);
+            }
+        }
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap)
+            part[((long)chunk * 9 + tap) * npair + (long)co * cinp + ci] = acc[tap];
+    }
+}
+
+// fixed-order sum of the partials into dw[tap][co][ci]
+__global__ void k_dw_reduce(const float *__restrict__ part, int nchunk, long n, float *__restrict__ dw)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(long)c * n + i];
+    dw[i] = s;
+}
+
+// bias gradient: db[co] = sum over pixels of dz[.., co] (one block per co)
+__global__ __launch_bounds__(256) void k_db(TV dz, int co_real, float *__restrict__ db)
+{
+    const int co = blockIdx.x;
+    const long npx = (long)dz.B * dz.H * dz.W;
+    float s = 0.f;
+    for (long p = threadIdx.x; p < npx; p += 256) {
+        const int xx = (int)(p % dz.W);
+        const long r = p / dz.W;
+        const int yy = (int)(r % dz.H);
+        const int b = (int)(r / dz.H);
+        s += bf2f(dz.p[dz.idx(b, yy, xx) + co]);
+    }
+    __shared__ float sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = co < co_real ? sh[0] : 0.f;
+}
+
+// FP32 master weights -> BF16 [tap][co][ci] and flipped/transposed
+// BF16 [tap'][ci][co] = W[8 - tap'][co][ci]
+__global__ void k_cvt_w(const float *__restrict__ w, int cinp, int coutp, bf16 *__restrict__ wb,
+                        bf16 *__restrict__ wt)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = 9L * cinp * coutp;
+    if (i >= n) return;
+    const int ci = (int)(i % cinp);
+    const int co = (int)((i / cinp) % coutp);
+    const int tap = (int)(i / ((long)cinp * coutp));
+    const bf16 h = f2bf(w[i]);
+    wb[i] = h;
+    wt[((long)(8 - tap) * cinp + ci) * coutp + co] = h;
+}
+
+// Adam (Kingma & Ba 2015) on FP32 parameters
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, long n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// output gradient of the MSE loss on channel 0, masked to the scored
+// region; partial sums of the squared error per block
+__global__ __launch_bounds__(256) void k_loss(TV out, const float *__restrict__ tgt, const float *__restrict__ wmask,
+                                              TV gout, float scale, float *__restrict__ part)
+{
+    const long n = (long)out.B * out.H * out.W;
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    float e2 = 0.f;
+    if (p < n) {
+        const int xx = (int)(p % out.W);
+        const long r = p / out.W;
+        const int yy = (int)(r % out.H);
+        const int b = (int)(r / out.H);
+        const float wgt = wmask[p];
+        const float d = (bf2f(out.p[out.idx(b, yy, xx)]) - tgt[p]) * wgt;
+        e2 = d * d;
+        const long gi = gout.idx(b, yy, xx);
+        gout.p[gi] = f2bf(2.f * d * scale);
+        for (int c = 1; c < gout.C; ++c) gout.p[gi + c] = 0;
+    }
+    __shared__ float sh[256];
+    sh[threadIdx.x] = e2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// per-grid max |u| and max |energy| (partials per block, then one block)
+__global__ __launch_bounds__(256) void k_absmax2(Geo g, const double *__restrict__ u, const double *__restrict__ e,
+                                                 float *__restrict__ part)
+{
+    float m0 = 0.f, m1 = 0.f;
+    const long n = (long)g.nx * g.ny;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int x = (int)(t % g.nx), y = (int)(t / g.nx);
+        const long i = g.off + (long)y * g.pitch + x;
+        m0 = fmaxf(m0, (float)fabs(u[i]));
+        m1 = fmaxf(m1, (float)fabs(e[i]));
+    }
+    __shared__ float s0[256], s1[256];
+    s0[threadIdx.x] = m0;
+    s1[threadIdx.x] = m1;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s0[threadIdx.x] = fmaxf(s0[threadIdx.x], s0[threadIdx.x + o]);
+            s1[threadIdx.x] = fmaxf(s1[threadIdx.x], s1[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+__global__ void k_absmax_fin(const float *__restrict__ part, int nb, float *__restrict__ scale)
+{
+    if (threadIdx.x != 0) return;
+    float a = 0.f, b = 0.f;
+    for (int i = 0; i < nb; ++i) {
+        a = fmaxf(a, part[2 * i]);
+        b = fmaxf(b, part[2 * i + 1]);
+    }
+    scale[0] = a > 0.f ? a : 1.f;
+    scale[1] = b > 0.f ? b : 1.f;
+}
+
+// network input for the window [x0, x0+W) x [y0, y0+H) of the grid (batch b)
+__global__ void k_input(Geo g, const double *__restrict__ u, const double *__restrict__ rho,
+                        const double *__restrict__ e, const double *__restrict__ kx, const double *__restrict__ ky,
+                        const float *__restrict__ scale, TV in, int b, int x0, int y0)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= in.W || y >= in.H) return;
+    const long i = g.off + (long)(y0 + y) * g.pitch + (x0 + x);
+    bf16 *o = in.p + in.idx(b, y, x);
+    o[0] = f2bf((float)(u[i] / scale[0]));
+    o[1] = f2bf((float)(log10(rho[i]) * 0.5));
+    o[2] = f2bf((float)(e[i] / scale[1]));
+    o[3] = f2bf((float)(log10(1.0 + kx[i]) / 6.0));
+    o[4] = f2bf((float)(log10(1.0 + ky[i]) / 6.0));
+    for (int c = 5; c < 16; ++c) o[c] = 0;
+}
+
+// training target (u_new - u_prev) / s_u for the same window
+__global__ void k_target(Geo g, const double *__restrict__ unew, const double *__restrict__ uprev,
+                         const float *__restrict__ scale, float *__restrict__ tgt, int W, int H, int x0, int y0)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const long i = g.off + (long)(y0 + y) * g.pitch + (x0 + x);
+    tgt[(long)y * W + x] = (float)((unew[i] - uprev[i]) / scale[0]);
+}
+
+// guess = u_prev + s_u * out (FP64) over the interior
+__global__ void k_guess(Geo g, const double *__restrict__ uprev, TV out, const float *__restrict__ scale,
+                        double *__restrict__ guess)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= g.nx || y >= g.ny) return;
+    const long i = g.off + (long)y * g.pitch + x;
+    guess[i] = uprev[i] + (double)scale[0] * (double)bf2f(out.p[out.idx(0, y, x)]);
+}
+
+// ------------------------------------------------------------- host side
+
+struct Tensor {
+    bf16 *p = nullptr;
+    int B = 0, H = 0, W = 0, C = 0;
+    size_t bytes() const { return (size_t)B * (H + 2) * (W + 2) * C * sizeof(bf16); }
+    TV v(int off = 0) const { return TV{p, B, H, W, C, off}; }
+};
+
+struct Acts {
+    int B = 0, H = 0, W = 0;
+    bool grads = false;
+    // forward
+    Tensor x0, e1a, c1, p1, e2a, c2, p2, e3a, e3b, d2a, d2b, d1a, d1b, out;
+    // backward
+    Tensor g_out, g_d1b, g_d1a, g_c1, g_d2b, g_d2a, g_c2, g_e3b, g_e3a, g_p2, g_e2a, g_p1, g_e1a, g_e1b, g_e2b, dz;
+};
+
 }  // namespace tl
 
-extern "C" int tl_unet_param_count(void) { return 0; }
-extern "C" int tl_unet_set_weights(tl_ctx *, const float *) { return TL_E_STATE; }
-extern "C" int tl_unet_get_weights(tl_ctx *, float *) { return TL_E_STATE; }
-extern "C" int tl_unet_record(tl_ctx *, int) { return TL_E_STATE; }
-extern "C" int tl_unet_dataset_size(tl_ctx *) { return 0; }
-extern "C" int tl_unet_train(tl_ctx *, const tl_train_params *, float *) { return TL_E_STATE; }
-extern "C" int tl_unet_predict(tl_ctx *, double, double, int) { return TL_E_STATE; }
+struct tl::Unet {
+    float *w[NL] = {}, *b[NL] = {};
+    float *mw[NL] = {}, *vw[NL] = {}, *mb[NL] = {}, *vb[NL] = {};
+    float *gw[NL] = {}, *gb[NL] = {};
+    bf16 *wb[NL] = {}, *wt[NL] = {};
+    bool weights_set = false;
+    int adam_t = 0;
+    Acts acts;
+    float *scale = nullptr;      // [2]: s_u, s_e
+    float *mx_part = nullptr;    // absmax partials
+    float *dw_part = nullptr;
+    size_t dw_part_len = 0;
+    float *loss_part = nullptr;
+    size_t loss_part_len = 0;
+    // dataset of training samples: inputs (bordered, 16 ch), target, loss mask
+    bf16 *ds_x = nullptr;
+    float *ds_y = nullptr, *ds_m = nullptr;
+    int ds_n = 0, ds_cap = 0, ds_crop = 0, ds_per_step = 0;
+    std::mt19937_64 rng{12345};
+    // batch buffers
+    float *bt_y = nullptr, *bt_m = nullptr;
+    int bt_cap = 0;
+};
+
+namespace tl {
+
+static size_t wcount(int l) { return 9UL * LAY[l].cinp * LAY[l].coutp; }
+
+#define UCHK(c, expr)                                                      \
+    do {                                                                   \
+        hipError_t e_ = (expr);                                            \
+        if (e_ != hipSuccess) {                                            \
+            (c)->err = std::string("unet: ") + #expr + ": " + hipGetErrorString(e_); \
+            return TL_E_HIP;                                               \
+        }                                                                  \
+    } while (0)
+
+static int ualloc(tl_ctx *c, void **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        c->err = "unet: device allocation failed";
+        *p = nullptr;
+        return TL_E_NOMEM;
+    }
+    if (hipMemsetAsync(*p, 0, bytes, c->stream) != hipSuccess) return TL_E_HIP;
+    return TL_OK;
+}
+
+static int unet_get(tl_ctx *c, Unet **out)
+{
+    if (!c->unet) {
+        Unet *u = new Unet();
+        c->unet = u;
+        for (int l = 0; l < NL; ++l) {
+            const size_t nw = wcount(l), nb = LAY[l].coutp;
+            float **fw[] = {&u->w[l], &u->mw[l], &u->vw[l], &u->gw[l]};
+            for (float **q : fw)
+                if (int rc = ualloc(c, (void **)q, nw * sizeof(float))) return rc;
+            float **fb[] = {&u->b[l], &u->mb[l], &u->vb[l], &u->gb[l]};
+            for (float **q : fb)
+                if (int rc = ualloc(c, (void **)q, nb * sizeof(float))) return rc;
+            if (int rc = ualloc(c, (void **)&u->wb[l], nw * sizeof(bf16))) return rc;
+            if (int rc = ualloc(c, (void **)&u->wt[l], nw * sizeof(bf16))) return rc;
+        }
+        if (int rc = ualloc(c, (void **)&u->scale, 2 * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->mx_part, 2 * 1024 * sizeof(float))) return rc;
+    }
+    *out = c->unet;
+    return TL_OK;
+}
+
+static void tfree(Tensor &t)
+{
+    if (t.p) (void)hipFree(t.p);
+    t = Tensor();
+}
+
+static int talloc(tl_ctx *c, Tensor &t, int B, int H, int W, int C)
+{
+    t.B = B;
+    t.H = H;
+    t.W = W;
+    t.C = C;
+    return ualloc(c, (void **)&t.p, t.bytes());
+}
+
+static void acts_free(Acts &a)
+{
+    Tensor *all[] = {&a.x0,    &a.e1a,   &a.c1,    &a.p1,    &a.e2a,   &a.c2,    &a.p2,    &a.e3a,
+                     &a.e3b,   &a.d2a,   &a.d2b,   &a.d1a,   &a.d1b,   &a.out,   &a.g_out, &a.g_d1b,
+                     &a.g_d1a, &a.g_c1,  &a.g_d2b, &a.g_d2a, &a.g_c2,  &a.g_e3b, &a.g_e3a, &a.g_p2,
+                     &a.g_e2a, &a.g_p1,  &a.g_e1a, &a.g_e1b, &a.g_e2b, &a.dz};
+    for (Tensor *t : all) tfree(*t);
+    a.B = a.H = a.W = 0;
+}
+
+// activations (and gradients) for B x H x W; H, W multiples of 4
+static int acts_ensure(tl_ctx *c, Unet *u, int B, int H, int W, bool grads)
+{
+    Acts &a = u->acts;
+    if (a.B == B && a.H == H && a.W == W && (a.grads || !grads)) return TL_OK;
+    acts_free(a);
+    a.B = B;
+    a.H = H;
+    a.W = W;
+    a.grads = grads;
+    const int H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
+    struct {
+        Tensor *t;
+        int h, w, ch;
+    } f[] = {{&a.x0, H, W, 16},   {&a.e1a, H, W, 16},    {&a.c1, H, W, 48},    {&a.p1, H2, W2, 16},
+             {&a.e2a, H2, W2, 32}, {&a.c2, H2, W2, 96},  {&a.p2, H4, W4, 32},  {&a.e3a, H4, W4, 64},
+             {&a.e3b, H4, W4, 64}, {&a.d2a, H2, W2, 32}, {&a.d2b, H2, W2, 32}, {&a.d1a, H, W, 16},
+             {&a.d1b, H, W, 16},   {&a.out, H, W, 16}};
+    for (auto &q : f)
+        if (int rc = talloc(c, *q.t, B, q.h, q.w, q.ch)) return rc;
+    if (grads) {
+        struct {
+            Tensor *t;
+            int h, w, ch;
+        } g[] = {{&a.g_out, H, W, 16},   {&a.g_d1b, H, W, 16},   {&a.g_d1a, H, W, 16},  {&a.g_c1, H, W, 48},
+                 {&a.g_d2b, H2, W2, 32}, {&a.g_d2a, H2, W2, 32}, {&a.g_c2, H2, W2, 96}, {&a.g_e3b, H4, W4, 64},
+                 {&a.g_e3a, H4, W4, 64}, {&a.g_p2, H4, W4, 32},  {&a.g_e2a, H2, W2, 32}, {&a.g_p1, H2, W2, 16},
+                 {&a.g_e1a, H, W, 16},   {&a.g_e1b, H, W, 16},   {&a.g_e2b, H2, W2, 32}, {&a.dz, H, W, 64}};
+        for (auto &q : g)
+            if (int rc = talloc(c, *q.t, B, q.h, q.w, q.ch)) return rc;
+    }
+    return TL_OK;
+}
+
+static unsigned blocks(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+static hipError_t conv(tl_ctx *c, const TV &in, const TV &out, const bf16 *w, const float *bias, int cinp, int coutp,
+                       int relu)
+{
+    const dim3 grid((in.W + 63) / 64, in.H, in.B);
+    switch (coutp) {
+    case 16: k_conv<1><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
+    case 32: k_conv<2><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
+    case 48: k_conv<3><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
+    case 64: k_conv<4><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
+    case 96: k_conv<6><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+static hipError_t fwd_layer(tl_ctx *c, Unet *u, int l, const TV &in, const TV &out)
+{
+    return conv(c, in, out, u->wb[l], u->b[l], LAY[l].cinp, LAY[l].coutp, LAY[l].relu);
+}
+
+static int refresh_weights(tl_ctx *c, Unet *u)
+{
+    for (int l = 0; l < NL; ++l) {
+        const long n = (long)wcount(l);
+        k_cvt_w<<<blocks(n), 256, 0, c->stream>>>(u->w[l], LAY[l].cinp, LAY[l].coutp, u->wb[l], u->wt[l]);
+    }
+    UCHK(c, hipGetLastError());
+    return TL_OK;
+}
+
+// forward pass over acts.x0
+static int forward(tl_ctx *c, Unet *u)
+{
+    Acts &a = u->acts;
+    const long n1 = (long)a.B * a.H * a.W, n2 = n1 / 4;
+    UCHK(c, fwd_layer(c, u, 0, a.x0.v(), a.e1a.v()));
+    UCHK(c, fwd_layer(c, u, 1, a.e1a.v(), a.c1.v(32)));
+    k_pool<<<blocks(n2 * 16), 256, 0, c->stream>>>(a.c1.v(32), a.p1.v(), 16);
+    UCHK(c, fwd_layer(c, u, 2, a.p1.v(), a.e2a.v()));
+    UCHK(c, fwd_layer(c, u, 3, a.e2a.v(), a.c2.v(64)));
+    k_pool<<<blocks(n2 / 4 * 32), 256, 0, c->stream>>>(a.c2.v(64), a.p2.v(), 32);
+    UCHK(c, fwd_layer(c, u, 4, a.p2.v(), a.e3a.v()));
+    UCHK(c, fwd_layer(c, u, 5, a.e3a.v(), a.e3b.v()));
+    k_up<<<blocks(n2 * 64), 256, 0, c->stream>>>(a.e3b.v(), a.c2.v(0), 64);
+    UCHK(c, fwd_layer(c, u, 6, a.c2.v(), a.d2a.v()));
+    UCHK(c, fwd_layer(c, u, 7, a.d2a.v(), a.d2b.v()));
+    k_up<<<blocks(n1 * 32), 256, 0, c->stream>>>(a.d2b.v(), a.c1.v(0), 32);
+    UCHK(c, fwd_layer(c, u, 8, a.c1.v(), a.d1a.v()));
+    UCHK(c, fwd_layer(c, u, 9, a.d1a.v(), a.d1b.v()));
+    UCHK(c, fwd_layer(c, u, 10, a.d1b.v(), a.out.v()));
+    return TL_OK;
+}
+
+// backward of layer l: g = dL/d(output), y = output, x = input; writes the
+// weight/bias gradients and (unless gx is null) dL/d(input) into gx
+static int back_layer(tl_ctx *c, Unet *u, int l, const TV &g, const TV &gp, int has_gp, const TV &y, const TV &x,
+                      const TV *gx)
+{
+    Acts &a = u->acts;
+    const LayerDef &L = LAY[l];
+    TV dz{a.dz.p, y.B, y.H, y.W, L.coutp, 0};
+    const long npx = (long)y.B * y.H * y.W;
+    // the buffer is shared by all layers: clear it so its border is zero
+    UCHK(c, hipMemsetAsync(a.dz.p, 0, (size_t)y.B * (y.H + 2) * (y.W + 2) * L.coutp * sizeof(bf16), c->stream));
+    k_dz<<<blocks(npx * L.coutp), 256, 0, c->stream>>>(g, gp, has_gp, y, L.relu, dz, L.coutp);
+    // weight gradient: partials over pixel chunks, then a fixed-order sum
+    const int px_chunk = 1024;
+    const int nchunk = (int)((npx + px_chunk - 1) / px_chunk);
+    const long npair = (long)L.cinp * L.coutp;
+    const size_t need = (size_t)nchunk * 9 * npair;
+    if (need > u->dw_part_len) {
+        if (u->dw_part) (void)hipFree(u->dw_part);
+        if (int rc = ualloc(c, (void **)&u->dw_part, need * sizeof(float))) return rc;
+        u->dw_part_len = need;
+    }
+    k_dw<<<dim3(blocks(npair), nchunk), 256, 0, c->stream>>>(x, dz, L.cinp, L.coutp, px_chunk, u->dw_part);
+    k_dw_reduce<<<blocks(9 * npair), 256, 0, c->stream>>>(u->dw_part, nchunk, 9 * npair, u->gw[l]);
+    k_db<<<L.coutp, 256, 0, c->stream>>>(dz, L.cout, u->gb[l]);
+    if (gx) UCHK(c, conv(c, dz, *gx, u->wt[l], nullptr, L.coutp, L.cinp, 0));
+    UCHK(c, hipGetLastError());
+    return TL_OK;
+}
+
+static int backward(tl_ctx *c, Unet *u)
+{
+    Acts &a = u->acts;
+    const long n1 = (long)a.B * a.H * a.W, n2 = n1 / 4;
+    const TV none{};
+    TV t;
+#define RB(expr)                   \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != TL_OK) return rc_; \
+    } while (0)
+    t = a.g_d1b.v();
+    RB(back_layer(c, u, 10, a.g_out.v(), none, 0, a.out.v(), a.d1b.v(), &t));
+    t = a.g_d1a.v();
+    RB(back_layer(c, u, 9, a.g_d1b.v(), none, 0, a.d1b.v(), a.d1a.v(), &t));
+    t = a.g_c1.v();
+    RB(back_layer(c, u, 8, a.g_d1a.v(), none, 0, a.d1a.v(), a.c1.v(), &t));
+    k_up_back<<<blocks(n2 * 32), 256, 0, c->stream>>>(a.g_c1.v(0), a.g_d2b.v(), 32);
+    t = a.g_d2a.v();
+    RB(back_layer(c, u, 7, a.g_d2b.v(), none, 0, a.d2b.v(), a.d2a.v(), &t));
+    t = a.g_c2.v();
+    RB(back_layer(c, u, 6, a.g_d2a.v(), none, 0, a.d2a.v(), a.c2.v(), &t));
+    k_up_back<<<blocks(n2 / 4 * 64), 256, 0, c->stream>>>(a.g_c2.v(0), a.g_e3b.v(), 64);
+    t = a.g_e3a.v();
+    RB(back_layer(c, u, 5, a.g_e3b.v(), none, 0, a.e3b.v(), a.e3a.v(), &t));
+    t = a.g_p2.v();
+    RB(back_layer(c, u, 4, a.g_e3a.v(), none, 0, a.e3a.v(), a.p2.v(), &t));
+    // skip2: gradient from the concat plus the pool's
+    t = a.g_e2a.v();
+    RB(back_layer(c, u, 3, a.g_c2.v(64), a.g_p2.v(), 1, a.c2.v(64), a.e2a.v(), &t));
+    t = a.g_p1.v();
+    RB(back_layer(c, u, 2, a.g_e2a.v(), none, 0, a.e2a.v(), a.p1.v(), &t));
+    t = a.g_e1a.v();
+    RB(back_layer(c, u, 1, a.g_c1.v(32), a.g_p1.v(), 1, a.c1.v(32), a.e1a.v(), &t));
+    RB(back_layer(c, u, 0, a.g_e1a.v(), none, 0, a.e1a.v(), a.x0.v(), nullptr));
+#undef RB
+    return TL_OK;
+}
+
+static int adam_step(tl_ctx *c, Unet *u, float lr)
+{
+    u->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - std::pow(b1, (float)u->adam_t), c2 = 1.f - std::pow(b2, (float)u->adam_t);
+    for (int l = 0; l < NL; ++l) {
+        const long nw = (long)wcount(l), nb = LAY[l].coutp;
+        k_adam<<<blocks(nw), 256, 0, c->stream>>>(u->w[l], u->gw[l], u->mw[l], u->vw[l], nw, lr, b1, b2, eps, c1, c2);
+        k_adam<<<blocks(nb), 256, 0, c->stream>>>(u->b[l], u->gb[l], u->mb[l], u->vb[l], nb, lr, b1, b2, eps, c1, c2);
+    }
+    UCHK(c, hipGetLastError());
+    return TL_OK;
+}
+
+// host-side He initialisation (deterministic), padded layout
+static int init_weights(tl_ctx *c, Unet *u, uint64_t seed)
+{
+    std::mt19937_64 g(seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    for (int l = 0; l < NL; ++l) {
+        const LayerDef &L = LAY[l];
+        std::vector<float> w(wcount(l), 0.f);
+        const float sd = std::sqrt((L.relu ? 2.f : 1.f) / (9.f * L.cin));
+        for (int t = 0; t < 9; ++t)
+            for (int co = 0; co < L.cout; ++co)
+                for (int ci = 0; ci < L.cin; ++ci) w[((size_t)t * L.coutp + co) * L.cinp + ci] = nd(g) * sd;
+        UCHK(c, hipMemcpyAsync(u->w[l], w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        UCHK(c, hipMemsetAsync(u->b[l], 0, L.coutp * sizeof(float), c->stream));
+        UCHK(c, hipStreamSynchronize(c->stream));
+    }
+    u->weights_set = true;
+    return refresh_weights(c, u);
+}
+
+static int grid_scales(tl_ctx *c, Unet *u, const double *uprev, const double *energy)
+{
+    const int nb = 1024;
+    k_absmax2<<<nb, 256, 0, c->stream>>>(c->g, uprev, energy, u->mx_part);
+    k_absmax_fin<<<1, 64, 0, c->stream>>>(u->mx_part, nb, u->scale);
+    UCHK(c, hipGetLastError());
+    return TL_OK;
+}
+
+// ------------------------------------------------------------ entry points
+
+int unet_predict_guess(tl_ctx *c, double, double, int, double *guess_dev)
+{
+    Unet *u = c->unet;
+    if (!u || !u->weights_set) {
+        c->err = "learned warm start: no network weights loaded (tl_unet_set_weights)";
+        return TL_E_STATE;
+    }
+    const Geo &g = c->g;
+    const int H = (g.ny + 3) / 4 * 4, W = (g.nx + 3) / 4 * 4;
+    if (H != g.ny || W != g.nx) {
+        c->err = "learned warm start needs nx, ny multiples of 4";
+        return TL_E_INVALID;
+    }
+    if (int rc = acts_ensure(c, u, 1, H, W, false)) return rc;
+    if (int rc = grid_scales(c, u, c->f[TL_F_U], c->f[TL_F_ENERGY])) return rc;
+    k_input<<<dim3(blocks(W), H), 256, 0, c->stream>>>(g, c->f[TL_F_U], c->f[TL_F_DENSITY], c->f[TL_F_ENERGY],
+                                                        c->f[TL_F_KX], c->f[TL_F_KY], u->scale, u->acts.x0.v(), 0, 0,
+                                                        0);
+    UCHK(c, hipGetLastError());
+    if (int rc = forward(c, u)) return rc;
+    k_guess<<<dim3(blocks(g.nx), g.ny), 256, 0, c->stream>>>(g, c->f[TL_F_U], u->acts.out.v(), u->scale, guess_dev);
+    UCHK(c, hipGetLastError());
+    return TL_OK;
+}
+
+void unet_destroy(tl_ctx *c)
+{
+    Unet *u = c->unet;
+    if (!u) return;
+    acts_free(u->acts);
+    for (int l = 0; l < NL; ++l) {
+        void *ps[] = {u->w[l], u->b[l], u->mw[l], u->vw[l], u->mb[l], u->vb[l], u->gw[l], u->gb[l], u->wb[l], u->wt[l]};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+    }
+    void *ps[] = {u->scale, u->mx_part, u->dw_part, u->loss_part, u->ds_x, u->ds_y, u->ds_m, u->bt_y, u->bt_m};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    delete u;
+    c->unet = nullptr;
+}
+
+// After a solve in recording mode: crops of (inputs of this step, target).
+// u0 holds u_prev, u the solution, energy the previous energy.
+int unet_record_step(tl_ctx *c, double, double, int)
+{
+    Unet *u;
+    if (int rc = unet_get(c, &u)) return rc;
+    const Geo &g = c->g;
+    const int crop = u->ds_crop;
+    if (crop <= 0 || crop > g.nx || crop > g.ny || u->ds_n >= u->ds_cap) return TL_OK;
+    if (int rc = grid_scales(c, u, c->f[TL_F_U0], c->f[TL_F_ENERGY])) return rc;
+    const int per_step = std::min(u->ds_cap - u->ds_n, u->ds_per_step);
+    const int margin = 16;
+    std::vector<float> mask((size_t)crop * crop);
+    for (int k = 0; k < per_step; ++k) {
+        const int x0 = (int)(u->rng() % (uint64_t)(g.nx - crop + 1));
+        const int y0 = (int)(u->rng() % (uint64_t)(g.ny - crop + 1));
+        const int s = u->ds_n++;
+        TV in{u->ds_x, u->ds_cap, crop, crop, 16, 0};
+        k_input<<<dim3(blocks(crop), crop), 256, 0, c->stream>>>(g, c->f[TL_F_U0], c->f[TL_F_DENSITY],
+                                                                 c->f[TL_F_ENERGY], c->f[TL_F_KX], c->f[TL_F_KY],
+                                                                 u->scale, in, s, x0, y0);
+        k_target<<<dim3(blocks(crop), crop), 256, 0, c->stream>>>(g, c->f[TL_F_U], c->f[TL_F_U0], u->scale,
+                                                                  u->ds_y + (size_t)s * crop * crop, crop, crop, x0,
+                                                                  y0);
+        // score pixels away from crop edges that are not domain edges (there
+        // the zero padding of the crop differs from the full-grid input)
+        for (int y = 0; y < crop; ++y)
+            for (int x = 0; x < crop; ++x) {
+                const bool okx = (x >= margin || x0 == 0) && (x < crop - margin || x0 + crop == g.nx);
+                const bool oky = (y >= margin || y0 == 0) && (y < crop - margin || y0 + crop == g.ny);
+                mask[(size_t)y * crop + x] = (okx && oky) ? 1.f : 0.f;
+            }
+        UCHK(c, hipMemcpyAsync(u->ds_m + (size_t)s * crop * crop, mask.data(), mask.size() * sizeof(float),
+                               hipMemcpyHostToDevice, c->stream));
+        UCHK(c, hipStreamSynchronize(c->stream));  // mask buffer reuse
+    }
+    UCHK(c, hipGetLastError());
+    return TL_OK;
+}
+
+}  // namespace tl
+
+using namespace tl;
+
+extern "C" int tl_unet_param_count(void)
+{
+    int n = 0;
+    for (int l = 0; l < NL; ++l) n += 9 * LAY[l].cin * LAY[l].cout + LAY[l].cout;
+    return n;
+}
+
+// flat real layout per layer: W[tap][co][ci] (cout x cin real), then b[co]
+extern "C" int tl_unet_set_weights(tl_ctx *c, const float *flat)
+{
+    if (!c) return TL_E_INVALID;
+    UCHK(c, hipSetDevice(c->device));
+    Unet *u;
+    if (int rc = unet_get(c, &u)) return rc;
+    if (!flat) return init_weights(c, u, 0x7ea1ea5ULL);
+    size_t o = 0;
+    for (int l = 0; l < NL; ++l) {
+        const LayerDef &L = LAY[l];
+        std::vector<float> w(wcount(l), 0.f), b(L.coutp, 0.f);
+        for (int t = 0; t < 9; ++t)
+            for (int co = 0; co < L.cout; ++co)
+                for (int ci = 0; ci < L.cin; ++ci) w[((size_t)t * L.coutp + co) * L.cinp + ci] = flat[o++];
+        for (int co = 0; co < L.cout; ++co) b[co] = flat[o++];
+        UCHK(c, hipMemcpyAsync(u->w[l], w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        UCHK(c, hipMemcpyAsync(u->b[l], b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        UCHK(c, hipStreamSynchronize(c->stream));
+    }
+    u->weights_set = true;
+    u->adam_t = 0;
+    return refresh_weights(c, u);
+}
+
+extern "C" int tl_unet_get_weights(tl_ctx *c, float *flat)
+{
+    if (!c || !flat || !c->unet) return TL_E_INVALID;
+    Unet *u = c->unet;
+    size_t o = 0;
+    for (int l = 0; l < NL; ++l) {
+        const LayerDef &L = LAY[l];
+        std::vector<float> w(wcount(l)), b(L.coutp);
+        UCHK(c, hipMemcpyAsync(w.data(), u->w[l], w.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        UCHK(c, hipMemcpyAsync(b.data(), u->b[l], b.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        UCHK(c, hipStreamSynchronize(c->stream));
+        for (int t = 0; t < 9; ++t)
+            for (int co = 0; co < L.cout; ++co)
+                for (int ci = 0; ci < L.cin; ++ci) flat[o++] = w[((size_t)t * L.coutp + co) * L.cinp + ci];
+        for (int co = 0; co < L.cout; ++co) flat[o++] = b[co];
+    }
+    return TL_OK;
+}
+
+// capacity > 0: record crops_per_step crops of 128 x 128 cells after every
+// solve, up to `capacity` samples in all; capacity 0 stops recording (the
+// dataset is kept)
+extern "C" int tl_unet_record(tl_ctx *c, int capacity, int crops_per_step)
+{
+    if (!c || capacity < 0 || crops_per_step < 0) return TL_E_INVALID;
+    UCHK(c, hipSetDevice(c->device));
+    Unet *u;
+    if (int rc = unet_get(c, &u)) return rc;
+    u->ds_per_step = crops_per_step;
+    if (capacity > 0 && !u->ds_x) {
+        u->ds_crop = 128;
+        u->ds_cap = capacity;
+        const size_t px = (size_t)u->ds_crop * u->ds_crop;
+        const size_t xb = (size_t)u->ds_cap * (u->ds_crop + 2) * (u->ds_crop + 2) * 16 * sizeof(bf16);
+        if (int rc = ualloc(c, (void **)&u->ds_x, xb)) return rc;
+        if (int rc = ualloc(c, (void **)&u->ds_y, (size_t)u->ds_cap * px * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->ds_m, (size_t)u->ds_cap * px * sizeof(float))) return rc;
+    }
+    c->recording = capacity > 0;
+    return TL_OK;
+}
+
+extern "C" int tl_unet_dataset_size(tl_ctx *c) { return (c && c->unet) ? c->unet->ds_n : 0; }
+
+// Adam on minibatches of recorded crops; loss_out[i] = mean squared error of
+// step i over the scored pixels
+extern "C" int tl_unet_train(tl_ctx *c, const tl_train_params *tp, float *loss_out)
+{
+    if (!c || !tp || tp->steps < 1 || tp->batch < 1) return TL_E_INVALID;
+    UCHK(c, hipSetDevice(c->device));
+    Unet *u;
+    if (int rc = unet_get(c, &u)) return rc;
+    if (u->ds_n < 1) {
+        c->err = "tl_unet_train: empty dataset (tl_unet_record, then solve)";
+        return TL_E_STATE;
+    }
+    if (!u->weights_set)
+        if (int rc = init_weights(c, u, 0x7ea1ea5ULL + (uint64_t)tp->seed)) return rc;
+    const int crop = u->ds_crop, B = tp->batch;
+    const size_t px = (size_t)crop * crop;
+    if (int rc = acts_ensure(c, u, B, crop, crop, true)) return rc;
+    if (u->bt_cap < B) {
+        if (u->bt_y) (void)hipFree(u->bt_y);
+        if (u->bt_m) (void)hipFree(u->bt_m);
+        if (int rc = ualloc(c, (void **)&u->bt_y, B * px * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->bt_m, B * px * sizeof(float))) return rc;
+        u->bt_cap = B;
+    }
+    const long npx = (long)B * px;
+    const unsigned nlb = blocks(npx);
+    if ((size_t)nlb > u->loss_part_len) {
+        if (u->loss_part) (void)hipFree(u->loss_part);
+        if (int rc = ualloc(c, (void **)&u->loss_part, nlb * sizeof(float))) return rc;
+        u->loss_part_len = nlb;
+    }
+    std::mt19937_64 g(0x5eed0000ULL + (uint64_t)tp->seed);
+    const size_t xs = (size_t)(crop + 2) * (crop + 2) * 16;
+    std::vector<float> lp(nlb), mk(npx);
+    for (int step = 0; step < tp->steps; ++step) {
+        // gather a batch
+        for (int b = 0; b < B; ++b) {
+            const int s = (int)(g() % (uint64_t)u->ds_n);
+            UCHK(c, hipMemcpyAsync(u->acts.x0.p + b * xs, u->ds_x + (size_t)s * xs, xs * sizeof(bf16),
+                                   hipMemcpyDeviceToDevice, c->stream));
+            UCHK(c, hipMemcpyAsync(u->bt_y + b * px, u->ds_y + (size_t)s * px, px * sizeof(float),
+                                   hipMemcpyDeviceToDevice, c->stream));
+            UCHK(c, hipMemcpyAsync(u->bt_m + b * px, u->ds_m + (size_t)s * px, px * sizeof(float),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        }
+        if (int rc = forward(c, u)) return rc;
+        // scale = 1 / (number of scored pixels) (from the masks of the batch)
+        UCHK(c, hipMemcpyAsync(mk.data(), u->bt_m, npx * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        UCHK(c, hipStreamSynchronize(c->stream));
+        double nsc = 0;
+        for (long i = 0; i < npx; ++i) nsc += mk[i];
+        const float scale = nsc > 0 ? (float)(1.0 / nsc) : 0.f;
+        k_loss<<<nlb, 256, 0, c->stream>>>(u->acts.out.v(), u->bt_y, u->bt_m, u->acts.g_out.v(), scale, u->loss_part);
+        UCHK(c, hipGetLastError());
+        if (int rc = backward(c, u)) return rc;
+        if (int rc = adam_step(c, u, tp->lr)) return rc;
+        if (int rc = refresh_weights(c, u)) return rc;
+        if (loss_out) {
+            UCHK(c, hipMemcpyAsync(lp.data(), u->loss_part, nlb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+            UCHK(c, hipStreamSynchronize(c->stream));
+            double s = 0;
+            for (float v : lp) s += v;
+            loss_out[step] = (float)(s * scale);
+        }
+    }
+    UCHK(c, hipStreamSynchronize(c->stream));
+    return TL_OK;
+}
+
+// prediction of the next u (FP64) into the TL_F_MI field, from the current
+// u (energy*density after CG.init!) -- as the warm start computes it
+extern "C" int tl_unet_predict(tl_ctx *c, double rx, double ry, int coefficient)
+{
+    if (!c) return TL_E_INVALID;
+    UCHK(c, hipSetDevice(c->device));
+    if (int rc = launch_init_coef(c, rx, ry, coefficient, INIT_CG)) return rc == hipSuccess ? TL_OK : TL_E_HIP;
+    if (int rc = unet_predict_guess(c, rx, ry, coefficient, c->f[TL_F_MI])) return rc;
+    UCHK(c, hipStreamSynchronize(c->stream));
+    return TL_OK;
+}
+
+// Diagnostics for the parity tests: forward / loss+gradient of the network
+// on host inputs x[B][H][W][5] (FP32, already normalised), target[B][H][W].
+extern "C" int tl_unet_eval(tl_ctx *c, const float *x, const float *target, int B, int H, int W, float *out,
+                            float *loss, float *grads)
+{
+    if (!c || !x || B < 1 || H < 4 || W < 4 || H % 4 || W % 4) return TL_E_INVALID;
+    UCHK(c, hipSetDevice(c->device));
+    Unet *u;
+    if (int rc = unet_get(c, &u)) return rc;
+    if (!u->weights_set) {
+        c->err = "tl_unet_eval: no weights";
+        return TL_E_STATE;
+    }
+    const bool want_grad = target && grads;
+    if (int rc = acts_ensure(c, u, B, H, W, want_grad)) return rc;
+    const size_t npx = (size_t)B * H * W;
+    std::vector<bf16> hx(u->acts.x0.bytes() / sizeof(bf16), 0);
+    auto tobf = [](float f) {
+        uint32_t v;
+        std::memcpy(&v, &f, 4);
+        v += 0x7FFFu + ((v >> 16) & 1u);
+        return (bf16)(v >> 16);
+    };
+    for (int b = 0; b < B; ++b)
+        for (int y = 0; y < H; ++y)
+            for (int xx = 0; xx < W; ++xx)
+                for (int ch = 0; ch < 5; ++ch)
+                    hx[((((size_t)b * (H + 2) + y + 1) * (W + 2)) + xx + 1) * 16 + ch] =
+                        tobf(x[(((size_t)b * H + y) * W + xx) * 5 + ch]);
+    UCHK(c, hipMemcpyAsync(u->acts.x0.p, hx.data(), hx.size() * sizeof(bf16), hipMemcpyHostToDevice, c->stream));
+    if (int rc = forward(c, u)) return rc;
+    std::vector<bf16> ho(u->acts.out.bytes() / sizeof(bf16));
+    UCHK(c, hipMemcpyAsync(ho.data(), u->acts.out.p, ho.size() * sizeof(bf16), hipMemcpyDeviceToHost, c->stream));
+    UCHK(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < B; ++b)
+        for (int y = 0; y < H; ++y)
+            for (int xx = 0; xx < W; ++xx) {
+                const uint32_t v = (uint32_t)ho[((((size_t)b * (H + 2) + y + 1) * (W + 2)) + xx + 1) * 16] << 16;
+                float f;
+                std::memcpy(&f, &v, 4);
+                if (out) out[((size_t)b * H + y) * W + xx] = f;
+            }
+    if (!want_grad) return TL_OK;
+    float *dy, *dm, *lp;
+    const unsigned nlb = blocks((long)npx);
+    if (hipMalloc((void **)&dy, npx * sizeof(float)) != hipSuccess) return TL_E_NOMEM;
+    if (hipMalloc((void **)&dm, npx * sizeof(float)) != hipSuccess) return TL_E_NOMEM;
+    if (hipMalloc((void **)&lp, nlb * sizeof(float)) != hipSuccess) return TL_E_NOMEM;
+    std::vector<float> ones(npx, 1.f), lph(nlb);
+    UCHK(c, hipMemcpyAsync(dy, target, npx * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    UCHK(c, hipMemcpyAsync(dm, ones.data(), npx * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const float scale = 1.f / (float)npx;
+    k_loss<<<nlb, 256, 0, c->stream>>>(u->acts.out.v(), dy, dm, u->acts.g_out.v(), scale, lp);
+    UCHK(c, hipGetLastError());
+    if (int rc = backward(c, u)) return rc;
+    UCHK(c, hipMemcpyAsync(lph.data(), lp, nlb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    UCHK(c, hipStreamSynchronize(c->stream));
+    double s = 0;
+    for (float v : lph) s += v;
+    if (loss) *loss = (float)(s * scale);
+    size_t o = 0;
+    for (int l = 0; l < NL; ++l) {
+        const LayerDef &L = LAY[l];
+        std::vector<float> w(wcount(l)), b(L.coutp);
+        UCHK(c, hipMemcpyAsync(w.data(), u->gw[l], w.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        UCHK(c, hipMemcpyAsync(b.data(), u->gb[l], b.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        UCHK(c, hipStreamSynchronize(c->stream));
+        for (int t = 0; t < 9; ++t)
+            for (int co = 0; co < L.cout; ++co)
+                for (int ci = 0; ci < L.cin; ++ci) grads[o++] = w[((size_t)t * L.coutp + co) * L.cinp + ci];
+        for (int co = 0; co < L.cout; ++co) grads[o++] = b[co];
+    }
+    (void)hipFree(dy);
+    (void)hipFree(dm);
+    (void)hipFree(lp);
+    return TL_OK;
+}

@@ -31,7 +31,7 @@ EXPORTS = [
     "tl_cheby_coef", "tl_cheby_calciter", "tl_apply_operator", "tl_get_cg_history", "tl_unet_param_count",
     "tl_unet_set_weights", "tl_unet_get_weights", "tl_unet_record", "tl_unet_dataset_size", "tl_unet_train",
     "tl_unet_predict", "tl_profile_begin", "tl_profile_read", "tl_profile_end",
-    "tl_kernel_repeat",
+    "tl_kernel_repeat", "tl_unet_eval",
 ]
 TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32 = range(6)
 KERNEL_NAMES = {0: "k_cg_w<double>", 1: "k_cg_ur<double>", 2: "k_cg_p<double>", 3: "k_cg_w<float>",
@@ -133,10 +133,11 @@ def load():
     L.tl_unet_param_count.argtypes = []
     L.tl_unet_set_weights.argtypes = [vp, fp]
     L.tl_unet_get_weights.argtypes = [vp, fp]
-    L.tl_unet_record.argtypes = [vp, C.c_int]
+    L.tl_unet_record.argtypes = [vp, C.c_int, C.c_int]
     L.tl_unet_dataset_size.argtypes = [vp]
     L.tl_unet_train.argtypes = [vp, C.POINTER(TrainParams), fp]
     L.tl_unet_predict.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+    L.tl_unet_eval.argtypes = [vp, fp, fp, C.c_int, C.c_int, C.c_int, fp, fp, fp]
     L.tl_profile_begin.argtypes = [vp, C.c_uint]
     L.tl_profile_read.argtypes = [vp, C.c_int, dp, C.c_int, ip]
     L.tl_profile_end.argtypes = [vp]

@@ -1,0 +1,85 @@
+"""GPU parity of the warm-start U-Net (BF16 activations and weights, FP32
+accumulation on MFMA) against the FP64 NumPy oracle (oracle/unet_ref.py).
+
+Tolerances are those of BF16 storage: inputs and weights are rounded to
+BF16 on both sides, so the remaining difference is the rounding of the
+activations and gradients between layers (relative ~2^-8 per rounding).
+"""
+import numpy as np
+import pytest
+
+import tealeaf_hip as T
+import unet_ref as U
+from tealeaf_hip import decks, unet
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def chunk():
+    s = T.Settings.from_text(decks.reference_deck(16, 1))
+    return T.initialiseapp(s)
+
+
+def _case(seed, B=2, H=16, W=24):
+    rng = np.random.default_rng(seed)
+    p = U.init_params(seed).astype(np.float32)
+    x = rng.standard_normal((B, H, W, 5)).astype(np.float32)
+    x16 = np.zeros((B, H, W, 16))
+    x16[..., :5] = bf16(x)
+    return p, x, x16, rng
+
+
+def test_param_count():
+    assert unet.param_count() == U.param_count() == 118689
+
+
+@pytest.mark.parametrize("shape", [(2, 16, 24), (1, 32, 32), (1, 8, 72)])
+def test_forward_matches_oracle(chunk, shape):
+    print(f"case: unet forward {shape}", flush=True)
+    p, x, x16, _ = _case(3, *shape)
+    unet.set_weights(chunk, p)
+    out = unet.evaluate(chunk, x)
+    ref, _ = U.forward(bf16(p), x16)
+    rel = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert rel < 2e-2, rel
+    np.testing.assert_array_equal(unet.get_weights(chunk), p)
+
+
+def test_gradients_match_oracle(chunk):
+    print("case: unet gradients 2x16x24", flush=True)
+    p, x, x16, rng = _case(5)
+    unet.set_weights(chunk, p)
+    target = rng.standard_normal(x.shape[:3]) * 0.3
+    out, loss, g = unet.evaluate(chunk, x, target)
+    L, gref = U.loss_and_grads(bf16(p), x16, target)
+    assert abs(loss - L) / L < 2e-2
+    cos = float(np.dot(g, gref) / (np.linalg.norm(g) * np.linalg.norm(gref)))
+    assert cos > 0.999, cos
+    assert np.linalg.norm(g - gref) / np.linalg.norm(gref) < 5e-2
+    # per layer: the largest gradients agree too
+    o = 0
+    for ci, _, co, _, _ in U.LAYERS:
+        n = 9 * ci * co + co
+        a, b = g[o:o + n], gref[o:o + n]
+        assert np.linalg.norm(a - b) <= 8e-2 * np.linalg.norm(b) + 1e-6, (ci, co)
+        o += n
+
+
+def test_records_and_trains():
+    print("case: unet record 256^2 x 4 steps, train 80 steps", flush=True)
+    s = T.Settings.from_text(decks.random_rect_deck(21, n=256, steps=4))
+    c = T.initialiseapp(s)
+    unet.record(c, capacity=64, crops_per_step=8)
+    T.diffuse(c, s, 4)
+    assert unet.dataset_size(c) == 32
+    loss = unet.train(c, steps=80, batch=8, lr=2e-3)
+    assert np.all(np.isfinite(loss))
+    assert np.mean(loss[-10:]) < 0.5 * np.mean(loss[:5]), (loss[:5], loss[-10:])

@@ -210,7 +210,8 @@ int tl_kernel_repeat(tl_ctx *ctx, int kernel_id, int reps);
 
 /* --------------------------------------------------- learned warm start */
 /* BF16 U-Net (3 levels, 16/32/64 channels, 3x3 convs, FP32 accumulation)
- * predicting the timestep's change of u from (u_prev, density, energy, Kx, Ky).
+ * predicting the timestep's change of u from (u_prev, density, energy, Kx, Ky)
+ * and the residual r0 = u_prev - A u_prev of the previous-step guess.
  * No counterpart in the reference. */
 typedef struct {
     int steps;            /* optimiser steps */
@@ -218,7 +219,8 @@ typedef struct {
     int crop;             /* crop edge in cells (multiple of 8) */
     float lr;             /* Adam learning rate */
     int seed;
-    int reserved[8];
+    int reserved[8];      /* reserved[0]: loss on the error e = pred - target: 0 = residual ||A e||^2,
+                             1 = squared error ||e||^2, 2 = energy norm e^T A e (what CG minimises) */
 } tl_train_params;
 
 /* number of FP32 parameters of the network (flat layout per layer:
@@ -238,7 +240,7 @@ int tl_unet_train(tl_ctx *ctx, const tl_train_params *tp, float *loss_out);
 /* the warm-start prediction of u for the current fields into TL_F_MI */
 int tl_unet_predict(tl_ctx *ctx, double rx, double ry, int coefficient);
 /* diagnostics: network output (and, with target and grads, the MSE loss and
- * the gradient w.r.t. the flat parameters) for host inputs x[B][H][W][5] */
+ * the gradient w.r.t. the flat parameters) for host inputs x[B][H][W][6] */
 int tl_unet_eval(tl_ctx *ctx, const float *x, const float *target, int B, int H, int W, float *out, float *loss,
                  float *grads);
 

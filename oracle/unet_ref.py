@@ -4,7 +4,8 @@ TEST INFRASTRUCTURE ONLY: the checker for the HIP BF16/MFMA kernels of
 libtealeaf_hip.  The network has no counterpart in the reference; this file
 fixes its definition:
 
-  16-channel input (5 used: u_prev, density, energy, Kx, Ky, normalised)
+  16-channel input (6 used: u_prev, density, energy, Kx, Ky and the residual
+  of the previous-step guess, normalised)
   enc1: conv3x3 16->16, ReLU; conv3x3 16->16, ReLU            (skip1, HxW)
   avg-pool 2x2
   enc2: conv3x3 16->32, ReLU; conv3x3 32->32, ReLU            (skip2, H/2)
@@ -27,7 +28,7 @@ import numpy as np
 
 # (cin_real, cin_pad, cout_real, cout_pad, relu)
 LAYERS = [
-    (5, 16, 16, 16, True), (16, 16, 16, 16, True),
+    (6, 16, 16, 16, True), (16, 16, 16, 16, True),
     (16, 16, 32, 32, True), (32, 32, 32, 32, True),
     (32, 32, 64, 64, True), (64, 64, 64, 64, True),
     (96, 96, 32, 32, True), (32, 32, 32, 32, True),

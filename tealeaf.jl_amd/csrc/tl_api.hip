@@ -22,6 +22,7 @@ using namespace tl;
 
 namespace tl {
 int unet_predict_guess(tl_ctx *c, double rx, double ry, int coefficient, double *guess_dev);
+hipError_t launch_smooth(tl_ctx *c, const double *x, double *xout, double omega);
 void unet_destroy(tl_ctx *c);
 int unet_record_step(tl_ctx *c, double rx, double ry, int coefficient);
 }  // namespace tl
@@ -445,6 +446,13 @@ static int choose_guess(tl_ctx *c, const tl_solve_params *p, double rx, double r
     if (p->warm_start == TL_WARM_LEARNED) {
         if (!c->guess) RETIF(alloc_d(c, &c->guess, c->g.elems));
         RETIF(unet_predict_guess(c, rx, ry, p->coefficient, c->guess));
+        // damped Jacobi sweeps on A x = b (b = u) remove the prediction's
+        // high-frequency error, which CG would otherwise spend its first
+        // iterations on and which dominates the residual acceptance test
+        for (int s = 0; s < 3; ++s) {
+            HIPCHK(c, launch_smooth(c, c->guess, c->u_alt, 0.7));
+            std::swap(c->guess, c->u_alt);
+        }
     } else if (p->warm_start == TL_WARM_GUESS) {
         if (!c->guess) {
             c->err = "TL_WARM_GUESS without tl_set_guess";

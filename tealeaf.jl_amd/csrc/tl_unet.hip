@@ -2,10 +2,12 @@
 // device from the solver's own trajectories.  No counterpart in the
 // reference; the network is defined by this file:
 //
-//   input: 16 channels (5 used), per cell of the grid, normalised
+//   input: 16 channels (6 used), per cell of the grid, normalised
 //          c0 = u_prev / s_u   c1 = log10(rho) / 2   c2 = energy / s_e
 //          c3 = log10(1 + Kx) / 6   c4 = log10(1 + Ky) / 6
-//          s_u = max|u_prev|, s_e = max|energy| over the grid
+//          c5 = r0 / s_r with r0 = u_prev - A u_prev, the residual of the
+//               previous-step guess (the step's right-hand side is u_prev)
+//          s_u, s_e, s_r = max |u_prev|, |energy|, |r0| over the grid
 //   enc1  conv3x3 16->16 ReLU, conv3x3 16->16 ReLU           (skip1, H x W)
 //   avg-pool 2x2
 //   enc2  conv3x3 16->32 ReLU, conv3x3 32->32 ReLU           (skip2, H/2)
@@ -13,7 +15,9 @@
 //   mid   conv3x3 32->64 ReLU, conv3x3 64->64 ReLU           (H/4)
 //   nearest x2, concat [up 64 | skip2 32] -> dec2 conv 96->32 ReLU, 32->32 ReLU
 //   nearest x2, concat [up 32 | skip1 16] -> dec1 conv 48->16 ReLU, 16->16 ReLU
-//   out   conv3x3 16->1 (linear): the predicted (u_new - u_prev) / s_u
+//   out   conv3x3 16->1 (linear): the predicted (u_new - u_prev) / s_r
+// The change of u over a step is A^-1 r0, linear in r0: scaling by s_r makes
+// every step of a trajectory look alike to the network.
 //
 // Activations are BF16 NHWC tensors with a one-cell zero border (the conv's
 // zero padding needs no bounds checks); weights are BF16 copies of FP32
@@ -47,7 +51,7 @@ struct LayerDef {
     int cin, cinp, cout, coutp, relu;
 };
 static const LayerDef LAY[NL] = {
-    {5, 16, 16, 16, 1},  {16, 16, 16, 16, 1}, {16, 16, 32, 32, 1}, {32, 32, 32, 32, 1},
+    {6, 16, 16, 16, 1},  {16, 16, 16, 16, 1}, {16, 16, 32, 32, 1}, {32, 32, 32, 32, 1},
     {32, 32, 64, 64, 1}, {64, 64, 64, 64, 1}, {96, 96, 32, 32, 1}, {32, 32, 32, 32, 1},
     {48, 48, 16, 16, 1}, {16, 16, 16, 16, 1}, {16, 16, 1, 16, 0},
 };
@@ -321,51 +325,209 @@ __global__ __launch_bounds__(256) void k_loss(TV out, const float *__restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// per-grid max |u| and max |energy| (partials per block, then one block)
-__global__ __launch_bounds__(256) void k_absmax2(Geo g, const double *__restrict__ u, const double *__restrict__ e,
-                                                 float *__restrict__ part)
+// face coefficients of a window: k[(y*W + x)*2 + {0,1}] = kx(x,y), ky(x,y)
+__global__ void k_coefs(Geo g, const double *__restrict__ kx, const double *__restrict__ ky, float *__restrict__ k,
+                        int W, int H, int x0, int y0)
 {
-    float m0 = 0.f, m1 = 0.f;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const long i = g.off + (long)(y0 + y) * g.pitch + (x0 + x);
+    k[((long)y * W + x) * 2 + 0] = (float)kx[i];
+    k[((long)y * W + x) * 2 + 1] = (float)ky[i];
+}
+
+// (A v)_i on a crop of the operator (smvp of kernels.jl:178-184): kx, ky
+// per pixel (left / bottom faces); at a crop edge that is a domain edge the
+// reflected neighbour equals v_i (edges: bit 0 left, 1 right, 2 bottom, 3 top);
+// at other crop edges the missing neighbour is taken as 0
+__device__ __forceinline__ float crop_apply(const float *__restrict__ v, const float *__restrict__ k, int x, int y,
+                                            int W, int H, int edges)
+{
+    const long i = (long)y * W + x;
+    const float vc = v[i];
+    const float kxl = k[2 * i], kyd = k[2 * i + 1];
+    const float kxr = x + 1 < W ? k[2 * (i + 1)] : kxl;
+    const float kyu = y + 1 < H ? k[2 * (i + W) + 1] : kyd;
+    const float vl = x > 0 ? v[i - 1] : ((edges & 1) ? vc : 0.f);
+    const float vr = x + 1 < W ? v[i + 1] : ((edges & 2) ? vc : 0.f);
+    const float vd = y > 0 ? v[i - W] : ((edges & 4) ? vc : 0.f);
+    const float vu = y + 1 < H ? v[i + W] : ((edges & 8) ? vc : 0.f);
+    return (1.f + kxl + kxr + kyd + kyu) * vc - kxl * vl - kxr * vr - kyd * vd - kyu * vu;
+}
+
+// residual loss, step 1: e = out - target on every pixel of the batch
+__global__ void k_res_e(TV out, const float *__restrict__ tgt, float *__restrict__ e)
+{
+    const long n = (long)out.B * out.H * out.W;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int xx = (int)(p % out.W);
+    const long r = p / out.W;
+    const int yy = (int)(r % out.H);
+    const int b = (int)(r / out.H);
+    e[p] = bf2f(out.p[out.idx(b, yy, xx)]) - tgt[p];
+}
+
+// step 2: r = mask * (A e) -- the residual of the predicted guess, in units
+// of s_u, on the scored pixels; partial sums of r^2 per block
+__global__ __launch_bounds__(256) void k_res_r(const float *__restrict__ e, const float *__restrict__ k,
+                                               const float *__restrict__ mask, const int *__restrict__ edges, int B,
+                                               int H, int W, float *__restrict__ r, float *__restrict__ part)
+{
+    const long n = (long)B * H * W;
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    float r2 = 0.f;
+    if (p < n) {
+        const int x = (int)(p % W);
+        const long q = p / W;
+        const int y = (int)(q % H);
+        const int b = (int)(q / H);
+        const long o = (long)b * H * W;
+        const float m = mask[p];
+        const float v = m != 0.f ? m * crop_apply(e + o, k + 2 * o, x, y, W, H, edges[b]) : 0.f;
+        r[p] = v;
+        r2 = v * v;
+    }
+    __shared__ float sh[256];
+    sh[threadIdx.x] = r2;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (threadIdx.x < s2) sh[threadIdx.x] += sh[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// step 3: dL/d(out) = 2 scale A r (A symmetric), into channel 0 of gout
+__global__ void k_res_g(const float *__restrict__ r, const float *__restrict__ k, const int *__restrict__ edges,
+                        TV gout, float scale)
+{
+    const long n = (long)gout.B * gout.H * gout.W;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int W = gout.W, H = gout.H;
+    const int x = (int)(p % W);
+    const long q = p / W;
+    const int y = (int)(q % H);
+    const int b = (int)(q / H);
+    const long o = (long)b * H * W;
+    const float g = 2.f * scale * crop_apply(r + o, k + 2 * o, x, y, W, H, edges[b]);
+    const long gi = gout.idx(b, y, x);
+    gout.p[gi] = f2bf(g);
+    for (int c = 1; c < gout.C; ++c) gout.p[gi + c] = 0;
+}
+
+// energy-norm loss: e (already masked) ; r = A e ; partial sums of e . r
+__global__ __launch_bounds__(256) void k_en_r(const float *__restrict__ e, const float *__restrict__ k,
+                                              const int *__restrict__ edges, int B, int H, int W,
+                                              float *__restrict__ r, float *__restrict__ part)
+{
+    const long n = (long)B * H * W;
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    float s = 0.f;
+    if (p < n) {
+        const int x = (int)(p % W);
+        const long q = p / W;
+        const int y = (int)(q % H);
+        const int b = (int)(q / H);
+        const long o = (long)b * H * W;
+        const float v = crop_apply(e + o, k + 2 * o, x, y, W, H, edges[b]);
+        r[p] = v;
+        s = e[p] * v;
+    }
+    __shared__ float sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (threadIdx.x < s2) sh[threadIdx.x] += sh[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// dL/d(out) = 2 scale mask (A (mask e)) into channel 0 of gout
+__global__ void k_en_g(const float *__restrict__ r, const float *__restrict__ mask, TV gout, float scale)
+{
+    const long n = (long)gout.B * gout.H * gout.W;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int W = gout.W, H = gout.H;
+    const int x = (int)(p % W);
+    const long q = p / W;
+    const int y = (int)(q % H);
+    const int b = (int)(q / H);
+    const long gi = gout.idx(b, y, x);
+    gout.p[gi] = f2bf(2.f * scale * mask[p] * r[p]);
+    for (int c = 1; c < gout.C; ++c) gout.p[gi + c] = 0;
+}
+
+// masked error e = mask * (out - target)
+__global__ void k_err_m(TV out, const float *__restrict__ tgt, const float *__restrict__ mask, float *__restrict__ e)
+{
+    const long n = (long)out.B * out.H * out.W;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int xx = (int)(p % out.W);
+    const long r = p / out.W;
+    const int yy = (int)(r % out.H);
+    const int b = (int)(r / out.H);
+    e[p] = mask[p] * (bf2f(out.p[out.idx(b, yy, xx)]) - tgt[p]);
+}
+
+// per-grid max |u|, |energy|, |r0| (partials per block, then one block)
+__global__ __launch_bounds__(256) void k_absmax3(Geo g, const double *__restrict__ u, const double *__restrict__ e,
+                                                 const double *__restrict__ r0, float *__restrict__ part)
+{
+    float m[3] = {0.f, 0.f, 0.f};
     const long n = (long)g.nx * g.ny;
     for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
         const int x = (int)(t % g.nx), y = (int)(t / g.nx);
         const long i = g.off + (long)y * g.pitch + x;
-        m0 = fmaxf(m0, (float)fabs(u[i]));
-        m1 = fmaxf(m1, (float)fabs(e[i]));
+        m[0] = fmaxf(m[0], (float)fabs(u[i]));
+        m[1] = fmaxf(m[1], (float)fabs(e[i]));
+        m[2] = fmaxf(m[2], (float)fabs(r0[i]));
     }
-    __shared__ float s0[256], s1[256];
-    s0[threadIdx.x] = m0;
-    s1[threadIdx.x] = m1;
+    __shared__ float sh[3][256];
+    for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = m[a];
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            s0[threadIdx.x] = fmaxf(s0[threadIdx.x], s0[threadIdx.x + o]);
-            s1[threadIdx.x] = fmaxf(s1[threadIdx.x], s1[threadIdx.x + o]);
-        }
+        if (threadIdx.x < o)
+            for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = fmaxf(sh[a][threadIdx.x], sh[a][threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = s0[0];
-        part[2 * blockIdx.x + 1] = s1[0];
-    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; ++a) part[3 * blockIdx.x + a] = sh[a][0];
 }
 
 __global__ void k_absmax_fin(const float *__restrict__ part, int nb, float *__restrict__ scale)
 {
     if (threadIdx.x != 0) return;
-    float a = 0.f, b = 0.f;
-    for (int i = 0; i < nb; ++i) {
-        a = fmaxf(a, part[2 * i]);
-        b = fmaxf(b, part[2 * i + 1]);
+    for (int a = 0; a < 3; ++a) {
+        float v = 0.f;
+        for (int i = 0; i < nb; ++i) v = fmaxf(v, part[3 * i + a]);
+        scale[a] = v > 0.f ? v : 1.f;
     }
-    scale[0] = a > 0.f ? a : 1.f;
-    scale[1] = b > 0.f ? b : 1.f;
+}
+
+// r0 = b - A x with b = x = u_prev: the residual of the previous-step guess
+__global__ void k_r0(Geo g, const double *__restrict__ u, const double *__restrict__ kx,
+                     const double *__restrict__ ky, double *__restrict__ r0)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= g.nx || y >= g.ny) return;
+    const long i = g.off + (long)y * g.pitch + x;
+    const double c = u[i];
+    const double l = x > 0 ? u[i - 1] : c, r = x < g.nx - 1 ? u[i + 1] : c;
+    const double d = y > 0 ? u[i - g.pitch] : c, up = y < g.ny - 1 ? u[i + g.pitch] : c;
+    const double kxl = kx[i], kxr = kx[i + 1], kyd = ky[i], kyu = ky[i + g.pitch];
+    const double au = (1.0 + kxr + kxl + kyu + kyd) * c - (kxr * r + kxl * l) - (kyu * up + kyd * d);
+    r0[i] = c - au;
 }
 
 // network input for the window [x0, x0+W) x [y0, y0+H) of the grid (batch b)
 __global__ void k_input(Geo g, const double *__restrict__ u, const double *__restrict__ rho,
                         const double *__restrict__ e, const double *__restrict__ kx, const double *__restrict__ ky,
-                        const float *__restrict__ scale, TV in, int b, int x0, int y0)
+                        const double *__restrict__ r0, const float *__restrict__ scale, TV in, int b, int x0, int y0)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= in.W || y >= in.H) return;
@@ -376,27 +538,28 @@ __global__ void k_input(Geo g, const double *__restrict__ u, const double *__res
     o[2] = f2bf((float)(e[i] / scale[1]));
     o[3] = f2bf((float)(log10(1.0 + kx[i]) / 6.0));
     o[4] = f2bf((float)(log10(1.0 + ky[i]) / 6.0));
-    for (int c = 5; c < 16; ++c) o[c] = 0;
+    o[5] = f2bf((float)(r0[i] / scale[2]));
+    for (int c = 6; c < 16; ++c) o[c] = 0;
 }
 
-// training target (u_new - u_prev) / s_u for the same window
+// training target (u_new - u_prev) / s_r for the same window
 __global__ void k_target(Geo g, const double *__restrict__ unew, const double *__restrict__ uprev,
                          const float *__restrict__ scale, float *__restrict__ tgt, int W, int H, int x0, int y0)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= W || y >= H) return;
     const long i = g.off + (long)(y0 + y) * g.pitch + (x0 + x);
-    tgt[(long)y * W + x] = (float)((unew[i] - uprev[i]) / scale[0]);
+    tgt[(long)y * W + x] = (float)((unew[i] - uprev[i]) / scale[2]);
 }
 
-// guess = u_prev + s_u * out (FP64) over the interior
+// guess = u_prev + s_r * out (FP64) over the interior
 __global__ void k_guess(Geo g, const double *__restrict__ uprev, TV out, const float *__restrict__ scale,
                         double *__restrict__ guess)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= g.nx || y >= g.ny) return;
     const long i = g.off + (long)y * g.pitch + x;
-    guess[i] = uprev[i] + (double)scale[0] * (double)bf2f(out.p[out.idx(0, y, x)]);
+    guess[i] = uprev[i] + (double)scale[2] * (double)bf2f(out.p[out.idx(0, y, x)]);
 }
 
 // ------------------------------------------------------------- host side
@@ -427,7 +590,7 @@ struct tl::Unet {
     bool weights_set = false;
     int adam_t = 0;
     Acts acts;
-    float *scale = nullptr;      // [2]: s_u, s_e
+    float *scale = nullptr;      // [3]: s_u, s_e, s_r
     float *mx_part = nullptr;    // absmax partials
     float *dw_part = nullptr;
     size_t dw_part_len = 0;
@@ -435,11 +598,13 @@ struct tl::Unet {
     size_t loss_part_len = 0;
     // dataset of training samples: inputs (bordered, 16 ch), target, loss mask
     bf16 *ds_x = nullptr;
-    float *ds_y = nullptr, *ds_m = nullptr;
+    float *ds_y = nullptr, *ds_m = nullptr, *ds_k = nullptr;
+    std::vector<int> ds_edges;
     int ds_n = 0, ds_cap = 0, ds_crop = 0, ds_per_step = 0;
     std::mt19937_64 rng{12345};
     // batch buffers
-    float *bt_y = nullptr, *bt_m = nullptr;
+    float *bt_y = nullptr, *bt_m = nullptr, *bt_k = nullptr, *bt_e = nullptr, *bt_r = nullptr;
+    int *bt_edges = nullptr;
     int bt_cap = 0;
 };
 
@@ -483,8 +648,8 @@ static int unet_get(tl_ctx *c, Unet **out)
             if (int rc = ualloc(c, (void **)&u->wb[l], nw * sizeof(bf16))) return rc;
             if (int rc = ualloc(c, (void **)&u->wt[l], nw * sizeof(bf16))) return rc;
         }
-        if (int rc = ualloc(c, (void **)&u->scale, 2 * sizeof(float))) return rc;
-        if (int rc = ualloc(c, (void **)&u->mx_part, 2 * 1024 * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->scale, 4 * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->mx_part, 3 * 1024 * sizeof(float))) return rc;
     }
     *out = c->unet;
     return TL_OK;
@@ -707,10 +872,12 @@ static int init_weights(tl_ctx *c, Unet *u, uint64_t seed)
     return refresh_weights(c, u);
 }
 
-static int grid_scales(tl_ctx *c, Unet *u, const double *uprev, const double *energy)
+// r0 of u_prev into `r0`, then the grid scales s_u, s_e, s_r
+static int grid_scales(tl_ctx *c, Unet *u, const double *uprev, const double *energy, double *r0)
 {
     const int nb = 1024;
-    k_absmax2<<<nb, 256, 0, c->stream>>>(c->g, uprev, energy, u->mx_part);
+    k_r0<<<dim3(blocks(c->g.nx), c->g.ny), 256, 0, c->stream>>>(c->g, uprev, c->f[TL_F_KX], c->f[TL_F_KY], r0);
+    k_absmax3<<<nb, 256, 0, c->stream>>>(c->g, uprev, energy, r0, u->mx_part);
     k_absmax_fin<<<1, 64, 0, c->stream>>>(u->mx_part, nb, u->scale);
     UCHK(c, hipGetLastError());
     return TL_OK;
@@ -732,10 +899,11 @@ int unet_predict_guess(tl_ctx *c, double, double, int, double *guess_dev)
         return TL_E_INVALID;
     }
     if (int rc = acts_ensure(c, u, 1, H, W, false)) return rc;
-    if (int rc = grid_scales(c, u, c->f[TL_F_U], c->f[TL_F_ENERGY])) return rc;
+    // r0 goes to the u_alt scratch field (free during a CG solve)
+    if (int rc = grid_scales(c, u, c->f[TL_F_U], c->f[TL_F_ENERGY], c->u_alt)) return rc;
     k_input<<<dim3(blocks(W), H), 256, 0, c->stream>>>(g, c->f[TL_F_U], c->f[TL_F_DENSITY], c->f[TL_F_ENERGY],
-                                                        c->f[TL_F_KX], c->f[TL_F_KY], u->scale, u->acts.x0.v(), 0, 0,
-                                                        0);
+                                                        c->f[TL_F_KX], c->f[TL_F_KY], c->u_alt, u->scale,
+                                                        u->acts.x0.v(), 0, 0, 0);
     UCHK(c, hipGetLastError());
     if (int rc = forward(c, u)) return rc;
     k_guess<<<dim3(blocks(g.nx), g.ny), 256, 0, c->stream>>>(g, c->f[TL_F_U], u->acts.out.v(), u->scale, guess_dev);
@@ -753,7 +921,8 @@ void unet_destroy(tl_ctx *c)
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
-    void *ps[] = {u->scale, u->mx_part, u->dw_part, u->loss_part, u->ds_x, u->ds_y, u->ds_m, u->bt_y, u->bt_m};
+    void *ps[] = {u->scale, u->mx_part, u->dw_part, u->loss_part, u->ds_x, u->ds_y, u->ds_m, u->ds_k,
+                  u->bt_y, u->bt_m, u->bt_k, u->bt_e, u->bt_r, u->bt_edges};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete u;
@@ -769,7 +938,7 @@ int unet_record_step(tl_ctx *c, double, double, int)
     const Geo &g = c->g;
     const int crop = u->ds_crop;
     if (crop <= 0 || crop > g.nx || crop > g.ny || u->ds_n >= u->ds_cap) return TL_OK;
-    if (int rc = grid_scales(c, u, c->f[TL_F_U0], c->f[TL_F_ENERGY])) return rc;
+    if (int rc = grid_scales(c, u, c->f[TL_F_U0], c->f[TL_F_ENERGY], c->u_alt)) return rc;
     const int per_step = std::min(u->ds_cap - u->ds_n, u->ds_per_step);
     const int margin = 16;
     std::vector<float> mask((size_t)crop * crop);
@@ -780,10 +949,15 @@ int unet_record_step(tl_ctx *c, double, double, int)
         TV in{u->ds_x, u->ds_cap, crop, crop, 16, 0};
         k_input<<<dim3(blocks(crop), crop), 256, 0, c->stream>>>(g, c->f[TL_F_U0], c->f[TL_F_DENSITY],
                                                                  c->f[TL_F_ENERGY], c->f[TL_F_KX], c->f[TL_F_KY],
-                                                                 u->scale, in, s, x0, y0);
+                                                                 c->u_alt, u->scale, in, s, x0, y0);
         k_target<<<dim3(blocks(crop), crop), 256, 0, c->stream>>>(g, c->f[TL_F_U], c->f[TL_F_U0], u->scale,
                                                                   u->ds_y + (size_t)s * crop * crop, crop, crop, x0,
                                                                   y0);
+        k_coefs<<<dim3(blocks(crop), crop), 256, 0, c->stream>>>(g, c->f[TL_F_KX], c->f[TL_F_KY],
+                                                                 u->ds_k + (size_t)s * 2 * crop * crop, crop, crop,
+                                                                 x0, y0);
+        u->ds_edges[s] = (x0 == 0 ? 1 : 0) | (x0 + crop == g.nx ? 2 : 0) | (y0 == 0 ? 4 : 0) |
+                         (y0 + crop == g.ny ? 8 : 0);
         // score pixels away from crop edges that are not domain edges (there
         // the zero padding of the crop differs from the full-grid input)
         for (int y = 0; y < crop; ++y)
@@ -873,6 +1047,8 @@ extern "C" int tl_unet_record(tl_ctx *c, int capacity, int crops_per_step)
         if (int rc = ualloc(c, (void **)&u->ds_x, xb)) return rc;
         if (int rc = ualloc(c, (void **)&u->ds_y, (size_t)u->ds_cap * px * sizeof(float))) return rc;
         if (int rc = ualloc(c, (void **)&u->ds_m, (size_t)u->ds_cap * px * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->ds_k, (size_t)u->ds_cap * 2 * px * sizeof(float))) return rc;
+        u->ds_edges.assign(u->ds_cap, 0);
     }
     c->recording = capacity > 0;
     return TL_OK;
@@ -898,12 +1074,19 @@ extern "C" int tl_unet_train(tl_ctx *c, const tl_train_params *tp, float *loss_o
     const size_t px = (size_t)crop * crop;
     if (int rc = acts_ensure(c, u, B, crop, crop, true)) return rc;
     if (u->bt_cap < B) {
-        if (u->bt_y) (void)hipFree(u->bt_y);
-        if (u->bt_m) (void)hipFree(u->bt_m);
+        void *old[] = {u->bt_y, u->bt_m, u->bt_k, u->bt_e, u->bt_r, u->bt_edges};
+        for (void *q : old)
+            if (q) (void)hipFree(q);
         if (int rc = ualloc(c, (void **)&u->bt_y, B * px * sizeof(float))) return rc;
         if (int rc = ualloc(c, (void **)&u->bt_m, B * px * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->bt_k, 2 * B * px * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->bt_e, B * px * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->bt_r, B * px * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->bt_edges, B * sizeof(int))) return rc;
         u->bt_cap = B;
     }
+    const int loss_kind = tp->reserved[0];  // 0 residual, 1 mse, 2 energy norm
+    std::vector<int> edges(B);
     const long npx = (long)B * px;
     const unsigned nlb = blocks(npx);
     if ((size_t)nlb > u->loss_part_len) {
@@ -918,6 +1101,9 @@ extern "C" int tl_unet_train(tl_ctx *c, const tl_train_params *tp, float *loss_o
         // gather a batch
         for (int b = 0; b < B; ++b) {
             const int s = (int)(g() % (uint64_t)u->ds_n);
+            edges[b] = u->ds_edges[s];
+            UCHK(c, hipMemcpyAsync(u->bt_k + 2 * b * px, u->ds_k + (size_t)s * 2 * px, 2 * px * sizeof(float),
+                                   hipMemcpyDeviceToDevice, c->stream));
             UCHK(c, hipMemcpyAsync(u->acts.x0.p + b * xs, u->ds_x + (size_t)s * xs, xs * sizeof(bf16),
                                    hipMemcpyDeviceToDevice, c->stream));
             UCHK(c, hipMemcpyAsync(u->bt_y + b * px, u->ds_y + (size_t)s * px, px * sizeof(float),
@@ -932,7 +1118,23 @@ extern "C" int tl_unet_train(tl_ctx *c, const tl_train_params *tp, float *loss_o
         double nsc = 0;
         for (long i = 0; i < npx; ++i) nsc += mk[i];
         const float scale = nsc > 0 ? (float)(1.0 / nsc) : 0.f;
-        k_loss<<<nlb, 256, 0, c->stream>>>(u->acts.out.v(), u->bt_y, u->bt_m, u->acts.g_out.v(), scale, u->loss_part);
+        if (loss_kind == 2) {
+            // energy norm of the error: L = scale * e^T A e with e = mask (out - target)
+            UCHK(c, hipMemcpyAsync(u->bt_edges, edges.data(), B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            k_err_m<<<nlb, 256, 0, c->stream>>>(u->acts.out.v(), u->bt_y, u->bt_m, u->bt_e);
+            k_en_r<<<nlb, 256, 0, c->stream>>>(u->bt_e, u->bt_k, u->bt_edges, B, crop, crop, u->bt_r, u->loss_part);
+            k_en_g<<<nlb, 256, 0, c->stream>>>(u->bt_r, u->bt_m, u->acts.g_out.v(), scale);
+        } else if (loss_kind == 0) {
+            // minimise the residual of the guess: L = scale * sum_scored (A (out - target))^2
+            UCHK(c, hipMemcpyAsync(u->bt_edges, edges.data(), B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            k_res_e<<<nlb, 256, 0, c->stream>>>(u->acts.out.v(), u->bt_y, u->bt_e);
+            k_res_r<<<nlb, 256, 0, c->stream>>>(u->bt_e, u->bt_k, u->bt_m, u->bt_edges, B, crop, crop, u->bt_r,
+                                                 u->loss_part);
+            k_res_g<<<nlb, 256, 0, c->stream>>>(u->bt_r, u->bt_k, u->bt_edges, u->acts.g_out.v(), scale);
+        } else {
+            k_loss<<<nlb, 256, 0, c->stream>>>(u->acts.out.v(), u->bt_y, u->bt_m, u->acts.g_out.v(), scale,
+                                               u->loss_part);
+        }
         UCHK(c, hipGetLastError());
         if (int rc = backward(c, u)) return rc;
         if (int rc = adam_step(c, u, tp->lr)) return rc;
@@ -962,7 +1164,7 @@ extern "C" int tl_unet_predict(tl_ctx *c, double rx, double ry, int coefficient)
 }
 
 // Diagnostics for the parity tests: forward / loss+gradient of the network
-// on host inputs x[B][H][W][5] (FP32, already normalised), target[B][H][W].
+// on host inputs x[B][H][W][6] (FP32, already normalised), target[B][H][W].
 extern "C" int tl_unet_eval(tl_ctx *c, const float *x, const float *target, int B, int H, int W, float *out,
                             float *loss, float *grads)
 {
@@ -987,9 +1189,9 @@ extern "C" int tl_unet_eval(tl_ctx *c, const float *x, const float *target, int 
     for (int b = 0; b < B; ++b)
         for (int y = 0; y < H; ++y)
             for (int xx = 0; xx < W; ++xx)
-                for (int ch = 0; ch < 5; ++ch)
+                for (int ch = 0; ch < 6; ++ch)
                     hx[((((size_t)b * (H + 2) + y + 1) * (W + 2)) + xx + 1) * 16 + ch] =
-                        tobf(x[(((size_t)b * H + y) * W + xx) * 5 + ch]);
+                        tobf(x[(((size_t)b * H + y) * W + xx) * 6 + ch]);
     UCHK(c, hipMemcpyAsync(u->acts.x0.p, hx.data(), hx.size() * sizeof(bf16), hipMemcpyHostToDevice, c->stream));
     if (int rc = forward(c, u)) return rc;
     std::vector<bf16> ho(u->acts.out.bytes() / sizeof(bf16));

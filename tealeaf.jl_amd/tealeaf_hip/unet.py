@@ -58,16 +58,20 @@ def dataset_size(chunk) -> int:
     return int(load().tl_unet_dataset_size(chunk._ctx))
 
 
-def train(chunk, steps: int, batch: int = 16, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+def train(chunk, steps: int, batch: int = 16, lr: float = 1e-3, seed: int = 0, loss: str = "energy") -> np.ndarray:
+    """Adam steps on the error e = pred - (u_new - u_prev)/s_u over the crops:
+    loss "energy" minimises e^T A e (the norm CG minimises), "residual"
+    ||A e||^2, "mse" ||e||^2."""
     tp = _lib.TrainParams()
     tp.steps, tp.batch, tp.crop, tp.lr, tp.seed = steps, batch, 128, lr, seed
-    loss = np.zeros(steps, dtype=np.float32)
-    check(load().tl_unet_train(chunk._ctx, C.byref(tp), _fp(loss)), chunk._ctx, "tl_unet_train")
-    return loss
+    tp.reserved[0] = {"residual": 0, "mse": 1, "energy": 2}[loss]
+    out = np.zeros(steps, dtype=np.float32)
+    check(load().tl_unet_train(chunk._ctx, C.byref(tp), _fp(out)), chunk._ctx, "tl_unet_train")
+    return out
 
 
 def evaluate(chunk, x: np.ndarray, target: Optional[np.ndarray] = None):
-    """Network output for x[B, H, W, 5]; with a target also the MSE loss and
+    """Network output for x[B, H, W, 6]; with a target also the MSE loss and
     the flat gradient (diagnostics for the parity tests)."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     B, H, W, _ = x.shape

@@ -31,14 +31,14 @@ def chunk():
 def _case(seed, B=2, H=16, W=24):
     rng = np.random.default_rng(seed)
     p = U.init_params(seed).astype(np.float32)
-    x = rng.standard_normal((B, H, W, 5)).astype(np.float32)
+    x = rng.standard_normal((B, H, W, 6)).astype(np.float32)
     x16 = np.zeros((B, H, W, 16))
-    x16[..., :5] = bf16(x)
+    x16[..., :6] = bf16(x)
     return p, x, x16, rng
 
 
 def test_param_count():
-    assert unet.param_count() == U.param_count() == 118689
+    assert unet.param_count() == U.param_count() == 118833
 
 
 @pytest.mark.parametrize("shape", [(2, 16, 24), (1, 32, 32), (1, 8, 72)])

@@ -10,7 +10,7 @@ def _case(seed, B=2, H=8, W=12):
     rng = np.random.default_rng(seed)
     p = U.init_params(seed)
     x = np.zeros((B, H, W, 16))
-    x[..., :5] = rng.standard_normal((B, H, W, 5))
+    x[..., :6] = rng.standard_normal((B, H, W, 6))
     t = rng.standard_normal((B, H, W)) * 0.1
     return p, x, t
 

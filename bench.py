@@ -37,10 +37,14 @@ import tealeaf_hip as T  # noqa: E402
 from tealeaf_hip import _lib, decks  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
-# algorithmic bytes per interior cell per launch (arrays read once + written once)
+# algorithmic bytes per interior cell per launch (arrays read once + written once).
+# The CG iteration is two kernels: PW = [p = r + beta p; w = A p] (reads p, r,
+# Kx, Ky; writes p', w) and UR = [u += alpha p; r -= alpha w] (reads u, p, r, w;
+# writes u, r).  FP64: 8-byte words; FP32-IR inner CG: 4-byte words.
+# P is only the final p update of a solve (FP64 path).
 KERNEL_BYTES = {
-    _lib.TL_K_CG_W64: 32, _lib.TL_K_CG_UR64: 48, _lib.TL_K_CG_P64: 24,
-    _lib.TL_K_CG_W32: 16, _lib.TL_K_CG_UR32: 24, _lib.TL_K_CG_P32: 12,
+    _lib.TL_K_CG_W64: 48, _lib.TL_K_CG_UR64: 48, _lib.TL_K_CG_P64: 24,
+    _lib.TL_K_CG_W32: 24, _lib.TL_K_CG_UR32: 24, _lib.TL_K_CG_P32: 12,
 }
 WEIGHTS = os.path.join(ROOT, "tealeaf.jl_amd", "weights", "unet_warmstart.npy")
 

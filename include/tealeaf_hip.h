@@ -189,13 +189,15 @@ int tl_apply_operator(tl_ctx *ctx, int field_id, double rx, double ry, int coeff
 int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 
 /* ------------------------------------------------------------ profiling */
-/* kernel ids for tl_profile_begin */
-#define TL_K_CG_W64 0
-#define TL_K_CG_UR64 1
-#define TL_K_CG_P64 2
-#define TL_K_CG_W32 3
-#define TL_K_CG_UR32 4
-#define TL_K_CG_P32 5
+/* kernel ids for tl_profile_begin.  A CG iteration is two kernels:
+ * PW = end of the previous iteration (beta, stop test, p = r + beta p) fused
+ * with w = A p and <p, w>; UR = u += alpha p, r -= alpha w and <r, r>. */
+#define TL_K_CG_W64 0  /* FP64 PW (also the unfused w = A p of Chebyshev/PPCG's CG presteps) */
+#define TL_K_CG_UR64 1 /* FP64 UR */
+#define TL_K_CG_P64 2  /* FP64 p = r + beta p: a solve's last iteration, and the CG presteps */
+#define TL_K_CG_W32 3  /* FP32 PW of the inner CG */
+#define TL_K_CG_UR32 4 /* FP32 UR of the inner CG */
+#define TL_K_CG_P32 5  /* unused: folded into TL_K_CG_W32 */
 /* Bracket every launch of the kernels in kernel_mask (bit = 1 << TL_K_*)
  * with HIP events on the context's stream; tl_profile_read returns one
  * kernel's per-launch durations (ms, launch order; launches made after a

@@ -122,7 +122,8 @@ extern "C" int tl_create(tl_ctx **out, int nx, int ny, int halo_depth, double xm
     for (int i = 0; i < TL_NFIELDS && rc == TL_OK; ++i) rc = alloc_d(c, &c->f[i], c->g.elems);
     if (rc == TL_OK) rc = alloc_d(c, &c->u_alt, c->g.elems);
     if (rc == TL_OK) rc = alloc_d(c, &c->sd_alt, c->g.elems);
-    c->partials_len = (size_t)7 * c->g.gx * c->g.gy;  // pw | pr0 | pr1 | 4 x misc
+    if (rc == TL_OK) rc = alloc_d(c, &c->p_alt, c->g.elems);
+    c->partials_len = (size_t)7 * c->g.nbmax;  // pw | pr0 | pr1 | 4 x misc
     if (rc == TL_OK) rc = alloc_d(c, &c->partials, c->partials_len);
     if (rc == TL_OK && hipMalloc((void **)&c->ctrl, sizeof(Ctrl)) != hipSuccess) rc = TL_E_NOMEM;
     if (rc == TL_OK && hipMemset(c->ctrl, 0, sizeof(Ctrl)) != hipSuccess) rc = TL_E_HIP;
@@ -151,7 +152,7 @@ extern "C" void tl_destroy(tl_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     unet_destroy(c);
-    void *bufs[] = {c->u_alt, c->sd_alt, c->guess, c->kx32, c->ky32, c->p32, c->r32, c->w32, c->x32,
+    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->kx32, c->ky32, c->p32, c->r32, c->w32, c->x32,
                     c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
     for (int i = 0; i < TL_NFIELDS; ++i)
         if (c->f[i]) (void)hipFree(c->f[i]);
@@ -319,6 +320,7 @@ static int ensure_fp32(tl_ctx *c)
     RETIF(alloc_f(c, &c->kx32, n));
     RETIF(alloc_f(c, &c->ky32, n));
     RETIF(alloc_f(c, &c->p32, n));
+    RETIF(alloc_f(c, &c->p32b, n));
     RETIF(alloc_f(c, &c->r32, n));
     RETIF(alloc_f(c, &c->w32, n));
     RETIF(alloc_f(c, &c->x32, n));
@@ -500,10 +502,25 @@ static int solve_cg64(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
     HIPCHK(c, launch_cg_init_resid(c, guess));
     Ctrl k;
     int t = 0;
-    RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return cg64_iteration(c, ++t); }, &k));
+    // fused iterations: PW(t) = [end of t-1: beta, stop test, p = r + beta p] + w = A p; UR(t)
+    RETIF(run_until_stop(c, 64, (long)p->max_iters + 16,
+                         [&]() -> hipError_t {
+                             ++t;
+                             hipError_t e = launch_cg_pw64(c, t);
+                             return e == hipSuccess ? launch_cg_urv64(c, t) : e;
+                         },
+                         &k));
+    const int T = k.stop_iter;
+    if (T >= 1) {
+        // the last iteration's CG.p!, so p ends as the reference leaves it
+        HIPCHK(c, launch_cg_pfinal64(c, T));
+        double *pT = ((T - 1) & 1) ? c->p_alt : c->f[TL_F_P];
+        if (pT != c->f[TL_F_P]) std::swap(c->f[TL_F_P], c->p_alt);
+        RETIF(read_ctrl(c, &k));
+    }
     st->init_rel_resid = k.bb > 0 ? std::sqrt(k.sums[0] / k.bb) : 0.0;
-    st->iterations = st->cg_iters = k.stop_iter;
-    c->last_cg_iters = k.stop_iter;
+    st->iterations = st->cg_iters = T;
+    c->last_cg_iters = T;
     fill_stats(k, st);
     HIPCHK(c, launch_halo(c, TL_X_U | TL_X_P, 1, nullptr, 0));
     return status_of(c, k);
@@ -684,11 +701,11 @@ static int solve_jacobi(tl_ctx *c, const tl_solve_params *p, double rx, double r
     return status_of(c, k);
 }
 
+// one fused FP32 iteration: PW(t) (end of t-1 + w = A p) and UR(t)
 static hipError_t cg32_iteration(tl_ctx *c, int t)
 {
     hipError_t e = launch_cg_w32(c, t);
     if (e == hipSuccess) e = launch_cg_ur32(c, t);
-    if (e == hipSuccess) e = launch_cg_p32(c, t);
     return e;
 }
 
@@ -798,18 +815,27 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
         HIPCHK(c, launch_ir_prepare(c));
         HIPCHK(c, launch_ir_residual(c, 1, 1));
     }
-    // one live iteration so the W and UR partials exist, then repeats of one
-    // kernel with a fixed iteration number (its inputs stay valid)
-    HIPCHK(c, kernel_id >= TL_K_CG_W32 ? cg32_iteration(c, 1) : cg64_iteration(c, 1));
+    // two live iterations so every partial-sum array exists, then repeats of
+    // one kernel of iteration 3 (its inputs stay valid: it reads only
+    // buffers that kernel does not write)
+    if (kernel_id >= TL_K_CG_W32) {
+        HIPCHK(c, cg32_iteration(c, 1));
+        HIPCHK(c, cg32_iteration(c, 2));
+    } else {
+        HIPCHK(c, launch_cg_pw64(c, 1));
+        HIPCHK(c, launch_cg_urv64(c, 1));
+        HIPCHK(c, launch_cg_pw64(c, 2));
+        HIPCHK(c, launch_cg_urv64(c, 2));
+    }
     for (int i = 0; i < reps; ++i) {
         hipError_t e = hipSuccess;
         switch (kernel_id) {
-        case TL_K_CG_W64: e = launch_cg_w64(c, 2); break;
-        case TL_K_CG_UR64: e = launch_cg_ur64(c, 2); break;
-        case TL_K_CG_P64: e = launch_cg_p64(c, 2); break;
-        case TL_K_CG_W32: e = launch_cg_w32(c, 2); break;
-        case TL_K_CG_UR32: e = launch_cg_ur32(c, 2); break;
-        default: e = launch_cg_p32(c, 2); break;
+        case TL_K_CG_W64: e = launch_cg_pw64(c, 3); break;
+        case TL_K_CG_UR64: e = launch_cg_urv64(c, 3); break;
+        case TL_K_CG_P64: e = launch_cg_p64(c, 3); break;
+        case TL_K_CG_W32: e = launch_cg_w32(c, 3); break;
+        case TL_K_CG_UR32: e = launch_cg_ur32(c, 3); break;
+        default: break;
         }
         HIPCHK(c, e);
     }

@@ -24,6 +24,21 @@ namespace tl {
 
 constexpr int TB = 256;        // threads per block (one row segment of 256 cells)
 constexpr int XCOL = 32;       // storage column of the first interior cell
+// vectorised CG kernels: rows per load batch, and the block count their
+// geometry aims at on large grids
+#ifndef TL_RV
+#define TL_RV 4
+#endif
+#ifndef TL_VEC_BLOCKS
+#define TL_VEC_BLOCKS 2048
+#endif
+// non-temporal (streaming) vector loads / stores in those kernels
+#ifndef TL_NT_LOAD
+#define TL_NT_LOAD 0
+#endif
+#ifndef TL_NT_STORE
+#define TL_NT_STORE 0
+#endif
 
 struct Geo {
     int nx, ny, hd;
@@ -31,6 +46,9 @@ struct Geo {
     long pitch;                // elements per stored row
     long off;                  // element offset of interior (0, 0)
     int gx, gy, ty;            // stencil launch: gx*gy blocks of TB columns x ty rows
+    int gx4, gy4, ty4;         // FP32 vector kernels: blocks of 4*TB columns x ty4 rows
+    int gx2, gy2, ty2;         // FP64 vector kernels: blocks of 2*TB columns x ty2 rows
+    int nbmax;                 // stride of the partial-sum arrays (max block count)
     size_t elems;              // elements per allocation
 };
 
@@ -83,6 +101,9 @@ struct tl_ctx {
     // reductions
     double *partials = nullptr;
     size_t partials_len = 0;
+    unsigned nb_w = 0, nb_r[2] = {0, 0}, nb_m = 0;  // block counts of the last producer of each partial array
+    double *p_alt = nullptr;   // ping-pong twin of p for the fused CG kernels (FP64)
+    float *p32b = nullptr;     // same for FP32
     tl::Ctrl *ctrl = nullptr;        // device
     tl::Ctrl *ctrl_h = nullptr;      // pinned host mirror
     double *hist_a = nullptr, *hist_b = nullptr;  // device CG alpha/beta history
@@ -120,6 +141,9 @@ hipError_t launch_init_coef(tl_ctx *c, double rx, double ry, int coefficient, in
 hipError_t launch_halo(tl_ctx *c, unsigned mask, int depth, double *const *extra, int nextra);
 hipError_t launch_cg_init_resid(tl_ctx *c, const double *guess);
 hipError_t launch_resid_norm_slot(tl_ctx *c, const double *b, const double *x, int slot);
+hipError_t launch_cg_pw64(tl_ctx *c, int t);
+hipError_t launch_cg_urv64(tl_ctx *c, int t);
+hipError_t launch_cg_pfinal64(tl_ctx *c, int t);
 hipError_t launch_cg_w64(tl_ctx *c, int t);
 hipError_t launch_cg_ur64(tl_ctx *c, int t);
 hipError_t launch_cg_p64(tl_ctx *c, int t);

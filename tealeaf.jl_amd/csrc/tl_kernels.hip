@@ -23,6 +23,7 @@
 // records the scalars in the device control block for the host.  A stopped
 // solve is expressed by Ctrl::stop_iter: launches for later iterations exit
 // at once.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -51,6 +52,20 @@ Geo make_geo(int nx, int ny, int hd)
     if (ty > 1024) ty = 1024;
     g.ty = ty;
     g.gy = (ny + ty - 1) / ty;
+    // vector kernels: the same target block count, 4 (FP32) or 2 (FP64) columns per thread
+    auto vec = [&](int v, int &gxv, int &gyv, int &tyv) {
+        gxv = (nx + v * TB - 1) / (v * TB);
+        const long r2 = (long)ny * gxv;
+        int t2 = (int)((r2 + TL_VEC_BLOCKS - 1) / TL_VEC_BLOCKS);
+        t2 = (t2 + TL_RV - 1) / TL_RV * TL_RV;  // a multiple of the vector kernels' row chunk
+        if (t2 < TL_RV) t2 = TL_RV;
+        if (t2 > 1024) t2 = 1024;
+        tyv = t2;
+        gyv = (ny + t2 - 1) / t2;
+    };
+    vec(4, g.gx4, g.gy4, g.ty4);
+    vec(2, g.gx2, g.gy2, g.ty2);
+    g.nbmax = std::max(g.gx * g.gy, std::max(g.gx4 * g.gy4, g.gx2 * g.gy2));
     return g;
 }
 
@@ -309,19 +324,19 @@ __global__ __launch_bounds__(TB) void k_set_mode(Ctrl *c, int mode, int maxiters
 enum FinKind : int { FK_SUMS = 0, FK_INIT = 1, FK_BB = 2, FK_CHEBY = 3, FK_JACOBI = 4, FK_IR = 5 };
 
 __global__ __launch_bounds__(TB) void k_fin(Ctrl *__restrict__ c, const double *__restrict__ part, unsigned nb,
-                                            int n, int slot, int kind, int t)
+                                            unsigned stride, int n, int slot, int kind, int t)
 {
     if ((kind == FK_CHEBY || kind == FK_JACOBI) && t > c->stop_iter) return;
     double tot[2] = {0.0, 0.0};
     if (n >= 2) {
-        sum_partials<2>(part, nb, nb, tot);
+        sum_partials<2>(part, nb, stride, tot);
     } else {
         double t1[1];
-        sum_partials<1>(part, nb, nb, t1);
+        sum_partials<1>(part, nb, stride, t1);
         tot[0] = t1[0];
     }
     double t34[2] = {0.0, 0.0};
-    if (n == 4) sum_partials<2>(part + 2 * (size_t)nb, nb, nb, t34);
+    if (n == 4) sum_partials<2>(part + 2 * (size_t)stride, nb, stride, t34);
     if (threadIdx.x != 0) return;
     const double v[4] = {tot[0], tot[1], t34[0], t34[1]};
     for (int a = 0; a < n && slot + a < 4; ++a) c->sums[slot + a] = v[a];
@@ -501,8 +516,7 @@ __global__ __launch_bounds__(TB) void k_cg_init_resid(Geo g, double *__restrict_
             }
         }
     }
-    const unsigned nb = nblocks(g);
-    write_partials<2>(acc, pm, nb);
+    write_partials<2>(acc, pm, g.nbmax);
     if (threadIdx.x == 0) pr0[block_id()] = pm[block_id()];
 }
 
@@ -532,17 +546,18 @@ __global__ __launch_bounds__(TB) void k_cg_w(Geo g, int t, const T *__restrict__
             }
         }
     }
-    write_partials<1>(acc, pw, nblocks(g));
+    write_partials<1>(acc, pw, g.nbmax);
 }
 
 // alpha = rro / pw (CG.jl:35), derived identically by every block
 __device__ __forceinline__ bool cg_alpha(const Geo &g, int t, Ctrl *__restrict__ ctrl,
-                                         const double *__restrict__ pw, const double *__restrict__ prprev,
+                                         const double *__restrict__ pw, unsigned nbw,
+                                         const double *__restrict__ prprev, unsigned nbp,
                                          double *__restrict__ hist_a, double *alpha)
 {
     double a1[1], a2[1];
-    sum_partials<1>(pw, nblocks(g), nblocks(g), a1);
-    sum_partials<1>(prprev, nblocks(g), nblocks(g), a2);
+    sum_partials<1>(pw, nbw, g.nbmax, a1);
+    sum_partials<1>(prprev, nbp, g.nbmax, a2);
     const double pwv = a1[0], rro = a2[0];
     const double a = rro / pwv;
     const bool ok = pwv > 0.0 && isfinite(a);
@@ -564,12 +579,13 @@ __device__ __forceinline__ bool cg_alpha(const Geo &g, int t, Ctrl *__restrict__
 template <typename T>
 __global__ __launch_bounds__(TB) void k_cg_ur(Geo g, int t, T *__restrict__ u, const T *__restrict__ p,
                                               T *__restrict__ r, const T *__restrict__ w, Ctrl *__restrict__ ctrl,
-                                              const double *__restrict__ pw, const double *__restrict__ prprev,
+                                              const double *__restrict__ pw, unsigned nbw,
+                                              const double *__restrict__ prprev, unsigned nbp,
                                               double *__restrict__ prcur, double *__restrict__ hist_a)
 {
     if (t > ctrl->stop_iter) return;
     double ad;
-    if (!cg_alpha(g, t, ctrl, pw, prprev, hist_a, &ad)) return;
+    if (!cg_alpha(g, t, ctrl, pw, nbw, prprev, nbp, hist_a, &ad)) return;
     const T alpha = (T)ad;
     const Col c = column(g);
     double acc[1] = {0.0};
@@ -592,18 +608,19 @@ __global__ __launch_bounds__(TB) void k_cg_ur(Geo g, int t, T *__restrict__ u, c
             }
         }
     }
-    write_partials<1>(acc, prcur, nblocks(g));
+    write_partials<1>(acc, prcur, g.nbmax);
 }
 
 // beta = rrn / rro (CG.jl:39) and the stop test, derived identically by
 // every block; block 0 records them
 __device__ __forceinline__ double cg_beta(const Geo &g, int t, Ctrl *__restrict__ ctrl,
-                                          const double *__restrict__ prprev, const double *__restrict__ prcur,
+                                          const double *__restrict__ prprev, unsigned nbp,
+                                          const double *__restrict__ prcur, unsigned nbc,
                                           double *__restrict__ hist_b)
 {
     double a1[1], a2[1];
-    sum_partials<1>(prcur, nblocks(g), nblocks(g), a1);
-    sum_partials<1>(prprev, nblocks(g), nblocks(g), a2);
+    sum_partials<1>(prcur, nbc, g.nbmax, a1);
+    sum_partials<1>(prprev, nbp, g.nbmax, a2);
     const double rrn = a1[0], rro = a2[0];
     const double beta = rrn / rro;
     if (block_id() == 0 && threadIdx.x == 0) {
@@ -620,11 +637,12 @@ __device__ __forceinline__ double cg_beta(const Geo &g, int t, Ctrl *__restrict_
 // CG.p!, CG.jl:101-104: p = beta p + r (runs in the iteration that stops)
 template <typename T>
 __global__ __launch_bounds__(TB) void k_cg_p(Geo g, int t, T *__restrict__ p, const T *__restrict__ r,
-                                             Ctrl *__restrict__ ctrl, const double *__restrict__ prprev,
-                                             const double *__restrict__ prcur, double *__restrict__ hist_b)
+                                             Ctrl *__restrict__ ctrl, const double *__restrict__ prprev, unsigned nbp,
+                                             const double *__restrict__ prcur, unsigned nbc,
+                                             double *__restrict__ hist_b)
 {
     if (t > ctrl->stop_iter) return;
-    const T beta = (T)cg_beta(g, t, ctrl, prprev, prcur, hist_b);
+    const T beta = (T)cg_beta(g, t, ctrl, prprev, nbp, prcur, nbc, hist_b);
     const Col c = column(g);
     if (!c.active) return;
     for (int yb = c.y0; yb < c.y1; yb += R) {
@@ -635,6 +653,269 @@ __global__ __launch_bounds__(TB) void k_cg_p(Geo g, int t, T *__restrict__ p, co
         for (int q = 0; q < R; ++q)
             if (yb + q < c.y1) p[at(g, c.x, yb + q)] = beta * pv[q] + rv[q];
     }
+}
+
+
+// ------------------------------------------- vectorised CG (fused p update)
+//
+// A thread owns V consecutive columns (16-byte loads: V = 4 for FP32, 2 for
+// FP64); a block of TB threads owns V*TB columns and ty rows, processed in
+// chunks of RV rows whose loads are issued together.  The left/right
+// neighbours of a thread's columns come from the adjacent lanes through wave
+// shuffles (lanes 0 and 63 read them from memory).  All lanes run every row
+// so the shuffles see the whole wave; lanes past the grid's right edge load a
+// valid clamped vector and store nothing.  When nx is not a multiple of V the
+// last vector of a row reads into the right halo/padding (inside the
+// allocation) and stores only its interior columns.
+constexpr int RV = TL_RV;
+
+template <typename T, int V>
+__device__ __forceinline__ void vload(T (&d)[V], const T *__restrict__ p)
+{
+    typedef T vec __attribute__((ext_vector_type(V)));
+#if TL_NT_LOAD
+    const vec v = __builtin_nontemporal_load(reinterpret_cast<const vec *>(p));
+#else
+    const vec v = *reinterpret_cast<const vec *>(p);
+#endif
+#pragma unroll
+    for (int j = 0; j < V; ++j) d[j] = v[j];
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void vstore(T *__restrict__ p, const T (&s)[V])
+{
+    typedef T vec __attribute__((ext_vector_type(V)));
+    vec v;
+#pragma unroll
+    for (int j = 0; j < V; ++j) v[j] = s[j];
+#if TL_NT_STORE
+    __builtin_nontemporal_store(v, reinterpret_cast<vec *>(p));
+#else
+    *reinterpret_cast<vec *>(p) = v;
+#endif
+}
+
+// store of the first n (< V: the grid's last, partial vector) or all V elements
+template <typename T, int V>
+__device__ __forceinline__ void vstore_n(T *__restrict__ p, const T (&s)[V], int n)
+{
+    if (n >= V) {
+        vstore<T, V>(p, s);
+    } else {
+#pragma unroll
+        for (int j = 0; j < V; ++j)
+            if (j < n) p[j] = s[j];
+    }
+}
+
+template <int V>
+struct VCol {
+    int x0;        // first column owned (may be >= nx: inactive)
+    int xl;        // first column loaded (clamped into the grid)
+    int y0, y1;
+    bool active;
+    int lane;
+};
+
+template <int V>
+__device__ __forceinline__ VCol<V> vcolumn(const Geo &g, int ty)
+{
+    VCol<V> c;
+    c.x0 = (blockIdx.x * TB + threadIdx.x) * V;
+    c.xl = min(c.x0, ((g.nx - 1) / V) * V);
+    c.y0 = blockIdx.y * ty;
+    c.y1 = min(c.y0 + ty, g.ny);
+    c.active = c.x0 < g.nx;
+    c.lane = threadIdx.x & 63;
+    return c;
+}
+
+// values of the columns left and right of the thread's V columns, given the
+// thread's own row values c[] (element index i of column xl)
+template <typename T, int V, class F>
+__device__ __forceinline__ void vneigh(const Geo &g, const VCol<V> &cl, const T (&c)[V], F &&at_col, T &lv,
+                                       T &rv)
+{
+    T l = __shfl_up(c[V - 1], 1, 64);
+    T r = __shfl_down(c[0], 1, 64);
+    if (cl.lane == 0) l = cl.xl > 0 ? at_col(cl.xl - 1) : c[0];
+    if (cl.lane == 63) r = cl.xl + V < g.nx ? at_col(cl.xl + V) : c[V - 1];
+    if (cl.xl == 0) l = c[0];
+    lv = l;
+    rv = r;
+}
+
+// 5-point operator on element j of a V-vector row: left/right from the
+// vector or the neighbour values, clamped at the grid edges
+template <typename T, int V>
+__device__ __forceinline__ T vsmvp(const Geo &g, int xl, int j, const T (&c)[V], T lv, T rv, T dn, T up, T kxl,
+                                   T kxr, T kyd, T kyu)
+{
+    const T l = j > 0 ? c[j - 1] : lv;
+    T r = j < V - 1 ? c[j + 1] : rv;
+    if (xl + j + 1 >= g.nx) r = c[j];
+    return smvp5(c[j], l, r, dn, up, kxl, kxr, kyd, kyu);
+}
+
+// Iteration t of the fused CG: for t >= 2, first the end of iteration t-1 --
+// beta = rr(t-1)/rr(t-2) and the stop test (CG.jl:39-42, 24) -- then
+// p = r + beta p (CG.p!, written to pout; p is ping-ponged because every
+// thread reads its neighbours' old p), then w = A p and the partial sums of
+// p.w (CG.w!).  For t = 1, p comes from CG.init!.
+template <typename T, int V>
+__global__ __launch_bounds__(TB) void k_cg_pwv(Geo g, int ty, int t, const T *__restrict__ pin, T *__restrict__ pout,
+                                               const T *__restrict__ r, const T *__restrict__ kx,
+                                               const T *__restrict__ ky, T *__restrict__ w, Ctrl *__restrict__ ctrl,
+                                               double *__restrict__ pw, const double *__restrict__ prprev,
+                                               unsigned nbp, const double *__restrict__ prcur, unsigned nbc,
+                                               double *__restrict__ hist_b)
+{
+    if (t > ctrl->stop_iter) return;
+    T beta = (T)0;
+    const bool fuse = t >= 2;
+    if (fuse) {
+        double a1[1], a2[1];
+        sum_partials<1>(prcur, nbc, g.nbmax, a1);
+        sum_partials<1>(prprev, nbp, g.nbmax, a2);
+        const double rrn = a1[0], rro = a2[0];
+        const double b = rrn / rro;
+        const int why = decide_why(ctrl, t - 1, rrn);
+        if (block_id() == 0 && threadIdx.x == 0) {
+            ctrl->beta = b;
+            ctrl->rro = rrn;
+            ctrl->err = rrn;
+            ctrl->iters = t - 1;
+            if (hist_b) hist_b[t - 2] = b;
+            record_stop(ctrl, t - 1, why);
+        }
+        if (why != WHY_NONE) return;  // every block derives the same decision
+        beta = (T)b;
+    }
+    const VCol<V> cl = vcolumn<V>(g, ty);
+    const T *pp = fuse ? pin : pout;
+    // p after the update at column x of row element index i
+    auto pnew = [&](long i) -> T { return fuse ? r[i] + beta * pp[i] : pp[i]; };
+    double acc[1] = {0.0};
+    if (cl.y0 < g.ny) {
+        T dn[V], cur[V], kyd[V];
+        {
+            const long i0 = at(g, cl.xl, cl.y0), id = at(g, cl.xl, max(cl.y0 - 1, 0));
+            T a[V], b2[V];
+            vload<T, V>(a, pp + i0);
+            if (fuse) {
+                vload<T, V>(b2, r + i0);
+#pragma unroll
+                for (int j = 0; j < V; ++j) cur[j] = b2[j] + beta * a[j];
+                vload<T, V>(a, pp + id);
+                vload<T, V>(b2, r + id);
+#pragma unroll
+                for (int j = 0; j < V; ++j) dn[j] = b2[j] + beta * a[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < V; ++j) cur[j] = a[j];
+                vload<T, V>(dn, pp + id);
+            }
+            vload<T, V>(kyd, ky + i0);
+        }
+        for (int yb = cl.y0; yb < cl.y1; yb += RV) {
+            T up[RV][V], kxv[RV][V], kyu[RV][V], rup[RV][V];
+#pragma unroll
+            for (int q = 0; q < RV; ++q) {
+                const int y = yb + q;
+                const long ic = atc(g, cl.xl, y), iu = atc(g, cl.xl, y + 1);
+                vload<T, V>(up[q], pp + iu);
+                if (fuse) vload<T, V>(rup[q], r + iu);
+                vload<T, V>(kxv[q], kx + ic);
+                vload<T, V>(kyu[q], ky + at(g, cl.xl, min(y + 1, g.ny)));
+            }
+#pragma unroll
+            for (int q = 0; q < RV; ++q) {
+                const int y = yb + q;
+                const long ic = atc(g, cl.xl, y);
+                T u2[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) u2[j] = fuse ? rup[q][j] + beta * up[q][j] : up[q][j];
+                // clamp the row above at the top edge (reflection: the row itself)
+                if (y + 1 >= g.ny) {
+#pragma unroll
+                    for (int j = 0; j < V; ++j) u2[j] = cur[j];
+                }
+                T lv, rv;
+                auto col_val = [&](int x) -> T { return pnew(ic + (x - cl.xl)); };
+                vneigh<T, V>(g, cl, cur, col_val, lv, rv);
+                // kx of the face right of the last column: next lane's first kx
+                T kxn = __shfl_down(kxv[q][0], 1, 64);
+                if (cl.lane == 63) kxn = kx[ic + V];
+                T wv[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const T kxr = j < V - 1 ? kxv[q][j + 1] : kxn;
+                    wv[j] = vsmvp<T, V>(g, cl.xl, j, cur, lv, rv, dn[j], u2[j], kxv[q][j], kxr, kyd[j], kyu[q][j]);
+                }
+                if (y < cl.y1 && cl.active) {
+                    vstore_n<T, V>(w + ic, wv, g.nx - cl.x0);
+                    if (fuse) vstore_n<T, V>(pout + ic, cur, g.nx - cl.x0);
+#pragma unroll
+                    for (int j = 0; j < V; ++j)
+                        if (cl.x0 + j < g.nx) acc[0] += (double)wv[j] * (double)cur[j];
+                }
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    dn[j] = cur[j];
+                    cur[j] = u2[j];
+                    kyd[j] = kyu[q][j];
+                }
+            }
+        }
+    }
+    write_partials<1>(acc, pw, g.nbmax);
+}
+
+// CG.ur!, CG.jl:93-98 on V-vectors: alpha = rro/pw from the partials, then
+// u += alpha p; r -= alpha w; partial sums of r^2
+template <typename T, int V>
+__global__ __launch_bounds__(TB) void k_cg_urv(Geo g, int ty, int t, T *__restrict__ u, const T *__restrict__ p,
+                                               T *__restrict__ r, const T *__restrict__ w, Ctrl *__restrict__ ctrl,
+                                               const double *__restrict__ pw, unsigned nbw,
+                                               const double *__restrict__ prprev, unsigned nbp,
+                                               double *__restrict__ prcur, double *__restrict__ hist_a)
+{
+    if (t > ctrl->stop_iter) return;
+    double ad;
+    if (!cg_alpha(g, t, ctrl, pw, nbw, prprev, nbp, hist_a, &ad)) return;
+    const T alpha = (T)ad;
+    const VCol<V> cl = vcolumn<V>(g, ty);
+    double acc[1] = {0.0};
+    if (cl.active && cl.y0 < g.ny) {
+        for (int yb = cl.y0; yb < cl.y1; yb += RV) {
+            T uv[RV][V], pv[RV][V], rv[RV][V], wv[RV][V];
+#pragma unroll
+            for (int q = 0; q < RV; ++q) {
+                const long i = atc(g, cl.xl, yb + q);
+                vload<T, V>(uv[q], u + i);
+                vload<T, V>(pv[q], p + i);
+                vload<T, V>(rv[q], r + i);
+                vload<T, V>(wv[q], w + i);
+            }
+#pragma unroll
+            for (int q = 0; q < RV; ++q) {
+                if (yb + q < cl.y1) {
+                    const long i = at(g, cl.xl, yb + q);
+                    T un[V], rn[V];
+#pragma unroll
+                    for (int j = 0; j < V; ++j) {
+                        un[j] = uv[q][j] + alpha * pv[q][j];
+                        rn[j] = rv[q][j] - alpha * wv[q][j];
+                        if (cl.x0 + j < g.nx) acc[0] += (double)rn[j] * (double)rn[j];
+                    }
+                    vstore_n<T, V>(u + i, un, g.nx - cl.x0);
+                    vstore_n<T, V>(r + i, rn, g.nx - cl.x0);
+                }
+            }
+        }
+    }
+    write_partials<1>(acc, prcur, g.nbmax);
 }
 
 // ---------------------------------------------- residuals and summaries
@@ -676,7 +957,7 @@ __global__ __launch_bounds__(TB) void k_residual(Geo g, int t, const double *__r
             }
         }
     }
-    if (part) write_partials<1>(acc, part, nblocks(g));
+    if (part) write_partials<1>(acc, part, g.nbmax);
 }
 
 // partial sums of (b - A x)^2
@@ -702,7 +983,7 @@ __global__ __launch_bounds__(TB) void k_resid_norm(Geo g, const double *__restri
             }
         }
     }
-    write_partials<1>(acc, part, nblocks(g));
+    write_partials<1>(acc, part, g.nbmax);
 }
 
 // dst = A src over the interior
@@ -766,7 +1047,7 @@ __global__ __launch_bounds__(TB) void k_field_summary(Geo g, const double *__res
             }
         }
     }
-    write_partials<4>(acc, part, nblocks(g));
+    write_partials<4>(acc, part, g.nbmax);
 }
 
 // partial sums of v^2 (skipped when t > stop_iter; t = 0 always runs)
@@ -785,7 +1066,7 @@ __global__ __launch_bounds__(TB) void k_sumsq(Geo g, int t, const double *__rest
                 if (yb + q < c.y1) acc[0] += vv[q] * vv[q];
         }
     }
-    write_partials<1>(acc, part, nblocks(g));
+    write_partials<1>(acc, part, g.nbmax);
 }
 
 // ---------------------------------------------------------------- halo
@@ -852,7 +1133,7 @@ __global__ __launch_bounds__(TB) void k_cheby_init(Geo g, const double *__restri
             }
         }
     }
-    write_partials<1>(acc, part, nblocks(g));
+    write_partials<1>(acc, part, g.nbmax);
 }
 
 // Cheby.mainstep!, Cheby.jl:84-106: w = A u; r = u0 - w; p = a p + b r;
@@ -889,7 +1170,7 @@ __global__ __launch_bounds__(TB) void k_cheby_iter(Geo g, int t, const double *_
             }
         }
     }
-    if (norm) write_partials<1>(acc, part, nblocks(g));
+    if (norm) write_partials<1>(acc, part, g.nbmax);
 }
 
 // ----------------------------------------------------------------- PPCG
@@ -900,11 +1181,12 @@ __global__ __launch_bounds__(TB) void k_ppcg_ur_sd(Geo g, int t, double *__restr
                                                    const double *__restrict__ p, double *__restrict__ r,
                                                    const double *__restrict__ w, double *__restrict__ sd,
                                                    double theta, Ctrl *__restrict__ ctrl,
-                                                   const double *__restrict__ pw, const double *__restrict__ prprev)
+                                                   const double *__restrict__ pw, unsigned nbw,
+                                                   const double *__restrict__ prprev, unsigned nbp)
 {
     if (t > ctrl->stop_iter) return;
     double alpha;
-    if (!cg_alpha(g, t, ctrl, pw, prprev, nullptr, &alpha)) return;
+    if (!cg_alpha(g, t, ctrl, pw, nbw, prprev, nbp, nullptr, &alpha)) return;
     const Col c = column(g);
     if (!c.active) return;
     for (int yb = c.y0; yb < c.y1; yb += R) {
@@ -962,7 +1244,7 @@ __global__ __launch_bounds__(TB) void k_ppcg_inner(Geo g, int t, const double *_
             }
         }
     }
-    if (last) write_partials<1>(acc, part, nblocks(g));
+    if (last) write_partials<1>(acc, part, g.nbmax);
 }
 
 // --------------------------------------------------------------- Jacobi
@@ -1005,7 +1287,7 @@ __global__ __launch_bounds__(TB) void k_jacobi(Geo g, int t, const double *__res
             }
         }
     }
-    write_partials<1>(acc, part, nblocks(g));
+    write_partials<1>(acc, part, g.nbmax);
 }
 
 // damped Jacobi sweep on A x = b with b = u: x' = x + omega (b - A x) / diag(A)
@@ -1067,10 +1349,17 @@ __global__ __launch_bounds__(TB) void k_ir_accumulate(Geo g, double *__restrict_
 static inline dim3 sgrid(const Geo &g) { return dim3(g.gx, g.gy); }
 static inline dim3 fgrid(const Geo &g) { return dim3((g.X + TB - 1) / TB, g.Y); }
 static inline unsigned nbh(const tl_ctx *c) { return (unsigned)(c->g.gx * c->g.gy); }
-// partial-sum buffers: [pw | pr0 | pr1 | pm0 pm1 pm2 pm3], nb doubles each
+static inline unsigned nb4(const tl_ctx *c) { return (unsigned)(c->g.gx4 * c->g.gy4); }
+static inline unsigned nb2(const tl_ctx *c) { return (unsigned)(c->g.gx2 * c->g.gy2); }
+static inline size_t stride(const tl_ctx *c) { return (size_t)c->g.nbmax; }
+// partial-sum arrays: [pw | pr0 | pr1 | pm0 pm1 pm2 pm3], nbmax doubles each
 static inline double *part_w(tl_ctx *c) { return c->partials; }
-static inline double *part_r(tl_ctx *c, int t) { return c->partials + (size_t)(1 + (t & 1)) * nbh(c); }
-static inline double *part_m(tl_ctx *c) { return c->partials + (size_t)3 * nbh(c); }
+static inline double *part_r(tl_ctx *c, int t) { return c->partials + (size_t)(1 + (t & 1)) * stride(c); }
+static inline double *part_m(tl_ctx *c) { return c->partials + (size_t)3 * stride(c); }
+static inline unsigned &nbr(tl_ctx *c, int t) { return c->nb_r[t & 1]; }
+// p of iteration t >= 1 of the fused CG lives in buffer (t-1) & 1
+static inline double *p64(tl_ctx *c, int t) { return ((t - 1) & 1) ? c->p_alt : c->f[TL_F_P]; }
+static inline float *p32(tl_ctx *c, int t) { return ((t - 1) & 1) ? c->p32b : c->p32; }
 
 // event pair around one launch when this kernel is being profiled
 static bool prof_start(tl_ctx *c, int kind)
@@ -1093,9 +1382,9 @@ static void prof_stop(tl_ctx *c)
     c->prof_used += 2;
 }
 
-static hipError_t fin(tl_ctx *c, const double *part, int n, int slot, int kind, int t)
+static hipError_t fin(tl_ctx *c, const double *part, unsigned nb, int n, int slot, int kind, int t)
 {
-    k_fin<<<1, TB, 0, c->stream>>>(c->ctrl, part, nbh(c), n, slot, kind, t);
+    k_fin<<<1, TB, 0, c->stream>>>(c->ctrl, part, nb, (unsigned)stride(c), n, slot, kind, t);
     return hipGetLastError();
 }
 
@@ -1141,15 +1430,17 @@ hipError_t launch_cg_init_resid(tl_ctx *c, const double *guess)
     k_cg_init_resid<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U], guess, c->f[TL_F_U0], c->f[TL_F_P],
                                                        c->f[TL_F_R], c->f[TL_F_W], c->f[TL_F_KX], c->f[TL_F_KY],
                                                        part_r(c, 0), part_m(c));
+    nbr(c, 0) = nbh(c);
+    c->nb_m = nbh(c);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_m(c), 2, 0, FK_INIT, 0) : e;
+    return e == hipSuccess ? fin(c, part_m(c), nbh(c), 2, 0, FK_INIT, 0) : e;
 }
 
 hipError_t launch_resid_norm_slot(tl_ctx *c, const double *b, const double *x, int slot)
 {
     k_resid_norm<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, b, x, c->f[TL_F_KX], c->f[TL_F_KY], part_m(c));
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_m(c), 1, slot, FK_SUMS, 0) : e;
+    return e == hipSuccess ? fin(c, part_m(c), nbh(c), 1, slot, FK_SUMS, 0) : e;
 }
 
 hipError_t launch_cg_w64(tl_ctx *c, int t)
@@ -1157,6 +1448,7 @@ hipError_t launch_cg_w64(tl_ctx *c, int t)
     const bool prof = prof_start(c, TL_K_CG_W64);
     k_cg_w<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_P], c->f[TL_F_KX], c->f[TL_F_KY],
                                                        c->f[TL_F_W], c->ctrl, part_w(c));
+    c->nb_w = nbh(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
 }
@@ -1165,8 +1457,9 @@ hipError_t launch_cg_ur64(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_UR64);
     k_cg_ur<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_U], c->f[TL_F_P], c->f[TL_F_R],
-                                                        c->f[TL_F_W], c->ctrl, part_w(c), part_r(c, t - 1),
-                                                        part_r(c, t), c->hist_a);
+                                                        c->f[TL_F_W], c->ctrl, part_w(c), c->nb_w, part_r(c, t - 1),
+                                                        nbr(c, t - 1), part_r(c, t), c->hist_a);
+    nbr(c, t) = nbh(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
 }
@@ -1175,16 +1468,53 @@ hipError_t launch_cg_p64(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_P64);
     k_cg_p<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_P], c->f[TL_F_R], c->ctrl,
-                                                       part_r(c, t - 1), part_r(c, t), c->hist_b);
+                                                       part_r(c, t - 1), nbr(c, t - 1), part_r(c, t), nbr(c, t),
+                                                       c->hist_b);
     if (prof) prof_stop(c);
+    return hipGetLastError();
+}
+
+// fused CG, FP64: PW(t) then UR(t)
+hipError_t launch_cg_pw64(tl_ctx *c, int t)
+{
+    const bool prof = prof_start(c, TL_K_CG_W64);
+    k_cg_pwv<double, 2><<<dim3(c->g.gx2, c->g.gy2), TB, 0, c->stream>>>(
+        c->g, c->g.ty2, t, t >= 2 ? p64(c, t - 1) : nullptr, p64(c, t), c->f[TL_F_R], c->f[TL_F_KX], c->f[TL_F_KY],
+        c->f[TL_F_W], c->ctrl, part_w(c), t >= 2 ? part_r(c, t - 2) : nullptr, t >= 2 ? nbr(c, t - 2) : 0,
+        t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, c->hist_b);
+    c->nb_w = nb2(c);
+    if (prof) prof_stop(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_urv64(tl_ctx *c, int t)
+{
+    const bool prof = prof_start(c, TL_K_CG_UR64);
+    k_cg_urv<double, 2><<<dim3(c->g.gx2, c->g.gy2), TB, 0, c->stream>>>(
+        c->g, c->g.ty2, t, c->f[TL_F_U], p64(c, t), c->f[TL_F_R], c->f[TL_F_W], c->ctrl, part_w(c), c->nb_w,
+        part_r(c, t - 1), nbr(c, t - 1), part_r(c, t), c->hist_a);
+    nbr(c, t) = nb2(c);
+    if (prof) prof_stop(c);
+    return hipGetLastError();
+}
+
+// the CG.p! of the last iteration T of a fused FP64 CG solve (p_{T+1} in the
+// buffer of p_T, as the unfused solver leaves it); returns that buffer
+hipError_t launch_cg_pfinal64(tl_ctx *c, int T)
+{
+    k_cg_p<double><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, T, p64(c, T), c->f[TL_F_R], c->ctrl, part_r(c, T - 1),
+                                                       nbr(c, T - 1), part_r(c, T), nbr(c, T), c->hist_b);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_w32(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_W32);
-    k_cg_w<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->p32, c->kx32, c->ky32, c->w32, c->ctrl,
-                                                      part_w(c));
+    k_cg_pwv<float, 4><<<dim3(c->g.gx4, c->g.gy4), TB, 0, c->stream>>>(
+        c->g, c->g.ty4, t, t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, c->kx32, c->ky32, c->w32, c->ctrl,
+        part_w(c), t >= 2 ? part_r(c, t - 2) : nullptr, t >= 2 ? nbr(c, t - 2) : 0,
+        t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, nullptr);
+    c->nb_w = nb4(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
 }
@@ -1192,19 +1522,19 @@ hipError_t launch_cg_w32(tl_ctx *c, int t)
 hipError_t launch_cg_ur32(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_UR32);
-    k_cg_ur<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->x32, c->p32, c->r32, c->w32, c->ctrl,
-                                                       part_w(c), part_r(c, t - 1), part_r(c, t), nullptr);
+    k_cg_urv<float, 4><<<dim3(c->g.gx4, c->g.gy4), TB, 0, c->stream>>>(
+        c->g, c->g.ty4, t, c->x32, p32(c, t), c->r32, c->w32, c->ctrl, part_w(c), c->nb_w, part_r(c, t - 1),
+        nbr(c, t - 1), part_r(c, t), nullptr);
+    nbr(c, t) = nb4(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_p32(tl_ctx *c, int t)
 {
-    const bool prof = prof_start(c, TL_K_CG_P32);
-    k_cg_p<float><<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->p32, c->r32, c->ctrl, part_r(c, t - 1),
-                                                      part_r(c, t), nullptr);
-    if (prof) prof_stop(c);
-    return hipGetLastError();
+    (void)c;
+    (void)t;
+    return hipSuccess;  // fused into the next iteration's launch_cg_w32
 }
 
 hipError_t launch_residual(tl_ctx *c)
@@ -1217,10 +1547,9 @@ hipError_t launch_residual(tl_ctx *c)
 hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t)
 {
     k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_U0], u, c->f[TL_F_KX], c->f[TL_F_KY],
-                                                  c->f[TL_F_R], nullptr, nullptr, c->ctrl,
-                                                  part_m(c) + nbh(c));
+                                                  c->f[TL_F_R], nullptr, nullptr, c->ctrl, part_m(c) + stride(c));
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_m(c), 2, 0, FK_JACOBI, t) : e;
+    return e == hipSuccess ? fin(c, part_m(c), nbh(c), 2, 0, FK_JACOBI, t) : e;
 }
 
 hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next)
@@ -1228,8 +1557,9 @@ hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next)
     k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],
                                                   c->f[TL_F_KY], nullptr, c->r32, reset_p ? c->p32 : nullptr,
                                                   c->ctrl, part_r(c, t_next - 1));
+    nbr(c, t_next - 1) = nbh(c);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_r(c, t_next - 1), 1, 0, FK_IR, 0) : e;
+    return e == hipSuccess ? fin(c, part_r(c, t_next - 1), nbh(c), 1, 0, FK_IR, 0) : e;
 }
 
 hipError_t launch_apply(tl_ctx *c, const double *src, double *dst)
@@ -1249,19 +1579,20 @@ hipError_t launch_field_summary(tl_ctx *c)
     k_field_summary<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_DENSITY], c->f[TL_F_ENERGY], c->f[TL_F_U],
                                                        c->dx * c->dy, part_m(c));
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_m(c), 4, 0, FK_SUMS, 0) : e;
+    return e == hipSuccess ? fin(c, part_m(c), nbh(c), 4, 0, FK_SUMS, 0) : e;
 }
 
 hipError_t launch_sumsq_slot(tl_ctx *c, const double *v, int slot)
 {
     k_sumsq<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, v, c->ctrl, part_m(c));
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_m(c), 1, slot, FK_SUMS, 0) : e;
+    return e == hipSuccess ? fin(c, part_m(c), nbh(c), 1, slot, FK_SUMS, 0) : e;
 }
 
 hipError_t launch_sumsq_rr(tl_ctx *c, const double *v, int t)
 {
     k_sumsq<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, v, c->ctrl, part_r(c, t));
+    nbr(c, t) = nbh(c);
     return hipGetLastError();
 }
 
@@ -1284,7 +1615,7 @@ hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double
     k_cheby_init<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_P], c->f[TL_F_R],
                                                     c->f[TL_F_KX], c->f[TL_F_KY], theta, part_m(c));
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_m(c), 1, 0, FK_BB, 0) : e;
+    return e == hipSuccess ? fin(c, part_m(c), nbh(c), 1, 0, FK_BB, 0) : e;
 }
 
 hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t)
@@ -1292,14 +1623,14 @@ hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, 
     k_cheby_iter<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_P],
                                                     c->f[TL_F_KX], c->f[TL_F_KY], a, b, norm, c->ctrl, part_m(c));
     hipError_t e = hipGetLastError();
-    return (e == hipSuccess && norm) ? fin(c, part_m(c), 1, 0, FK_CHEBY, t) : e;
+    return (e == hipSuccess && norm) ? fin(c, part_m(c), nbh(c), 1, 0, FK_CHEBY, t) : e;
 }
 
 hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t)
 {
     k_ppcg_ur_sd<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, c->f[TL_F_U], c->f[TL_F_P], c->f[TL_F_R],
                                                     c->f[TL_F_W], c->f[TL_F_SD], theta, c->ctrl, part_w(c),
-                                                    part_r(c, t - 1));
+                                                    c->nb_w, part_r(c, t - 1), nbr(c, t - 1));
     return hipGetLastError();
 }
 
@@ -1308,6 +1639,7 @@ hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in,
     k_ppcg_inner<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, sd_in, sd_out, c->f[TL_F_R], c->f[TL_F_U],
                                                     c->f[TL_F_KX], c->f[TL_F_KY], a, b, last, c->ctrl,
                                                     part_r(c, t));
+    if (last) nbr(c, t) = nbh(c);
     return hipGetLastError();
 }
 
@@ -1316,7 +1648,7 @@ hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, in
     k_jacobi<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_KX], c->f[TL_F_KY],
                                                 c->ctrl, part_m(c));
     hipError_t e = hipGetLastError();
-    return (e == hipSuccess && !defer) ? fin(c, part_m(c), 1, 0, FK_JACOBI, t) : e;
+    return (e == hipSuccess && !defer) ? fin(c, part_m(c), nbh(c), 1, 0, FK_JACOBI, t) : e;
 }
 
 hipError_t launch_ir_prepare(tl_ctx *c)

@@ -34,8 +34,9 @@ EXPORTS = [
     "tl_kernel_repeat", "tl_unet_eval",
 ]
 TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32 = range(6)
-KERNEL_NAMES = {0: "k_cg_w<double>", 1: "k_cg_ur<double>", 2: "k_cg_p<double>", 3: "k_cg_w<float>",
-                4: "k_cg_ur<float>", 5: "k_cg_p<float>", -1: "k_copy_full"}
+KERNEL_NAMES = {0: "k_cg_pwv<double, 2>", 1: "k_cg_urv<double, 2>", 2: "k_cg_p<double>",
+                3: "k_cg_pwv<float, 4>", 4: "k_cg_urv<float, 4>", 5: "(fused into k_cg_pwv<float, 4>)",
+                -1: "k_copy_full"}
 
 
 class SolveParams(C.Structure):

@@ -130,3 +130,28 @@ def test_warm_guess_fallback_and_gain():
     assert st2[0]["fallback"] == 0
     assert st2[0]["iterations"] < st0[0]["iterations"] // 4
     assert st2[0]["rel_resid"] <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_cg_ragged_grid_whole_fields(precision):
+    """Grid sizes that are not multiples of the kernels' vector width (4
+    FP32 / 2 FP64 columns per thread) or row chunk: every field, halos
+    included, must match the oracle -- no stores outside the interior."""
+    print(f"case: CG {precision} 131x77 reference deck, 2 steps", flush=True)
+    extra = "stop_rule=relative\nrel_tol=1e-10\nprecision=fp32_ir\n" if precision == "fp32_ir" else ""
+    text = decks.reference_deck(131, 2, extra=extra).replace("y_cells=131", "y_cells=77")
+    if precision == "fp32_ir":
+        o, c, s = _pair(text, stop_rule=O.STOP_RELATIVE, rel_tol=1e-12, maxiters=100000)
+    else:
+        o, c, s = _pair(text)
+    osum, ost = o.run(2)
+    gsum, gst = T.diffuse(c, s, 2)
+    tol = RTOL if precision == "fp64" else 1e-9
+    for t in range(2):
+        assert _rel(gsum[t], osum[t]) < tol, (t, gsum[t], osum[t])
+    fields = ("density", "energy0", "energy", "u") + (("p",) if precision == "fp64" else ())
+    for f in fields:
+        a, b = c.field(f), o.field(f)
+        assert a.shape == b.shape
+        scale = np.max(np.abs(b))
+        assert np.max(np.abs(a - b)) <= 1e-8 * scale, (f, np.max(np.abs(a - b)), scale)

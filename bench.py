@@ -38,13 +38,16 @@ from tealeaf_hip import _lib, decks  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 # algorithmic bytes per interior cell per launch (arrays read once + written once).
-# The CG iteration is two kernels: PW = [p = r + beta p; w = A p] (reads p, r,
-# Kx, Ky; writes p', w) and UR = [u += alpha p; r -= alpha w] (reads u, p, r, w;
-# writes u, r).  FP64: 8-byte words; FP32-IR inner CG: 4-byte words.
-# P is only the final p update of a solve (FP64 path).
+# A CG iteration is two kernels.
+#   FP64:  PW = [p = r + beta p; w = A p] reads p, r, Kx, Ky, writes p', w  (48 B)
+#          UR = [u += alpha p; r -= alpha w] reads u, p, r, w, writes u, r   (48 B)
+#   FP32 inner CG (operator from one cell-coefficient array gc, w not stored):
+#          PW reads p, r, gc, writes p'                                     (16 B)
+#          UR = [x += alpha p; r -= alpha A p] reads x, p, r, gc, writes x, r (24 B)
+# P64 is only the final p update of an FP64 solve.
 KERNEL_BYTES = {
     _lib.TL_K_CG_W64: 48, _lib.TL_K_CG_UR64: 48, _lib.TL_K_CG_P64: 24,
-    _lib.TL_K_CG_W32: 24, _lib.TL_K_CG_UR32: 24, _lib.TL_K_CG_P32: 12,
+    _lib.TL_K_CG_W32: 16, _lib.TL_K_CG_UR32: 24, _lib.TL_K_CG_P32: 0,
 }
 WEIGHTS = os.path.join(ROOT, "tealeaf.jl_amd", "weights", "unet_warmstart.npy")
 

@@ -29,6 +29,7 @@ extern "C" {
 #define TL_E_EIGEN 5        /* negative eigenvalue or eigen-solve failure (kernels.jl:41-43, 80) */
 #define TL_E_PARSE 6        /* malformed deck */
 #define TL_E_STATE 7        /* call out of order (e.g. a learned warm start without a network) */
+#define TL_E_IO 8           /* file could not be read (checkingvalue's open, settings.jl:181) */
 
 /* solvers, settings.jl:103-111 */
 #define TL_CG 0
@@ -173,6 +174,15 @@ int tl_deck_create(tl_ctx **ctx, const tl_deck *deck, int device);
 typedef void (*tl_step_cb)(int step, const double summary[4], const tl_solve_stats *stats, void *user);
 int tl_run(tl_ctx *ctx, const tl_deck *deck, const tl_solve_params *p, int nsteps, tl_step_cb cb, void *user);
 
+/* checkingvalue, settings.jl:180-196: the 4th field of the first line
+ * "xcells ycells endstep value" of problemfile whose three integers match;
+ * *value = 1.0 and *found = 0 when no line matches (the reference's fallback).
+ * TL_E_IO if the file cannot be opened, TL_E_PARSE for a malformed line. */
+int tl_checking_value(const char *problemfile, int xcells, int ycells, int endstep, double *value, int *found);
+/* fieldsummary's QA rule, kernels.jl:127-132: *qa_diff = |100 actual/cv - 100|;
+ * returns 1 (PASSED) if *qa_diff < 0.001, else 0 (FAILED) */
+int tl_qa_check(double actual, double checking_value, double *qa_diff);
+
 /* ------------------------------------------------------ host-side scalars */
 /* eigenvalues! (kernels.jl:19-51): extrema of the Lanczos tridiagonal of n CG
  * steps (alpha, beta 0-based, n entries), scaled by 0.95 / 1.05 */
@@ -195,8 +205,8 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 #define TL_K_CG_W64 0  /* FP64 PW (also the unfused w = A p of Chebyshev/PPCG's CG presteps) */
 #define TL_K_CG_UR64 1 /* FP64 UR */
 #define TL_K_CG_P64 2  /* FP64 p = r + beta p: a solve's last iteration, and the CG presteps */
-#define TL_K_CG_W32 3  /* FP32 PW of the inner CG */
-#define TL_K_CG_UR32 4 /* FP32 UR of the inner CG */
+#define TL_K_CG_W32 3  /* FP32 PW of the inner CG (w = A p only reduced, not stored) */
+#define TL_K_CG_UR32 4 /* FP32 UR of the inner CG (recomputes A p) */
 #define TL_K_CG_P32 5  /* unused: folded into TL_K_CG_W32 */
 /* Bracket every launch of the kernels in kernel_mask (bit = 1 << TL_K_*)
  * with HIP events on the context's stream; tl_profile_read returns one
@@ -206,8 +216,10 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 int tl_profile_begin(tl_ctx *ctx, unsigned kernel_mask);
 int tl_profile_read(tl_ctx *ctx, int kernel_id, double *durations_ms, int max, int *count);
 int tl_profile_end(tl_ctx *ctx);
-/* `reps` back-to-back launches of one kernel on the current fields (a copy
- * kernel of the same size when kernel_id = -1), for counter collection */
+/* `reps` back-to-back launches of one kernel on the current fields, for
+ * counter collection and timing; kernel_id -1 is a copy of one field's
+ * (nx+2h)*(ny+2h) cells (8 B read + 8 B written per cell, one cell per lane),
+ * -2 a 16-byte-per-lane copy of one field's whole allocation */
 int tl_kernel_repeat(tl_ctx *ctx, int kernel_id, int reps);
 
 /* --------------------------------------------------- learned warm start */

@@ -152,7 +152,7 @@ extern "C" void tl_destroy(tl_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     unet_destroy(c);
-    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->kx32, c->ky32, c->p32, c->r32, c->w32, c->x32,
+    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->gc32, c->p32, c->r32, c->x32,
                     c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
     for (int i = 0; i < TL_NFIELDS; ++i)
         if (c->f[i]) (void)hipFree(c->f[i]);
@@ -315,14 +315,12 @@ static int ensure_history(tl_ctx *c, int n)
 
 static int ensure_fp32(tl_ctx *c)
 {
-    if (c->kx32) return TL_OK;
+    if (c->gc32) return TL_OK;
     const size_t n = c->g.elems;
-    RETIF(alloc_f(c, &c->kx32, n));
-    RETIF(alloc_f(c, &c->ky32, n));
+    RETIF(alloc_f(c, &c->gc32, n));
     RETIF(alloc_f(c, &c->p32, n));
     RETIF(alloc_f(c, &c->p32b, n));
     RETIF(alloc_f(c, &c->r32, n));
-    RETIF(alloc_f(c, &c->w32, n));
     RETIF(alloc_f(c, &c->x32, n));
     return TL_OK;
 }
@@ -718,7 +716,7 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
 {
     RETIF(ensure_fp32(c));
     HIPCHK(c, launch_init_coef(c, rx, ry, p->coefficient, INIT_CG));
-    HIPCHK(c, launch_ir_prepare(c));
+    HIPCHK(c, launch_ir_prepare(c, rx, ry, p->coefficient));
     const double delta = p->ir_delta > 0 && p->ir_delta < 1 ? p->ir_delta : 0.1;
     HIPCHK(c, launch_init_ctrl(c, MODE_IR, p->max_iters, p->eps, p->rel_tol, delta, 0, 0, p->epslim));
     const double *guess = nullptr;
@@ -798,10 +796,12 @@ extern "C" int tl_profile_end(tl_ctx *c)
 
 extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
 {
-    if (!c || reps < 1 || kernel_id < -1 || kernel_id > TL_K_CG_P32) return TL_E_INVALID;
+    if (!c || reps < 1 || kernel_id < -2 || kernel_id > TL_K_CG_P32) return TL_E_INVALID;
     HIPCHK(c, hipSetDevice(c->device));
-    if (kernel_id == -1) {
-        for (int i = 0; i < reps; ++i) HIPCHK(c, launch_copy_full(c, c->f[TL_F_U], c->u_alt));
+    if (kernel_id < 0) {
+        for (int i = 0; i < reps; ++i)
+            HIPCHK(c, kernel_id == -1 ? launch_copy_full(c, c->f[TL_F_U], c->u_alt)
+                                      : launch_copy_vec(c, c->f[TL_F_U], c->u_alt));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return TL_OK;
     }
@@ -812,7 +812,7 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
     HIPCHK(c, launch_cg_init_resid(c, nullptr));
     if (kernel_id >= TL_K_CG_W32) {
         RETIF(ensure_fp32(c));
-        HIPCHK(c, launch_ir_prepare(c));
+        HIPCHK(c, launch_ir_prepare(c, 1.0, 1.0, 1));
         HIPCHK(c, launch_ir_residual(c, 1, 1));
     }
     // two live iterations so every partial-sum array exists, then repeats of

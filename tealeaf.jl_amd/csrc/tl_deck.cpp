@@ -13,6 +13,8 @@
 //   rel_tol = <float>               warm_start = none | learned
 //   ir_delta = <float>
 #include <cctype>
+#include <cmath>
+#include <cstdio>
 #include <cerrno>
 #include <climits>
 #include <cstdlib>
@@ -275,4 +277,55 @@ extern "C" int tl_deck_parse(const char *text, tl_deck *d)
     }
     if (d->nstates < 1) return TL_E_PARSE;
     return TL_OK;
+}
+
+// checkingvalue, settings.jl:180-196.  Lines are split on single spaces as the
+// reference does (split(line, " ")); the first three fields are integers,
+// the fourth the checking value.  An empty line ends nothing: it is skipped.
+extern "C" int tl_checking_value(const char *problemfile, int xcells, int ycells, int endstep, double *value,
+                                 int *found)
+{
+    if (!problemfile || !value || !found) return TL_E_INVALID;
+    *value = 1.0;
+    *found = 0;
+    FILE *f = std::fopen(problemfile, "r");
+    if (!f) return TL_E_IO;
+    std::string line;
+    int rc = TL_OK;
+    for (int ch = 0; ch != EOF;) {
+        line.clear();
+        while ((ch = std::fgetc(f)) != EOF && ch != '\n') line.push_back((char)ch);
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (strip(line).empty()) continue;
+        std::vector<std::string> tok;
+        size_t a = 0;
+        for (;;) {
+            const size_t b = line.find(' ', a);
+            tok.push_back(line.substr(a, b == std::string::npos ? std::string::npos : b - a));
+            if (b == std::string::npos) break;
+            a = b + 1;
+        }
+        long long v[3];
+        double cv;
+        if (tok.size() < 4 || !parse_int(tok[0], &v[0]) || !parse_int(tok[1], &v[1]) || !parse_int(tok[2], &v[2]) ||
+            !parse_float(tok[3], &cv)) {
+            rc = TL_E_PARSE;
+            break;
+        }
+        if (v[0] == xcells && v[1] == ycells && v[2] == endstep) {
+            *value = cv;
+            *found = 1;
+            break;
+        }
+    }
+    std::fclose(f);
+    return rc;
+}
+
+// fieldsummary's PASSED/FAILED rule, kernels.jl:127-132
+extern "C" int tl_qa_check(double actual, double checking_value, double *qa_diff)
+{
+    const double d = std::fabs(100.0 * actual / checking_value - 100.0);
+    if (qa_diff) *qa_diff = d;
+    return d < 0.001 ? 1 : 0;
 }

@@ -29,8 +29,17 @@ constexpr int XCOL = 32;       // storage column of the first interior cell
 #ifndef TL_RV
 #define TL_RV 4
 #endif
-#ifndef TL_VEC_BLOCKS
-#define TL_VEC_BLOCKS 2048
+#ifndef TL_VEC_BLOCKS32
+#define TL_VEC_BLOCKS32 2048
+#endif
+#ifndef TL_VEC_BLOCKS64
+#define TL_VEC_BLOCKS64 8192
+#endif
+// optional occupancy target (waves per SIMD) of the FP32 inner-CG kernels
+#ifdef TL_WPE32
+#define TL_CGG_ATTR __attribute__((amdgpu_waves_per_eu(TL_WPE32)))
+#else
+#define TL_CGG_ATTR
 #endif
 // non-temporal (streaming) vector loads / stores in those kernels
 #ifndef TL_NT_LOAD
@@ -97,7 +106,10 @@ struct tl_ctx {
     double *f[TL_NFIELDS] = {};
     double *u_alt = nullptr, *sd_alt = nullptr, *guess = nullptr;
     // FP32 fields of the mixed-precision solve
-    float *kx32 = nullptr, *ky32 = nullptr, *p32 = nullptr, *r32 = nullptr, *w32 = nullptr, *x32 = nullptr;
+    // FP32 inner CG: cell coefficient gc = 1/w and its face scale (rx/2, ry/2),
+    // search direction, residual, correction
+    float *gc32 = nullptr, *p32 = nullptr, *r32 = nullptr, *x32 = nullptr;
+    float hx32 = 0.0f, hy32 = 0.0f;
     // reductions
     double *partials = nullptr;
     size_t partials_len = 0;
@@ -134,6 +146,7 @@ hipError_t launch_init_ctrl(tl_ctx *c, int mode, int maxiters, double eps, doubl
 hipError_t launch_set_mode(tl_ctx *c, int mode, int maxiters);
 hipError_t launch_set_state(tl_ctx *c, const tl_state *states_d, int nstates);
 hipError_t launch_copy_full(tl_ctx *c, const double *a, double *b);
+hipError_t launch_copy_vec(tl_ctx *c, const double *a, double *b);
 hipError_t launch_copy_energy(tl_ctx *c);
 // what k_init_coef writes besides kx, ky
 enum { INIT_CG = 0, INIT_JACOBI = 1, INIT_COEF_ONLY = 2 };
@@ -163,7 +176,7 @@ hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, 
 hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t);
 hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t);
 hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer);
-hipError_t launch_ir_prepare(tl_ctx *c);
+hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient);
 hipError_t launch_ir_accumulate(tl_ctx *c);
 
 Geo make_geo(int nx, int ny, int hd);

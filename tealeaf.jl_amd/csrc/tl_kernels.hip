@@ -53,18 +53,20 @@ Geo make_geo(int nx, int ny, int hd)
     g.ty = ty;
     g.gy = (ny + ty - 1) / ty;
     // vector kernels: the same target block count, 4 (FP32) or 2 (FP64) columns per thread
-    auto vec = [&](int v, int &gxv, int &gyv, int &tyv) {
+    // (measured at 8192^2: the FP64 kernels run fastest with about 8192
+    // blocks, the FP32 ones with about 2048)
+    auto vec = [&](int v, long target, int &gxv, int &gyv, int &tyv) {
         gxv = (nx + v * TB - 1) / (v * TB);
         const long r2 = (long)ny * gxv;
-        int t2 = (int)((r2 + TL_VEC_BLOCKS - 1) / TL_VEC_BLOCKS);
+        int t2 = (int)((r2 + target - 1) / target);
         t2 = (t2 + TL_RV - 1) / TL_RV * TL_RV;  // a multiple of the vector kernels' row chunk
         if (t2 < TL_RV) t2 = TL_RV;
         if (t2 > 1024) t2 = 1024;
         tyv = t2;
         gyv = (ny + t2 - 1) / t2;
     };
-    vec(4, g.gx4, g.gy4, g.ty4);
-    vec(2, g.gx2, g.gy2, g.ty2);
+    vec(4, TL_VEC_BLOCKS32, g.gx4, g.gy4, g.ty4);
+    vec(2, TL_VEC_BLOCKS64, g.gx2, g.gy2, g.ty2);
     g.nbmax = std::max(g.gx * g.gy, std::max(g.gx4 * g.gy4, g.gx2 * g.gy2));
     return g;
 }
@@ -439,6 +441,13 @@ __global__ __launch_bounds__(TB) void k_copy_full(Geo g, const double *__restric
     if (k >= g.X || j >= g.Y) return;
     const long i = (long)j * g.pitch + k + (XCOL - g.hd);
     b[i] = a[i];
+}
+
+// plain 16-byte-per-lane copy of n2 double pairs (bandwidth calibration)
+__global__ __launch_bounds__(TB) void k_copy_vec(const double2 *__restrict__ a, double2 *__restrict__ b, size_t n2)
+{
+    const size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n2) b[i] = a[i];
 }
 
 // CG.init! lines 51-68 (CG.jl) / Jacobi.init! lines 38-55 (Jacobi.jl):
@@ -918,6 +927,164 @@ __global__ __launch_bounds__(TB) void k_cg_urv(Geo g, int ty, int t, T *__restri
     write_partials<1>(acc, prcur, g.nbmax);
 }
 
+// ------------------------------- FP32 inner CG on a cell-coefficient array
+//
+// The inner CG of the mixed-precision solver builds the operator from one
+// FP32 array per cell, gc = 1/w (w = density or 1/density, CG.jl:56-58),
+// instead of the two face arrays: kx(x) = hx (gc(x-1) + gc(x)) with
+// hx = rx/2 equals rx (w(x-1) + w(x)) / (2 w(x-1) w(x)) of CG.jl:61-68 up to
+// FP32 rounding (ky likewise with hy = ry/2).  At the grid edge the clamped
+// neighbour supplies what the reflective halo holds (kernels.jl:191-210).
+// w = A p is never stored: UR recomputes it from p, so an iteration moves 40
+// bytes per cell -- PW reads p, r, gc and writes p'; UR reads x, p, r, gc and
+// writes x, r.
+//
+// UR = false (PW, iteration t): for t >= 2 first the end of iteration t-1 --
+//   beta = rr(t-1)/rr(t-2), the stop test (CG.jl:39-42, 24) -- then
+//   p = r + beta p (CG.p!, written to pout; p is ping-ponged because every
+//   thread reads its neighbours' old p); partial sums of p.(A p) (CG.w!).
+// UR = true (iteration t): alpha = rr(t-1)/pw(t), then x += alpha p,
+//   r -= alpha A p and partial sums of r^2 (CG.ur!, CG.jl:93-98).
+template <int V, bool UR>
+__global__ __launch_bounds__(TB) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, const float *__restrict__ pin,
+                                            float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
+                                            const float *__restrict__ gc, float hx, float hy,
+                                            Ctrl *__restrict__ ctrl, const double *__restrict__ pw_in, unsigned nbw,
+                                            const double *__restrict__ prprev, unsigned nbp,
+                                            const double *__restrict__ prcur, unsigned nbc,
+                                            double *__restrict__ part_out)
+{
+    if (t > ctrl->stop_iter) return;
+    float beta = 0.0f, alpha = 0.0f;
+    const bool fuse = !UR && t >= 2;
+    if (UR) {
+        double ad;
+        if (!cg_alpha(g, t, ctrl, pw_in, nbw, prprev, nbp, nullptr, &ad)) return;
+        alpha = (float)ad;
+    } else if (fuse) {
+        double a1[1], a2[1];
+        sum_partials<1>(prcur, nbc, g.nbmax, a1);
+        sum_partials<1>(prprev, nbp, g.nbmax, a2);
+        const double rrn = a1[0], rro = a2[0];
+        const double b = rrn / rro;
+        const int why = decide_why(ctrl, t - 1, rrn);
+        if (block_id() == 0 && threadIdx.x == 0) {
+            ctrl->beta = b;
+            ctrl->rro = rrn;
+            ctrl->err = rrn;
+            ctrl->iters = t - 1;
+            record_stop(ctrl, t - 1, why);
+        }
+        if (why != WHY_NONE) return;  // every block derives the same decision
+        beta = (float)b;
+    }
+    const VCol<V> cl = vcolumn<V>(g, ty);
+    const float *pp = fuse ? pin : pout;
+    // p (after this launch's update) at element i
+    auto pnew = [&](long i) -> float { return fuse ? r[i] + beta * pp[i] : pp[i]; };
+    auto gcol = [&](long i) -> float { return gc[i]; };
+    double acc[1] = {0.0};
+    if (cl.y0 < g.ny) {
+        float dn[V], cur[V], gdn[V], gcur[V];
+        {
+            const long i0 = at(g, cl.xl, cl.y0), id = at(g, cl.xl, max(cl.y0 - 1, 0));
+            float a[V], b2[V];
+            vload<float, V>(a, pp + i0);
+            vload<float, V>(gcur, gc + i0);
+            vload<float, V>(gdn, gc + id);
+            if (fuse) {
+                vload<float, V>(b2, r + i0);
+#pragma unroll
+                for (int j = 0; j < V; ++j) cur[j] = b2[j] + beta * a[j];
+                vload<float, V>(a, pp + id);
+                vload<float, V>(b2, r + id);
+#pragma unroll
+                for (int j = 0; j < V; ++j) dn[j] = b2[j] + beta * a[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < V; ++j) cur[j] = a[j];
+                vload<float, V>(dn, pp + id);
+            }
+        }
+        for (int yb = cl.y0; yb < cl.y1; yb += RV) {
+            float up[RV][V], rup[RV][V], gup[RV][V], xv[RV][V], rv[RV][V];
+#pragma unroll
+            for (int q = 0; q < RV; ++q) {
+                const int y = yb + q;
+                const long iu = atc(g, cl.xl, y + 1);
+                vload<float, V>(up[q], pp + iu);
+                if (fuse) vload<float, V>(rup[q], r + iu);
+                vload<float, V>(gup[q], gc + iu);
+                if (UR) {
+                    const long ic = atc(g, cl.xl, y);
+                    vload<float, V>(xv[q], x + ic);
+                    vload<float, V>(rv[q], r + ic);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RV; ++q) {
+                const int y = yb + q;
+                const long ic = atc(g, cl.xl, y);
+                float u2[V], gu[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    u2[j] = fuse ? rup[q][j] + beta * up[q][j] : up[q][j];
+                    gu[j] = gup[q][j];
+                }
+                if (y + 1 >= g.ny) {  // reflection at the top edge: the row itself
+#pragma unroll
+                    for (int j = 0; j < V; ++j) {
+                        u2[j] = cur[j];
+                        gu[j] = gcur[j];
+                    }
+                }
+                float lv, rvn, glv, grv;
+                auto col_val = [&](int xx) -> float { return pnew(ic + (xx - cl.xl)); };
+                auto col_g = [&](int xx) -> float { return gcol(ic + (xx - cl.xl)); };
+                vneigh<float, V>(g, cl, cur, col_val, lv, rvn);
+                vneigh<float, V>(g, cl, gcur, col_g, glv, grv);
+                float wv[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const float gl = j > 0 ? gcur[j - 1] : glv;
+                    float gr = j < V - 1 ? gcur[j + 1] : grv;
+                    if (cl.xl + j + 1 >= g.nx) gr = gcur[j];
+                    const float kxl = hx * (gl + gcur[j]), kxr = hx * (gcur[j] + gr);
+                    const float kyd = hy * (gdn[j] + gcur[j]), kyu = hy * (gcur[j] + gu[j]);
+                    wv[j] = vsmvp<float, V>(g, cl.xl, j, cur, lv, rvn, dn[j], u2[j], kxl, kxr, kyd, kyu);
+                }
+                if (y < cl.y1 && cl.active) {
+                    const int nv = g.nx - cl.x0;
+                    if (UR) {
+                        float xn[V], rn[V];
+#pragma unroll
+                        for (int j = 0; j < V; ++j) {
+                            xn[j] = xv[q][j] + alpha * cur[j];
+                            rn[j] = rv[q][j] - alpha * wv[j];
+                            if (j < nv) acc[0] += (double)rn[j] * (double)rn[j];
+                        }
+                        vstore_n<float, V>(x + ic, xn, nv);
+                        vstore_n<float, V>(r + ic, rn, nv);
+                    } else {
+                        if (fuse) vstore_n<float, V>(pout + ic, cur, nv);
+#pragma unroll
+                        for (int j = 0; j < V; ++j)
+                            if (j < nv) acc[0] += (double)wv[j] * (double)cur[j];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    dn[j] = cur[j];
+                    cur[j] = u2[j];
+                    gdn[j] = gcur[j];
+                    gcur[j] = gu[j];
+                }
+            }
+        }
+    }
+    write_partials<1>(acc, part_out, g.nbmax);
+}
+
 // ---------------------------------------------- residuals and summaries
 
 // residual!, kernels.jl:227-232: r = u0 - A u.  Optional outputs: r (FP64),
@@ -1316,11 +1483,16 @@ __global__ __launch_bounds__(TB) void k_smooth(Geo g, const double *__restrict__
 
 // ------------------------------------------------------ mixed precision
 
-// FP32 copies of the face coefficients over the whole allocation
-__global__ __launch_bounds__(TB) void k_to_f32(const double *__restrict__ a, float *__restrict__ b, size_t n)
+// FP32 cell coefficient of the inner CG over the whole allocation:
+// gc = 1/w with w = density (coefficient 1) or 1/density (CG.jl:56-58);
+// cells without a density (padding) get 0
+__global__ __launch_bounds__(TB) void k_gc32(const double *__restrict__ rho, float *__restrict__ gc, size_t n,
+                                             int coefficient)
 {
     const size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
+    if (i >= n) return;
+    const double d = rho[i];
+    gc[i] = coefficient == 2 ? (float)d : (d != 0.0 ? (float)(1.0 / d) : 0.0f);
 }
 
 // FP64 solution accumulation of the mixed-precision CG: u += x; x = 0
@@ -1412,6 +1584,13 @@ hipError_t launch_set_state(tl_ctx *c, const tl_state *states_d, int nstates)
 hipError_t launch_copy_full(tl_ctx *c, const double *a, double *b)
 {
     k_copy_full<<<fgrid(c->g), TB, 0, c->stream>>>(c->g, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_vec(tl_ctx *c, const double *a, double *b)
+{
+    const size_t n2 = c->g.elems / 2;
+    k_copy_vec<<<(unsigned)((n2 + TB - 1) / TB), TB, 0, c->stream>>>((const double2 *)a, (double2 *)b, n2);
     return hipGetLastError();
 }
 
@@ -1510,10 +1689,10 @@ hipError_t launch_cg_pfinal64(tl_ctx *c, int T)
 hipError_t launch_cg_w32(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_W32);
-    k_cg_pwv<float, 4><<<dim3(c->g.gx4, c->g.gy4), TB, 0, c->stream>>>(
-        c->g, c->g.ty4, t, t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, c->kx32, c->ky32, c->w32, c->ctrl,
-        part_w(c), t >= 2 ? part_r(c, t - 2) : nullptr, t >= 2 ? nbr(c, t - 2) : 0,
-        t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, nullptr);
+    k_cgg<4, false><<<dim3(c->g.gx4, c->g.gy4), TB, 0, c->stream>>>(
+        c->g, c->g.ty4, t, t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gc32, c->hx32, c->hy32,
+        c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 2) : nullptr, t >= 2 ? nbr(c, t - 2) : 0,
+        t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c));
     c->nb_w = nb4(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
@@ -1522,9 +1701,9 @@ hipError_t launch_cg_w32(tl_ctx *c, int t)
 hipError_t launch_cg_ur32(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_UR32);
-    k_cg_urv<float, 4><<<dim3(c->g.gx4, c->g.gy4), TB, 0, c->stream>>>(
-        c->g, c->g.ty4, t, c->x32, p32(c, t), c->r32, c->w32, c->ctrl, part_w(c), c->nb_w, part_r(c, t - 1),
-        nbr(c, t - 1), part_r(c, t), nullptr);
+    k_cgg<4, true><<<dim3(c->g.gx4, c->g.gy4), TB, 0, c->stream>>>(
+        c->g, c->g.ty4, t, nullptr, p32(c, t), c->r32, c->x32, c->gc32, c->hx32, c->hy32, c->ctrl, part_w(c),
+        c->nb_w, part_r(c, t - 1), nbr(c, t - 1), nullptr, 0, part_r(c, t));
     nbr(c, t) = nb4(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
@@ -1651,12 +1830,13 @@ hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, in
     return (e == hipSuccess && !defer) ? fin(c, part_m(c), nbh(c), 1, 0, FK_JACOBI, t) : e;
 }
 
-hipError_t launch_ir_prepare(tl_ctx *c)
+hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient)
 {
     const size_t n = c->g.elems;
     const unsigned nb = (unsigned)((n + TB - 1) / TB);
-    k_to_f32<<<nb, TB, 0, c->stream>>>(c->f[TL_F_KX], c->kx32, n);
-    k_to_f32<<<nb, TB, 0, c->stream>>>(c->f[TL_F_KY], c->ky32, n);
+    k_gc32<<<nb, TB, 0, c->stream>>>(c->f[TL_F_DENSITY], c->gc32, n, coefficient);
+    c->hx32 = (float)(0.5 * rx);
+    c->hy32 = (float)(0.5 * ry);
     return hipGetLastError();
 }
 

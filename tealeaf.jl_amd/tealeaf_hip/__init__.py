@@ -26,7 +26,7 @@ from ._lib import (FID, FIELDS, TL_CG, TL_CHEBY, TL_FP32_IR, TL_FP64, TL_JACOBI,
 
 __all__ = [
     "Settings", "Chunk", "CG", "Cheby", "PPCG", "Jacobi", "initialiseapp", "diffuse", "fieldsummary",
-    "solvefinished", "TeaLeafError", "load", "FIELDS",
+    "solvefinished", "checkingvalue", "qa_check", "TeaLeafError", "load", "FIELDS",
 ]
 
 
@@ -211,6 +211,32 @@ def fieldsummary(chunk: Chunk) -> np.ndarray:
     out = np.zeros(4)
     check(load().tl_field_summary(chunk._ctx, _dp(out)), chunk._ctx, "tl_field_summary")
     return out
+
+
+def checkingvalue(settings: Settings, problemfile: str = "tea.problems") -> float:
+    """checkingvalue, settings.jl:180-196: the problem file's value for
+    (xcells, ycells, endstep); 1.0 with a warning when no line matches."""
+    v, found = C.c_double(), C.c_int()
+    rc = load().tl_checking_value(problemfile.encode(), settings.xcells, settings.ycells, settings.endstep,
+                                  C.byref(v), C.byref(found))
+    if rc != _lib.TL_OK:
+        raise TeaLeafError(rc, f"checkingvalue: cannot read {problemfile}")
+    if not found.value:
+        import warnings
+        warnings.warn("Problem was not found in the test problems file.")
+    return v.value
+
+
+def qa_check(chunk: Chunk, settings: Settings, problemfile: str = "tea.problems") -> Optional[dict]:
+    """The QA half of fieldsummary, kernels.jl:119-133: temp = sum vol*rho*u
+    against the checking value; None when check_result is off."""
+    if not settings.checkresult:
+        return None
+    actual = float(fieldsummary(chunk)[3])
+    cv = checkingvalue(settings, problemfile)
+    d = C.c_double()
+    passed = bool(load().tl_qa_check(actual, cv, C.byref(d)))
+    return {"actual": actual, "checking_value": cv, "qa_diff": d.value, "passed": passed}
 
 
 def step(chunk: Chunk, settings: Settings) -> dict:

@@ -50,6 +50,17 @@ KERNEL_BYTES = {
     _lib.TL_K_CG_W32: 16, _lib.TL_K_CG_UR32: 24, _lib.TL_K_CG_P32: 0,
 }
 WEIGHTS = os.path.join(ROOT, "tealeaf.jl_amd", "weights", "unet_warmstart.npy")
+# HBM bytes per launch of the CG kernels at 8192^2 from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE counter runs, scaled by a copy kernel of known size
+PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_8192.json")
+
+
+def measured_traffic(kernel, n):
+    if n != 8192 or not os.path.exists(PMC_TRAFFIC):
+        return None
+    with open(PMC_TRAFFIC) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return None if k is None else k["traffic_bytes"]
 
 
 def log(*a):
@@ -125,7 +136,7 @@ def roofline_from(prof, n):
     achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(_lib.KERNEL_NAMES[kid], n),
         "kernel": _lib.KERNEL_NAMES[kid], "avg_ms": round(avg_ms, 5), "launches": int(len(live)),
         "stopped_launches_excluded": int(skipped), "algorithmic_bytes_per_launch": algo_bytes,
         "bytes_per_cell": KERNEL_BYTES[kid],
@@ -193,8 +204,9 @@ def main():
     ap.add_argument("--kernel-repeat", type=int, nargs=2, metavar=("KERNEL_ID", "REPS"))
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the solve shards by 2-D domain decomposition "
-              "(see DESIGN.md) but no multi-GPU path is built, so --gpus must be 1", file=sys.stderr)
+        print("bench.py: this build measures one GPU; the solve would shard by 2-D domain decomposition "
+              "(halo exchange of p and u, two scalar all-reduces per CG iteration) but no multi-GPU path "
+              "is built, so --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     if a.kernel_repeat:
         kernel_repeat(a.n, a.seed, a.kernel_repeat[0], a.kernel_repeat[1])
@@ -219,6 +231,8 @@ def main():
         },
         "flagship": {
             "s_per_step": [round(t, 4) for t in flag["times"]],
+            "median": round(float(np.median(flag["times"])), 4),
+            "p90": round(float(np.percentile(flag["times"], 90)), 4),
             "iterations": [st["iterations"] for st in flag["stats"]],
             "refreshes": [st["refreshes"] for st in flag["stats"]],
             "fallback": [st["fallback"] for st in flag["stats"]],
@@ -229,6 +243,8 @@ def main():
         bval = float(np.mean(base["times"]))
         out["baseline_fp64_cg"] = {
             "s_per_step": [round(t, 4) for t in base["times"]], "mean": round(bval, 4),
+            "median": round(float(np.median(base["times"])), 4),
+            "p90": round(float(np.percentile(base["times"], 90)), 4),
             "iterations": [st["iterations"] for st in base["stats"]],
             "rel_resid": [st["rel_resid"] for st in base["stats"]],
         }

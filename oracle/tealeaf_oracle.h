@@ -9,6 +9,27 @@
  * X*Y array of doubles, X = xcells + 2*halodepth, Y = ycells + 2*halodepth,
  * column-major with x fastest (Julia Matrix{Float64}).  Index (k, j) here is
  * 0-based: element = j*X + k; the reference's 1-based [k+1, j+1].
+ *
+ * The reference ships no test vectors and cannot run here (Julia), so each
+ * function is pinned by mathematics (tests/test_oracle.py):
+ *   or_initialise        exact covered-cell counts and initial sum vol*rho*u
+ *                        of the 512^2 deck (intended and literal D3/D4)
+ *   or_smvp_field        cosine eigenmode (closed-form eigenvalue), symmetry,
+ *                        unit row sums under reflection
+ *   or_halo_update       ghost indices, intended and literal (D2)
+ *   or_residual          u0 - A u against the assembled sparse matrix
+ *   or_solve/or_step     CG, Chebyshev, PPCG, Jacobi: converged u equals a
+ *                        sparse direct solve; cross-solver agreement; PPCG
+ *                        with 0 inner steps equals CG; ie conservation
+ *   or_field_summary,    uniform-deck closed form per step
+ *   or_finalise
+ *   or_tqli,             scipy eigvalsh_tridiagonal; Ritz values inside the
+ *   or_eigenvalues       dense spectrum
+ *   or_cheby_coef,       Chebyshev-polynomial closed form; the formula
+ *   or_calciter
+ *   literal D1           u_{n+1} = u_n / D cell by cell
+ * Not pinned (parity unpinned): the reference's as-written alpha/beta path
+ * (only its limit is pinned) and the rounding of Julia's pairwise sum.
  */
 #ifndef TEALEAF_ORACLE_H
 #define TEALEAF_ORACLE_H

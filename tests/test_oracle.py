@@ -190,6 +190,23 @@ def test_operator_symmetric_unit_row_sums():
     np.testing.assert_allclose(o.interior("w"), 1.0, rtol=1e-12)
 
 
+def test_residual_matches_assembled_operator():
+    # residual!, kernels.jl:227-232: r = u0 - A u on the interior, A with the
+    # reflective boundary; checked against the independently assembled matrix
+    o = _make(decks.random_rect_deck(4, n=20, steps=1))
+    o.cg_init()
+    rng = np.random.default_rng(5)
+    h = o.hd
+    u = rng.standard_normal((20, 20))
+    u0 = rng.standard_normal((20, 20))
+    o.field("u")[h:-h, h:-h] = u
+    o.field("u0")[h:-h, h:-h] = u0
+    o.halo_update(O.X_U)
+    o.residual()
+    want = u0.ravel() - _assemble(o) @ u.ravel()
+    np.testing.assert_allclose(o.interior("r").ravel(), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
 def test_halo_intended_and_literal_indices():
     t = _uniform_deck(6, 1.0, 1.0)
     for lit in (0, O.LIT_D2):

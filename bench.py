@@ -181,6 +181,33 @@ def cpu_baseline(n, seed, gpu_iters, budget_s):
     }
 
 
+def solvers_run(n, steps):
+    """The 3-state reference deck at n^2 (presteps 8, 10 PPCG inner steps, eps
+    1e-15, each solver's own stop rule) for CG, Chebyshev and PPCG: one JSON
+    line per solver with s/timestep, iterations and field summaries, then the
+    largest relative summary difference from CG.  For kernel traces of the
+    FP64 solver kernels."""
+    res = {}
+    for solver in ("cg", "cheby", "ppcg"):
+        s = T.Settings.from_text(decks.reference_deck(n, steps, solver=solver, presteps=8, ppcg_inner_steps=10))
+        chunk = T.initialiseapp(s)
+        times, its, summ = [], [], []
+        for k in range(steps):
+            t0 = time.perf_counter()
+            st = T.step(chunk, s)
+            sm = T.fieldsummary(chunk)
+            times.append(time.perf_counter() - t0)
+            its.append([st["iterations"], st["cg_iters"], st["cheby_iters"]])
+            summ.append(sm)
+        chunk.close()
+        res[solver] = np.array(summ)
+        print(json.dumps({"solver": solver, "n": n, "s_per_step": [round(x, 4) for x in times],
+                          "iterations_cg_cheby": its, "summary": [list(map(float, x)) for x in summ]}), flush=True)
+    for solver in ("cheby", "ppcg"):
+        d = float(np.max(np.abs(res[solver] - res["cg"]) / np.abs(res["cg"])))
+        print(json.dumps({"solver": solver, "max_rel_summary_diff_vs_cg": d}), flush=True)
+
+
 def kernel_repeat(n, seed, kid, reps):
     """Back-to-back launches of one kernel on the 8192^2 problem (for
     counter runs); kid -1 is a same-size copy kernel (16 B/cell)."""
@@ -202,6 +229,8 @@ def main():
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-repeat", type=int, nargs=2, metavar=("KERNEL_ID", "REPS"))
+    ap.add_argument("--solvers", action="store_true",
+                    help="CG, Chebyshev and PPCG on the 3-state reference deck at --n (default 4096 here)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the solve would shard by 2-D domain decomposition "
@@ -210,6 +239,9 @@ def main():
         sys.exit(2)
     if a.kernel_repeat:
         kernel_repeat(a.n, a.seed, a.kernel_repeat[0], a.kernel_repeat[1])
+        return
+    if a.solvers:
+        solvers_run(a.n if a.n != 8192 else 4096, a.steps)
         return
     learned = os.path.exists(WEIGHTS)
     warm = "learned" if learned else "none"
@@ -236,6 +268,8 @@ def main():
             "iterations": [st["iterations"] for st in flag["stats"]],
             "refreshes": [st["refreshes"] for st in flag["stats"]],
             "fallback": [st["fallback"] for st in flag["stats"]],
+            "start": [["previous", "learned", "smoothed previous"][st["start"]] for st in flag["stats"]],
+            "init_rel_resid": [st["init_rel_resid"] for st in flag["stats"]],
             "rel_resid": [st["rel_resid"] for st in flag["stats"]],
         },
     }
@@ -252,6 +286,18 @@ def main():
         diffs = [float(np.max(np.abs(f - b) / np.abs(b))) for f, b in zip(flag["summaries"], base["summaries"])]
         out["field_summary_max_rel_diff_vs_fp64"] = max(diffs)
     out["roofline"] = roofline_from(flag["prof"], a.n)
+    # every profiled kernel of the flagship's timed steps, and the share of the
+    # timed wall time they cover (launches past a solve's stop excluded)
+    kern = {}
+    for kid, d in flag["prof"].items():
+        if len(d) == 0:
+            continue
+        live = d[d > 0.25 * float(np.median(d))]
+        kern[_lib.KERNEL_NAMES[kid]] = {"launches": int(len(live)), "avg_ms": round(float(live.mean()), 5),
+                                         "total_s": round(float(live.sum()) * 1e-3, 4),
+                                         "GBps": round(KERNEL_BYTES[kid] * a.n * a.n / (float(live.mean()) * 1e-3) / 1e9, 1)}
+    out["flagship_kernels"] = kern
+    out["flagship_inner_cg_time_frac"] = round(sum(k["total_s"] for k in kern.values()) / flag["total"], 4)
     if not a.no_cpu:
         it = (np.mean([st["iterations"] for st in base["stats"]]) if base is not None
               else np.mean([st["iterations"] for st in flag["stats"]]))

@@ -47,7 +47,8 @@ extern "C" {
 
 /* initial guess of the solve */
 #define TL_WARM_NONE 0      /* u = energy*density, the reference's start (CG.jl:52) */
-#define TL_WARM_LEARNED 1   /* U-Net prediction; falls back to TL_WARM_NONE when its residual is worse */
+#define TL_WARM_LEARNED 1   /* U-Net prediction; the start with the smallest residual among it, the
+                               smoothed previous-step start and the previous-step start is used */
 #define TL_WARM_GUESS 2     /* caller-provided guess set with tl_set_guess, same fallback */
 
 /* field ids, in the order of the Chunk struct (chunk.jl:25-38) */
@@ -110,8 +111,15 @@ typedef struct {
     double rel_resid;     /* ||u0 - A u|| / ||u0|| of the returned u, FP64 */
     double init_rel_resid;/* same for the initial guess that was used */
     double solve_ms;      /* wall time of tl_solve */
-    double reserved[4];
+    int start;            /* initial guess used: TL_START_* */
+    int reserved_i;
+    double reserved[3];
 } tl_solve_stats;
+
+/* tl_solve_stats.start */
+#define TL_START_PREVIOUS 0 /* u = energy*density (the reference's start) */
+#define TL_START_WARM 1     /* the learned prediction or the caller's guess */
+#define TL_START_SMOOTHED 2 /* TL_WARM_LEARNED only: the previous-step start after the same smoothing sweeps */
 
 /* a parsed tea.in deck (settings.jl:90-173); states keep the deck's coordinates */
 #define TL_MAX_STATES 128

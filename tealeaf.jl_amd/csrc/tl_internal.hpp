@@ -24,16 +24,27 @@ namespace tl {
 
 constexpr int TB = 256;        // threads per block (one row segment of 256 cells)
 constexpr int XCOL = 32;       // storage column of the first interior cell
-// vectorised CG kernels: rows per load batch, and the block count their
-// geometry aims at on large grids
+// vectorised CG kernels: threads per block, rows per load batch, and the
+// block count their geometry aims at on large grids
+#ifndef TL_TBV
+#define TL_TBV 256
+#endif
+constexpr int TBV = TL_TBV;
 #ifndef TL_RV
 #define TL_RV 4
 #endif
 #ifndef TL_VEC_BLOCKS32
 #define TL_VEC_BLOCKS32 2048
 #endif
+#ifndef TL_VEC_BLOCKS32_PW
+#define TL_VEC_BLOCKS32_PW 1024
+#endif
 #ifndef TL_VEC_BLOCKS64
 #define TL_VEC_BLOCKS64 8192
+#endif
+// XCD-aware block-to-tile mapping of the vectorised kernels
+#ifndef TL_XCD_MAP
+#define TL_XCD_MAP 0  // measured at 8192^2: 1.5% less traffic but 5-8% slower kernels
 #endif
 // optional occupancy target (waves per SIMD) of the FP32 inner-CG kernels
 #ifdef TL_WPE32
@@ -56,6 +67,7 @@ struct Geo {
     long off;                  // element offset of interior (0, 0)
     int gx, gy, ty;            // stencil launch: gx*gy blocks of TB columns x ty rows
     int gx4, gy4, ty4;         // FP32 vector kernels: blocks of 4*TB columns x ty4 rows
+    int gyp4, typ4;            // FP32 PW kernel: gx4*gyp4 blocks of 4*TB columns x typ4 rows
     int gx2, gy2, ty2;         // FP64 vector kernels: blocks of 2*TB columns x ty2 rows
     int nbmax;                 // stride of the partial-sum arrays (max block count)
     size_t elems;              // elements per allocation
@@ -105,6 +117,7 @@ struct tl_ctx {
     // FP64 fields (reference Chunk fields; u and sd have ping-pong twins)
     double *f[TL_NFIELDS] = {};
     double *u_alt = nullptr, *sd_alt = nullptr, *guess = nullptr;
+    double *guess2 = nullptr;  // smoothed previous-step start (learned warm start)
     // FP32 fields of the mixed-precision solve
     // FP32 inner CG: cell coefficient gc = 1/w and its face scale (rx/2, ry/2),
     // search direction, residual, correction

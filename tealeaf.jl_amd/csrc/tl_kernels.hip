@@ -54,20 +54,26 @@ Geo make_geo(int nx, int ny, int hd)
     g.gy = (ny + ty - 1) / ty;
     // vector kernels: the same target block count, 4 (FP32) or 2 (FP64) columns per thread
     // (measured at 8192^2: the FP64 kernels run fastest with about 8192
-    // blocks, the FP32 ones with about 2048)
+    // blocks, the FP32 UR with about 2048 and the FP32 PW with about 1024)
     auto vec = [&](int v, long target, int &gxv, int &gyv, int &tyv) {
-        gxv = (nx + v * TB - 1) / (v * TB);
+        gxv = (nx + v * TBV - 1) / (v * TBV);
         const long r2 = (long)ny * gxv;
         int t2 = (int)((r2 + target - 1) / target);
         t2 = (t2 + TL_RV - 1) / TL_RV * TL_RV;  // a multiple of the vector kernels' row chunk
-        if (t2 < TL_RV) t2 = TL_RV;
+        // bands of at least 16 rows (each re-reads the row above and below
+        // it) unless that leaves fewer than about 1024 blocks
+        const long tcap = std::min<long>(16, std::max<long>(TL_RV, r2 / 1024));
+        const int tmin = (int)((tcap + TL_RV - 1) / TL_RV * TL_RV);
+        if (t2 < tmin) t2 = tmin;
         if (t2 > 1024) t2 = 1024;
         tyv = t2;
         gyv = (ny + t2 - 1) / t2;
     };
     vec(4, TL_VEC_BLOCKS32, g.gx4, g.gy4, g.ty4);
+    int gxp;
+    vec(4, TL_VEC_BLOCKS32_PW, gxp, g.gyp4, g.typ4);  // FP32 PW: same columns, its own bands
     vec(2, TL_VEC_BLOCKS64, g.gx2, g.gy2, g.ty2);
-    g.nbmax = std::max(g.gx * g.gy, std::max(g.gx4 * g.gy4, g.gx2 * g.gy2));
+    g.nbmax = std::max(std::max(g.gx * g.gy, g.gx4 * g.gyp4), std::max(g.gx4 * g.gy4, g.gx2 * g.gy2));
     return g;
 }
 
@@ -83,10 +89,10 @@ __device__ __forceinline__ double wave_sum(double v)
 __device__ __forceinline__ unsigned block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
 
 // one partial per block: part[a * stride + block] = sum over the block of v[a]
-template <int N>
+template <int N, int NT = TB>
 __device__ __forceinline__ void write_partials(const double (&v)[N], double *__restrict__ part, size_t stride)
 {
-    __shared__ double sh[N][TB / 64];
+    __shared__ double sh[N][NT / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < N; ++a) {
@@ -100,27 +106,27 @@ __device__ __forceinline__ void write_partials(const double (&v)[N], double *__r
         for (int a = 0; a < N; ++a) {
             double s = sh[a][0];
 #pragma unroll
-            for (int q = 1; q < TB / 64; ++q) s += sh[a][q];
+            for (int q = 1; q < NT / 64; ++q) s += sh[a][q];
             part[(size_t)a * stride + b] = s;
         }
     }
 }
 
 // every thread of the block gets the fixed-order sum of nb partials
-template <int N>
+template <int N, int NT = TB>
 __device__ __forceinline__ void sum_partials(const double *__restrict__ part, unsigned nb, size_t stride,
                                              double (&tot)[N])
 {
-    __shared__ double sh[N][TB / 64];
+    __shared__ double sh[N][NT / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         double s = 0.0;
-        for (unsigned q0 = threadIdx.x; q0 < nb; q0 += 8 * TB) {
+        for (unsigned q0 = threadIdx.x; q0 < nb; q0 += 8 * NT) {
             double v[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const unsigned q = q0 + (unsigned)k * TB;
+                const unsigned q = q0 + (unsigned)k * NT;
                 v[k] = q < nb ? part[(size_t)a * stride + q] : 0.0;
             }
 #pragma unroll
@@ -134,7 +140,7 @@ __device__ __forceinline__ void sum_partials(const double *__restrict__ part, un
     for (int a = 0; a < N; ++a) {
         double s = sh[a][0];
 #pragma unroll
-        for (int q = 1; q < TB / 64; ++q) s += sh[a][q];
+        for (int q = 1; q < NT / 64; ++q) s += sh[a][q];
         tot[a] = s;
     }
     __syncthreads();  // sh may be reused by the next call
@@ -559,14 +565,15 @@ __global__ __launch_bounds__(TB) void k_cg_w(Geo g, int t, const T *__restrict__
 }
 
 // alpha = rro / pw (CG.jl:35), derived identically by every block
+template <int NT = TB>
 __device__ __forceinline__ bool cg_alpha(const Geo &g, int t, Ctrl *__restrict__ ctrl,
                                          const double *__restrict__ pw, unsigned nbw,
                                          const double *__restrict__ prprev, unsigned nbp,
                                          double *__restrict__ hist_a, double *alpha)
 {
     double a1[1], a2[1];
-    sum_partials<1>(pw, nbw, g.nbmax, a1);
-    sum_partials<1>(prprev, nbp, g.nbmax, a2);
+    sum_partials<1, NT>(pw, nbw, g.nbmax, a1);
+    sum_partials<1, NT>(prprev, nbp, g.nbmax, a2);
     const double pwv = a1[0], rro = a2[0];
     const double a = rro / pwv;
     const bool ok = pwv > 0.0 && isfinite(a);
@@ -727,13 +734,36 @@ struct VCol {
     int lane;
 };
 
+// Tile of this block.  The vector kernels launch a 1-D grid of gx*gy blocks,
+// row-major over the tiles.  With TL_XCD_MAP the hardware's round-robin
+// (block b to XCD b % 8) is undone, tile = (b % 8) * (nb / 8) + b / 8, so
+// each XCD works on a contiguous slab of row bands and the rows and columns
+// a block re-reads from its neighbours' tiles stay in one L2; measured at
+// 8192^2 this saves 1.5% of the fetched bytes but slows the kernels by 5-8%,
+// so it is off by default.
+template <int V>
+__device__ __forceinline__ void vtile(const Geo &g, int &bx, int &by)
+{
+    const int gxv = V == 4 ? g.gx4 : g.gx2;
+    const unsigned b = blockIdx.x;
+    unsigned tile = b;
+#if TL_XCD_MAP
+    const unsigned nb = gridDim.x;
+    if (nb % 8 == 0) tile = (b % 8) * (nb / 8) + b / 8;
+#endif
+    bx = (int)(tile % (unsigned)gxv);
+    by = (int)(tile / (unsigned)gxv);
+}
+
 template <int V>
 __device__ __forceinline__ VCol<V> vcolumn(const Geo &g, int ty)
 {
     VCol<V> c;
-    c.x0 = (blockIdx.x * TB + threadIdx.x) * V;
+    int bx, by;
+    vtile<V>(g, bx, by);
+    c.x0 = (bx * TBV + threadIdx.x) * V;
     c.xl = min(c.x0, ((g.nx - 1) / V) * V);
-    c.y0 = blockIdx.y * ty;
+    c.y0 = by * ty;
     c.y1 = min(c.y0 + ty, g.ny);
     c.active = c.x0 < g.nx;
     c.lane = threadIdx.x & 63;
@@ -773,7 +803,7 @@ __device__ __forceinline__ T vsmvp(const Geo &g, int xl, int j, const T (&c)[V],
 // thread reads its neighbours' old p), then w = A p and the partial sums of
 // p.w (CG.w!).  For t = 1, p comes from CG.init!.
 template <typename T, int V>
-__global__ __launch_bounds__(TB) void k_cg_pwv(Geo g, int ty, int t, const T *__restrict__ pin, T *__restrict__ pout,
+__global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *__restrict__ pin, T *__restrict__ pout,
                                                const T *__restrict__ r, const T *__restrict__ kx,
                                                const T *__restrict__ ky, T *__restrict__ w, Ctrl *__restrict__ ctrl,
                                                double *__restrict__ pw, const double *__restrict__ prprev,
@@ -785,8 +815,8 @@ __global__ __launch_bounds__(TB) void k_cg_pwv(Geo g, int ty, int t, const T *__
     const bool fuse = t >= 2;
     if (fuse) {
         double a1[1], a2[1];
-        sum_partials<1>(prcur, nbc, g.nbmax, a1);
-        sum_partials<1>(prprev, nbp, g.nbmax, a2);
+        sum_partials<1, TBV>(prcur, nbc, g.nbmax, a1);
+        sum_partials<1, TBV>(prprev, nbp, g.nbmax, a2);
         const double rrn = a1[0], rro = a2[0];
         const double b = rrn / rro;
         const int why = decide_why(ctrl, t - 1, rrn);
@@ -878,13 +908,13 @@ __global__ __launch_bounds__(TB) void k_cg_pwv(Geo g, int ty, int t, const T *__
             }
         }
     }
-    write_partials<1>(acc, pw, g.nbmax);
+    write_partials<1, TBV>(acc, pw, g.nbmax);
 }
 
 // CG.ur!, CG.jl:93-98 on V-vectors: alpha = rro/pw from the partials, then
 // u += alpha p; r -= alpha w; partial sums of r^2
 template <typename T, int V>
-__global__ __launch_bounds__(TB) void k_cg_urv(Geo g, int ty, int t, T *__restrict__ u, const T *__restrict__ p,
+__global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restrict__ u, const T *__restrict__ p,
                                                T *__restrict__ r, const T *__restrict__ w, Ctrl *__restrict__ ctrl,
                                                const double *__restrict__ pw, unsigned nbw,
                                                const double *__restrict__ prprev, unsigned nbp,
@@ -892,7 +922,7 @@ __global__ __launch_bounds__(TB) void k_cg_urv(Geo g, int ty, int t, T *__restri
 {
     if (t > ctrl->stop_iter) return;
     double ad;
-    if (!cg_alpha(g, t, ctrl, pw, nbw, prprev, nbp, hist_a, &ad)) return;
+    if (!cg_alpha<TBV>(g, t, ctrl, pw, nbw, prprev, nbp, hist_a, &ad)) return;
     const T alpha = (T)ad;
     const VCol<V> cl = vcolumn<V>(g, ty);
     double acc[1] = {0.0};
@@ -924,7 +954,7 @@ __global__ __launch_bounds__(TB) void k_cg_urv(Geo g, int ty, int t, T *__restri
             }
         }
     }
-    write_partials<1>(acc, prcur, g.nbmax);
+    write_partials<1, TBV>(acc, prcur, g.nbmax);
 }
 
 // ------------------------------- FP32 inner CG on a cell-coefficient array
@@ -946,7 +976,7 @@ __global__ __launch_bounds__(TB) void k_cg_urv(Geo g, int ty, int t, T *__restri
 // UR = true (iteration t): alpha = rr(t-1)/pw(t), then x += alpha p,
 //   r -= alpha A p and partial sums of r^2 (CG.ur!, CG.jl:93-98).
 template <int V, bool UR>
-__global__ __launch_bounds__(TB) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, const float *__restrict__ pin,
+__global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, const float *__restrict__ pin,
                                             float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
                                             const float *__restrict__ gc, float hx, float hy,
                                             Ctrl *__restrict__ ctrl, const double *__restrict__ pw_in, unsigned nbw,
@@ -959,12 +989,12 @@ __global__ __launch_bounds__(TB) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, co
     const bool fuse = !UR && t >= 2;
     if (UR) {
         double ad;
-        if (!cg_alpha(g, t, ctrl, pw_in, nbw, prprev, nbp, nullptr, &ad)) return;
+        if (!cg_alpha<TBV>(g, t, ctrl, pw_in, nbw, prprev, nbp, nullptr, &ad)) return;
         alpha = (float)ad;
     } else if (fuse) {
         double a1[1], a2[1];
-        sum_partials<1>(prcur, nbc, g.nbmax, a1);
-        sum_partials<1>(prprev, nbp, g.nbmax, a2);
+        sum_partials<1, TBV>(prcur, nbc, g.nbmax, a1);
+        sum_partials<1, TBV>(prprev, nbp, g.nbmax, a2);
         const double rrn = a1[0], rro = a2[0];
         const double b = rrn / rro;
         const int why = decide_why(ctrl, t - 1, rrn);
@@ -1082,7 +1112,7 @@ __global__ __launch_bounds__(TB) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, co
             }
         }
     }
-    write_partials<1>(acc, part_out, g.nbmax);
+    write_partials<1, TBV>(acc, part_out, g.nbmax);
 }
 
 // ---------------------------------------------- residuals and summaries
@@ -1522,6 +1552,7 @@ static inline dim3 sgrid(const Geo &g) { return dim3(g.gx, g.gy); }
 static inline dim3 fgrid(const Geo &g) { return dim3((g.X + TB - 1) / TB, g.Y); }
 static inline unsigned nbh(const tl_ctx *c) { return (unsigned)(c->g.gx * c->g.gy); }
 static inline unsigned nb4(const tl_ctx *c) { return (unsigned)(c->g.gx4 * c->g.gy4); }
+static inline unsigned nb4p(const tl_ctx *c) { return (unsigned)(c->g.gx4 * c->g.gyp4); }
 static inline unsigned nb2(const tl_ctx *c) { return (unsigned)(c->g.gx2 * c->g.gy2); }
 static inline size_t stride(const tl_ctx *c) { return (size_t)c->g.nbmax; }
 // partial-sum arrays: [pw | pr0 | pr1 | pm0 pm1 pm2 pm3], nbmax doubles each
@@ -1657,7 +1688,7 @@ hipError_t launch_cg_p64(tl_ctx *c, int t)
 hipError_t launch_cg_pw64(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_W64);
-    k_cg_pwv<double, 2><<<dim3(c->g.gx2, c->g.gy2), TB, 0, c->stream>>>(
+    k_cg_pwv<double, 2><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, t, t >= 2 ? p64(c, t - 1) : nullptr, p64(c, t), c->f[TL_F_R], c->f[TL_F_KX], c->f[TL_F_KY],
         c->f[TL_F_W], c->ctrl, part_w(c), t >= 2 ? part_r(c, t - 2) : nullptr, t >= 2 ? nbr(c, t - 2) : 0,
         t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, c->hist_b);
@@ -1669,7 +1700,7 @@ hipError_t launch_cg_pw64(tl_ctx *c, int t)
 hipError_t launch_cg_urv64(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_UR64);
-    k_cg_urv<double, 2><<<dim3(c->g.gx2, c->g.gy2), TB, 0, c->stream>>>(
+    k_cg_urv<double, 2><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, t, c->f[TL_F_U], p64(c, t), c->f[TL_F_R], c->f[TL_F_W], c->ctrl, part_w(c), c->nb_w,
         part_r(c, t - 1), nbr(c, t - 1), part_r(c, t), c->hist_a);
     nbr(c, t) = nb2(c);
@@ -1689,11 +1720,11 @@ hipError_t launch_cg_pfinal64(tl_ctx *c, int T)
 hipError_t launch_cg_w32(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_W32);
-    k_cgg<4, false><<<dim3(c->g.gx4, c->g.gy4), TB, 0, c->stream>>>(
-        c->g, c->g.ty4, t, t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gc32, c->hx32, c->hy32,
+    k_cgg<4, false><<<nb4p(c), TBV, 0, c->stream>>>(
+        c->g, c->g.typ4, t, t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gc32, c->hx32, c->hy32,
         c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 2) : nullptr, t >= 2 ? nbr(c, t - 2) : 0,
         t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c));
-    c->nb_w = nb4(c);
+    c->nb_w = nb4p(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
 }
@@ -1701,7 +1732,7 @@ hipError_t launch_cg_w32(tl_ctx *c, int t)
 hipError_t launch_cg_ur32(tl_ctx *c, int t)
 {
     const bool prof = prof_start(c, TL_K_CG_UR32);
-    k_cgg<4, true><<<dim3(c->g.gx4, c->g.gy4), TB, 0, c->stream>>>(
+    k_cgg<4, true><<<nb4(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty4, t, nullptr, p32(c, t), c->r32, c->x32, c->gc32, c->hx32, c->hy32, c->ctrl, part_w(c),
         c->nb_w, part_r(c, t - 1), nbr(c, t - 1), nullptr, 0, part_r(c, t));
     nbr(c, t) = nb4(c);

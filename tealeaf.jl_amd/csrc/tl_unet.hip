@@ -125,19 +125,11 @@ __global__ __launch_bounds__(256) void k_conv(TV in, TV out, const bf16 *__restr
     }
 }
 
-// zero a whole bordered tensor (all channels)
-__global__ void k_zero16(bf16 *p, long n)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 // 2x2 average pool: out (H/2) from in (H) view
 __global__ void k_pool(TV in, TV out, int C)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)out.B * out.H * out.W * C;
-    if (t >= n) return;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     const int c = (int)(t % C);
     long r = t / C;
     const int x = (int)(r % out.W);
@@ -147,14 +139,14 @@ __global__ void k_pool(TV in, TV out, int C)
     float s = bf2f(in.p[in.idx(b, 2 * y, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y, 2 * x + 1) + c]) +
               bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x + 1) + c]);
     out.p[out.idx(b, y, x) + c] = f2bf(0.25f * s);
+    }
 }
 
 // nearest x2 upsample: out (2H) from in (H)
 __global__ void k_up(TV in, TV out, int C)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)out.B * out.H * out.W * C;
-    if (t >= n) return;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     const int c = (int)(t % C);
     long r = t / C;
     const int x = (int)(r % out.W);
@@ -162,14 +154,14 @@ __global__ void k_up(TV in, TV out, int C)
     const int y = (int)(r % out.H);
     const int b = (int)(r / out.H);
     out.p[out.idx(b, y, x) + c] = in.p[in.idx(b, y / 2, x / 2) + c];
+    }
 }
 
 // backward of the upsample: out (H) = sum of the 2x2 block of in (2H)
 __global__ void k_up_back(TV in, TV out, int C)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)out.B * out.H * out.W * C;
-    if (t >= n) return;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     const int c = (int)(t % C);
     long r = t / C;
     const int x = (int)(r % out.W);
@@ -179,15 +171,15 @@ __global__ void k_up_back(TV in, TV out, int C)
     float s = bf2f(in.p[in.idx(b, 2 * y, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y, 2 * x + 1) + c]) +
               bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x + 1) + c]);
     out.p[out.idx(b, y, x) + c] = f2bf(s);
+    }
 }
 
 // dz = (g [+ pool_back(gp)]) * (y > 0 if relu): the gradient at a conv's
 // pre-activation; gp (H/2) is the gradient of the 2x2 pool that reads y
 __global__ void k_dz(TV g, TV gp, int has_gp, TV y, int relu, TV dz, int C)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)dz.B * dz.H * dz.W * C;
-    if (t >= n) return;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     const int c = (int)(t % C);
     long r = t / C;
     const int x = (int)(r % dz.W);
@@ -198,6 +190,7 @@ __global__ void k_dz(TV g, TV gp, int has_gp, TV y, int relu, TV dz, int C)
     if (has_gp) v += 0.25f * bf2f(gp.p[gp.idx(b, yy / 2, x / 2) + c]);
     if (relu && !(bf2f(y.p[y.idx(b, yy, x) + c]) > 0.f)) v = 0.f;
     dz.p[dz.idx(b, yy, x) + c] = f2bf(v);
+    }
 }
 
 // weight-gradient partials: block (chunk) sums over its pixels, thread = one
@@ -715,6 +708,9 @@ static int acts_ensure(tl_ctx *c, Unet *u, int B, int H, int W, bool grads)
 }
 
 static unsigned blocks(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+// grid of a grid-stride kernel: a dispatch counts its work-items in 32 bits,
+// so full-grid tensors of 16384^2 x 64 channels need the loop
+static unsigned gs_blocks(long n, int bs = 256) { return (unsigned)std::min<long>((n + bs - 1) / bs, 1L << 20); }
 
 static hipError_t conv(tl_ctx *c, const TV &in, const TV &out, const bf16 *w, const float *bias, int cinp, int coutp,
                        int relu)
@@ -753,16 +749,16 @@ static int forward(tl_ctx *c, Unet *u)
     const long n1 = (long)a.B * a.H * a.W, n2 = n1 / 4;
     UCHK(c, fwd_layer(c, u, 0, a.x0.v(), a.e1a.v()));
     UCHK(c, fwd_layer(c, u, 1, a.e1a.v(), a.c1.v(32)));
-    k_pool<<<blocks(n2 * 16), 256, 0, c->stream>>>(a.c1.v(32), a.p1.v(), 16);
+    k_pool<<<gs_blocks(n2 * 16), 256, 0, c->stream>>>(a.c1.v(32), a.p1.v(), 16);
     UCHK(c, fwd_layer(c, u, 2, a.p1.v(), a.e2a.v()));
     UCHK(c, fwd_layer(c, u, 3, a.e2a.v(), a.c2.v(64)));
-    k_pool<<<blocks(n2 / 4 * 32), 256, 0, c->stream>>>(a.c2.v(64), a.p2.v(), 32);
+    k_pool<<<gs_blocks(n2 / 4 * 32), 256, 0, c->stream>>>(a.c2.v(64), a.p2.v(), 32);
     UCHK(c, fwd_layer(c, u, 4, a.p2.v(), a.e3a.v()));
     UCHK(c, fwd_layer(c, u, 5, a.e3a.v(), a.e3b.v()));
-    k_up<<<blocks(n2 * 64), 256, 0, c->stream>>>(a.e3b.v(), a.c2.v(0), 64);
+    k_up<<<gs_blocks(n2 * 64), 256, 0, c->stream>>>(a.e3b.v(), a.c2.v(0), 64);
     UCHK(c, fwd_layer(c, u, 6, a.c2.v(), a.d2a.v()));
     UCHK(c, fwd_layer(c, u, 7, a.d2a.v(), a.d2b.v()));
-    k_up<<<blocks(n1 * 32), 256, 0, c->stream>>>(a.d2b.v(), a.c1.v(0), 32);
+    k_up<<<gs_blocks(n1 * 32), 256, 0, c->stream>>>(a.d2b.v(), a.c1.v(0), 32);
     UCHK(c, fwd_layer(c, u, 8, a.c1.v(), a.d1a.v()));
     UCHK(c, fwd_layer(c, u, 9, a.d1a.v(), a.d1b.v()));
     UCHK(c, fwd_layer(c, u, 10, a.d1b.v(), a.out.v()));
@@ -780,7 +776,7 @@ static int back_layer(tl_ctx *c, Unet *u, int l, const TV &g, const TV &gp, int 
     const long npx = (long)y.B * y.H * y.W;
     // the buffer is shared by all layers: clear it so its border is zero
     UCHK(c, hipMemsetAsync(a.dz.p, 0, (size_t)y.B * (y.H + 2) * (y.W + 2) * L.coutp * sizeof(bf16), c->stream));
-    k_dz<<<blocks(npx * L.coutp), 256, 0, c->stream>>>(g, gp, has_gp, y, L.relu, dz, L.coutp);
+    k_dz<<<gs_blocks(npx * L.coutp), 256, 0, c->stream>>>(g, gp, has_gp, y, L.relu, dz, L.coutp);
     // weight gradient: partials over pixel chunks, then a fixed-order sum
     const int px_chunk = 1024;
     const int nchunk = (int)((npx + px_chunk - 1) / px_chunk);
@@ -816,12 +812,12 @@ static int backward(tl_ctx *c, Unet *u)
     RB(back_layer(c, u, 9, a.g_d1b.v(), none, 0, a.d1b.v(), a.d1a.v(), &t));
     t = a.g_c1.v();
     RB(back_layer(c, u, 8, a.g_d1a.v(), none, 0, a.d1a.v(), a.c1.v(), &t));
-    k_up_back<<<blocks(n2 * 32), 256, 0, c->stream>>>(a.g_c1.v(0), a.g_d2b.v(), 32);
+    k_up_back<<<gs_blocks(n2 * 32), 256, 0, c->stream>>>(a.g_c1.v(0), a.g_d2b.v(), 32);
     t = a.g_d2a.v();
     RB(back_layer(c, u, 7, a.g_d2b.v(), none, 0, a.d2b.v(), a.d2a.v(), &t));
     t = a.g_c2.v();
     RB(back_layer(c, u, 6, a.g_d2a.v(), none, 0, a.d2a.v(), a.c2.v(), &t));
-    k_up_back<<<blocks(n2 / 4 * 64), 256, 0, c->stream>>>(a.g_c2.v(0), a.g_e3b.v(), 64);
+    k_up_back<<<gs_blocks(n2 / 4 * 64), 256, 0, c->stream>>>(a.g_c2.v(0), a.g_e3b.v(), 64);
     t = a.g_e3a.v();
     RB(back_layer(c, u, 5, a.g_e3b.v(), none, 0, a.e3b.v(), a.e3a.v(), &t));
     t = a.g_p2.v();

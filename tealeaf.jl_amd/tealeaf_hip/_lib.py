@@ -33,6 +33,7 @@ EXPORTS = [
     "tl_unet_predict", "tl_profile_begin", "tl_profile_read", "tl_profile_end",
     "tl_kernel_repeat", "tl_unet_eval", "tl_checking_value", "tl_qa_check",
 ]
+TL_START_PREVIOUS, TL_START_WARM, TL_START_SMOOTHED = 0, 1, 2
 TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32 = range(6)
 KERNEL_NAMES = {0: "k_cg_pwv<double, 2>", 1: "k_cg_urv<double, 2>", 2: "k_cg_p<double>",
                 3: "k_cgg<4, false>", 4: "k_cgg<4, true>", 5: "(fused into k_cgg<4, false>)",
@@ -53,11 +54,12 @@ class SolveStats(C.Structure):
         ("iterations", C.c_int), ("cg_iters", C.c_int), ("cheby_iters", C.c_int), ("est_iters", C.c_int),
         ("inner_iters", C.c_int), ("refreshes", C.c_int), ("converged", C.c_int), ("fallback", C.c_int),
         ("error", C.c_double), ("eigmin", C.c_double), ("eigmax", C.c_double), ("rel_resid", C.c_double),
-        ("init_rel_resid", C.c_double), ("solve_ms", C.c_double), ("reserved", C.c_double * 4),
+        ("init_rel_resid", C.c_double), ("solve_ms", C.c_double), ("start", C.c_int), ("reserved_i", C.c_int),
+        ("reserved", C.c_double * 3),
     ]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+        return {n: getattr(self, n) for n, _ in self._fields_ if not n.startswith("reserved")}
 
 
 class State(C.Structure):

@@ -13,8 +13,10 @@ what it gains.
 2. Training: Adam on minibatches of crops, all on the device.
 3. Evaluation on unseen seeds at --eval-res: the same timesteps solved from
    the previous-step guess and from the learned guess (with its fallback),
-   for FP64 CG and for FP32-IR CG; iterations, initial residuals and solve
-   times per step are printed as JSON lines.
+   for FP64 CG and for FP32-IR CG; iterations, initial residuals, the start
+   chosen (0 previous-step, 1 learned, 2 smoothed previous-step) and solve
+   times per step are printed as JSON lines.  The "smooth_only" arm runs the
+   learned path with an all-zero network (the smoothing sweeps alone).
 """
 from __future__ import annotations
 
@@ -85,7 +87,7 @@ def evaluate(w, a):
                 c = initialiseapp(s)
                 if arm != "none":
                     unet.set_weights(c, w if arm == "learned" else np.zeros_like(w))
-                its, secs, init_r, fb, summ = [], [], [], [], []
+                its, secs, init_r, fb, summ, starts = [], [], [], [], [], []
                 for t in range(a.eval_steps):
                     t0 = time.time()
                     st = step(c, s)
@@ -94,8 +96,9 @@ def evaluate(w, a):
                     its.append(st["iterations"])
                     init_r.append(st["init_rel_resid"])
                     fb.append(st["fallback"])
+                    starts.append(st["start"])
                 res[arm] = dict(iterations=its, seconds=[round(x, 3) for x in secs], init_rel_resid=init_r,
-                                 fallback=fb, summary=np.array(summ))
+                                 fallback=fb, start=starts, summary=np.array(summ))
                 c.close()
             d = np.max(np.abs(res["learned"]["summary"] - res["none"]["summary"]) / np.abs(res["none"]["summary"]))
             row = dict(phase="eval", seed=seed, res=a.eval_res, precision=prec,
@@ -104,7 +107,8 @@ def evaluate(w, a):
                        seconds_prev_guess=res["none"]["seconds"], seconds_learned=res["learned"]["seconds"],
                        init_rel_resid_prev_guess=res["none"]["init_rel_resid"],
                        init_rel_resid_learned=res["learned"]["init_rel_resid"],
-                       fallback=res["learned"]["fallback"], summary_max_rel_diff=float(d),
+                       fallback=res["learned"]["fallback"], start_learned=res["learned"]["start"],
+                       summary_max_rel_diff=float(d),
                        iterations_smooth_only=res["smooth_only"]["iterations"],
                        init_rel_resid_smooth_only=res["smooth_only"]["init_rel_resid"],
                        fallback_smooth_only=res["smooth_only"]["fallback"])

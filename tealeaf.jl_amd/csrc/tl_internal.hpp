@@ -175,7 +175,6 @@ hipError_t launch_cg_ur64(tl_ctx *c, int t);
 hipError_t launch_cg_p64(tl_ctx *c, int t);
 hipError_t launch_cg_w32(tl_ctx *c, int t);
 hipError_t launch_cg_ur32(tl_ctx *c, int t);
-hipError_t launch_cg_p32(tl_ctx *c, int t);
 hipError_t launch_residual(tl_ctx *c);
 hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t);
 hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next);

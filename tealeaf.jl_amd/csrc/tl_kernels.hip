@@ -1740,13 +1740,6 @@ hipError_t launch_cg_ur32(tl_ctx *c, int t)
     return hipGetLastError();
 }
 
-hipError_t launch_cg_p32(tl_ctx *c, int t)
-{
-    (void)c;
-    (void)t;
-    return hipSuccess;  // fused into the next iteration's launch_cg_w32
-}
-
 hipError_t launch_residual(tl_ctx *c)
 {
     k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],

@@ -26,7 +26,7 @@ from ._lib import (FID, FIELDS, TL_CG, TL_CHEBY, TL_FP32_IR, TL_FP64, TL_JACOBI,
 
 __all__ = [
     "Settings", "Chunk", "CG", "Cheby", "PPCG", "Jacobi", "initialiseapp", "diffuse", "fieldsummary",
-    "solvefinished", "checkingvalue", "qa_check", "TeaLeafError", "load", "FIELDS",
+    "solvefinished", "checkingvalue", "qa_check", "debugrecord", "TeaLeafError", "load", "FIELDS",
 ]
 
 
@@ -237,6 +237,45 @@ def qa_check(chunk: Chunk, settings: Settings, problemfile: str = "tea.problems"
     d = C.c_double()
     passed = bool(load().tl_qa_check(actual, cv, C.byref(d)))
     return {"actual": actual, "checking_value": cv, "qa_diff": d.value, "passed": passed}
+
+
+def _julia_float(x: float) -> str:
+    """A Float64 as Julia's print writes it: the shortest round-trip digits,
+    with "e" notation outside 1e-4 <= |x| < 1e6 ("1.0e-5", "1.234567e6"),
+    ".0" on integral values and "NaN"/"Inf" spellings."""
+    if x != x:
+        return "NaN"
+    if x in (float("inf"), float("-inf")):
+        return "Inf" if x > 0 else "-Inf"
+    if x == 0.0:
+        return "-0.0" if str(x).startswith("-") else "0.0"
+    ax = abs(x)
+    if 1e-4 <= ax < 1e6:
+        r = repr(x)
+        if "e" in r:  # Python switches to exponents earlier for small values
+            r = np.format_float_positional(x, unique=True, trim="-")
+        return r if ("." in r) else r + ".0"
+    m, e = np.format_float_scientific(x, unique=True, trim="-").split("e")
+    if "." not in m:
+        m += ".0"
+    return f"{m}e{int(e)}"
+
+
+def debugrecord(chunk: Chunk, path: str):
+    """debugrecord, TeaLeaf.jl:90-102: append every field of the chunk to
+    `path` -- the field name, then one line per column j (the values of
+    k = 1..X separated by spaces, getstring of TeaLeaf.jl:104-106), then an
+    empty line; two newlines after the chunk.  The device holds the 13
+    fields of chunk.jl:25-38 (not the mesh vectors or the solver history,
+    which the reference also writes).  The number format follows Julia's
+    print; it is not checked against Julia (none here)."""
+    with open(path, "a") as f:
+        for name in FIELDS:
+            a = chunk.field(name)  # [j, k]
+            f.write(name + "\n")
+            f.write("\n".join(" ".join(_julia_float(float(v)) for v in row) for row in a))
+            f.write("\n\n")
+        f.write("\n\n")
 
 
 def step(chunk: Chunk, settings: Settings) -> dict:

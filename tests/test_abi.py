@@ -102,3 +102,15 @@ def test_invalid_arguments_return_codes():
     ctx = C.c_void_p()
     assert L.tl_create(C.byref(ctx), 0, 10, 2, 0.0, 0.0, 1.0, 1.0, 0) == _lib.TL_E_INVALID
     assert L.tl_create(C.byref(ctx), 10, 10, 0, 0.0, 0.0, 1.0, 1.0, 0) == _lib.TL_E_INVALID
+
+
+def test_julia_float_format():
+    # Julia's print(::Float64): shortest digits, ".0" on integral values,
+    # "e" notation outside [1e-4, 1e6) (not checked against Julia: none here)
+    from tealeaf_hip import _julia_float as j
+    cases = {0.1: "0.1", 1e-5: "1.0e-5", 1e-4: "0.0001", 1e5: "100000.0", 1e6: "1.0e6", 1234567.0: "1.234567e6",
+             2.5: "2.5", 100.0: "100.0", -3e-7: "-3.0e-7", 0.0: "0.0", 9.99e-5: "9.99e-5"}
+    for x, want in cases.items():
+        assert j(x) == want, (x, j(x), want)
+    for x in np.random.default_rng(0).standard_normal(200) * 10.0 ** np.random.default_rng(1).integers(-8, 9, 200):
+        assert float(j(float(x))) == x  # round trip

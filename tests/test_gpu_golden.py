@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import tealeaf_hip as T
+from tealeaf_hip import decks
 
 pytestmark = pytest.mark.gpu
 
@@ -78,3 +79,19 @@ def test_cli_runs_deck_and_prints_passed():
     assert len(lines) == 11 and lines[-1]["check"] == "PASSED", lines[-1]
     g = _load("c1_512_cg.json")
     assert _rel(lines[9]["temp"], g["intended"]["summary"][9][3]) < RTOL
+
+
+def test_debugrecord_layout(tmp_path):
+    print("case: debugrecord dump of a 6x4 chunk", flush=True)
+    text = decks.reference_deck(6, 1).replace("y_cells=6", "y_cells=4")
+    s = T.Settings.from_text(text)
+    c = T.initialiseapp(s)
+    path = tmp_path / "debug.txt"
+    T.debugrecord(c, str(path))
+    lines = path.read_text().split("\n")
+    assert lines[0] == "density0" and len(lines[1].split(" ")) == c.X
+    i = lines.index("energy")
+    rows = lines[i + 1:i + 1 + c.Y]
+    got = np.array([[float(v) for v in r.split(" ")] for r in rows])
+    assert np.array_equal(got, c.field("energy"))
+    assert [ln for ln in lines if ln in T.FIELDS] == list(T.FIELDS)

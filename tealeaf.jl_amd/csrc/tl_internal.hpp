@@ -96,6 +96,8 @@ struct Ctrl {
     double rel_tol;     // MODE_CG_REL / MODE_IR: tol_abs = rel_tol * sqrt(bb)
     double ir_delta;    // MODE_IR: refresh_abs = ir_delta * sqrt(rr_true)
     double sums[4];     // results of single-block finalizers (field summary, norms)
+    double rrs[2];      // vectorised CG: rr(t) in rrs[t & 1], written once per iteration by
+                        // block 0 of the kernel that sums it (or by the init/refresh finalizer)
     int iters;          // last completed iteration
     int maxiters;
     int mode;

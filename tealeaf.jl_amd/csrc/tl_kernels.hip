@@ -351,6 +351,7 @@ __global__ __launch_bounds__(TB) void k_fin(Ctrl *__restrict__ c, const double *
     switch (kind) {
     case FK_INIT:  // after CG.init!: rro = sum r^2, bb = sum u0^2
         c->rro = v[0];
+        c->rrs[0] = v[0];
         c->err = v[0];
         c->bb = v[1];
         c->tol_abs = c->rel_tol * sqrt(v[1]);
@@ -378,9 +379,10 @@ __global__ __launch_bounds__(TB) void k_fin(Ctrl *__restrict__ c, const double *
         record_stop(c, t, decide_why(c, t, err));
         break;
     }
-    case FK_IR: {  // FP64 refresh of the mixed-precision CG: converged or resume
+    case FK_IR: {  // FP64 refresh of the mixed-precision CG: converged or resume (t = last iteration)
         const double rr = v[0];
         c->rro = rr;
+        c->rrs[t & 1] = rr;
         c->err = rr;
         c->refresh_abs = c->ir_delta * sqrt(rr);
         if (!isfinite(rr)) {
@@ -575,6 +577,32 @@ __device__ __forceinline__ bool cg_alpha(const Geo &g, int t, Ctrl *__restrict__
     sum_partials<1, NT>(pw, nbw, g.nbmax, a1);
     sum_partials<1, NT>(prprev, nbp, g.nbmax, a2);
     const double pwv = a1[0], rro = a2[0];
+    const double a = rro / pwv;
+    const bool ok = pwv > 0.0 && isfinite(a);
+    if (block_id() == 0 && threadIdx.x == 0) {
+        ctrl->pw = pwv;
+        ctrl->alpha = a;
+        if (hist_a) hist_a[t - 1] = a;
+        if (!ok) {  // non-positive curvature: stop before this iteration's update
+            ctrl->status = TL_E_BREAKDOWN;
+            ctrl->why = WHY_BREAKDOWN;
+            ctrl->stop_iter = t - 1;
+        }
+    }
+    *alpha = a;
+    return ok;
+}
+
+// alpha = rr(t-1) / pw(t) for the vectorised kernels: pw summed from the
+// partials, rr(t-1) from the slot the previous reduction left in Ctrl
+template <int NT>
+__device__ __forceinline__ bool cg_alpha_slot(const Geo &g, int t, Ctrl *__restrict__ ctrl,
+                                              const double *__restrict__ pw, unsigned nbw,
+                                              double *__restrict__ hist_a, double *alpha)
+{
+    double a1[1];
+    sum_partials<1, NT>(pw, nbw, g.nbmax, a1);
+    const double pwv = a1[0], rro = ctrl->rrs[(t - 1) & 1];
     const double a = rro / pwv;
     const bool ok = pwv > 0.0 && isfinite(a);
     if (block_id() == 0 && threadIdx.x == 0) {
@@ -806,21 +834,22 @@ template <typename T, int V>
 __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *__restrict__ pin, T *__restrict__ pout,
                                                const T *__restrict__ r, const T *__restrict__ kx,
                                                const T *__restrict__ ky, T *__restrict__ w, Ctrl *__restrict__ ctrl,
-                                               double *__restrict__ pw, const double *__restrict__ prprev,
-                                               unsigned nbp, const double *__restrict__ prcur, unsigned nbc,
+                                               double *__restrict__ pw, const double *__restrict__ prcur, unsigned nbc,
                                                double *__restrict__ hist_b)
 {
     if (t > ctrl->stop_iter) return;
     T beta = (T)0;
     const bool fuse = t >= 2;
     if (fuse) {
-        double a1[1], a2[1];
+        // rr(t-1) from the partials of UR(t-1); rr(t-2) as this kernel's
+        // previous launch summed it
+        double a1[1];
         sum_partials<1, TBV>(prcur, nbc, g.nbmax, a1);
-        sum_partials<1, TBV>(prprev, nbp, g.nbmax, a2);
-        const double rrn = a1[0], rro = a2[0];
+        const double rrn = a1[0], rro = ctrl->rrs[(t - 2) & 1];
         const double b = rrn / rro;
         const int why = decide_why(ctrl, t - 1, rrn);
         if (block_id() == 0 && threadIdx.x == 0) {
+            ctrl->rrs[(t - 1) & 1] = rrn;
             ctrl->beta = b;
             ctrl->rro = rrn;
             ctrl->err = rrn;
@@ -917,12 +946,11 @@ template <typename T, int V>
 __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restrict__ u, const T *__restrict__ p,
                                                T *__restrict__ r, const T *__restrict__ w, Ctrl *__restrict__ ctrl,
                                                const double *__restrict__ pw, unsigned nbw,
-                                               const double *__restrict__ prprev, unsigned nbp,
                                                double *__restrict__ prcur, double *__restrict__ hist_a)
 {
     if (t > ctrl->stop_iter) return;
     double ad;
-    if (!cg_alpha<TBV>(g, t, ctrl, pw, nbw, prprev, nbp, hist_a, &ad)) return;
+    if (!cg_alpha_slot<TBV>(g, t, ctrl, pw, nbw, hist_a, &ad)) return;
     const T alpha = (T)ad;
     const VCol<V> cl = vcolumn<V>(g, ty);
     double acc[1] = {0.0};
@@ -980,7 +1008,6 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                                             float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
                                             const float *__restrict__ gc, float hx, float hy,
                                             Ctrl *__restrict__ ctrl, const double *__restrict__ pw_in, unsigned nbw,
-                                            const double *__restrict__ prprev, unsigned nbp,
                                             const double *__restrict__ prcur, unsigned nbc,
                                             double *__restrict__ part_out)
 {
@@ -989,16 +1016,18 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
     const bool fuse = !UR && t >= 2;
     if (UR) {
         double ad;
-        if (!cg_alpha<TBV>(g, t, ctrl, pw_in, nbw, prprev, nbp, nullptr, &ad)) return;
+        if (!cg_alpha_slot<TBV>(g, t, ctrl, pw_in, nbw, nullptr, &ad)) return;
         alpha = (float)ad;
     } else if (fuse) {
-        double a1[1], a2[1];
+        // rr(t-1) from the partials of UR(t-1) (or of the FP64 refresh);
+        // rr(t-2) as this kernel's previous launch summed it
+        double a1[1];
         sum_partials<1, TBV>(prcur, nbc, g.nbmax, a1);
-        sum_partials<1, TBV>(prprev, nbp, g.nbmax, a2);
-        const double rrn = a1[0], rro = a2[0];
+        const double rrn = a1[0], rro = ctrl->rrs[(t - 2) & 1];
         const double b = rrn / rro;
         const int why = decide_why(ctrl, t - 1, rrn);
         if (block_id() == 0 && threadIdx.x == 0) {
+            ctrl->rrs[(t - 1) & 1] = rrn;
             ctrl->beta = b;
             ctrl->rro = rrn;
             ctrl->err = rrn;
@@ -1690,8 +1719,8 @@ hipError_t launch_cg_pw64(tl_ctx *c, int t)
     const bool prof = prof_start(c, TL_K_CG_W64);
     k_cg_pwv<double, 2><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, t, t >= 2 ? p64(c, t - 1) : nullptr, p64(c, t), c->f[TL_F_R], c->f[TL_F_KX], c->f[TL_F_KY],
-        c->f[TL_F_W], c->ctrl, part_w(c), t >= 2 ? part_r(c, t - 2) : nullptr, t >= 2 ? nbr(c, t - 2) : 0,
-        t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, c->hist_b);
+        c->f[TL_F_W], c->ctrl, part_w(c), t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0,
+        c->hist_b);
     c->nb_w = nb2(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
@@ -1702,7 +1731,7 @@ hipError_t launch_cg_urv64(tl_ctx *c, int t)
     const bool prof = prof_start(c, TL_K_CG_UR64);
     k_cg_urv<double, 2><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, t, c->f[TL_F_U], p64(c, t), c->f[TL_F_R], c->f[TL_F_W], c->ctrl, part_w(c), c->nb_w,
-        part_r(c, t - 1), nbr(c, t - 1), part_r(c, t), c->hist_a);
+        part_r(c, t), c->hist_a);
     nbr(c, t) = nb2(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
@@ -1722,8 +1751,7 @@ hipError_t launch_cg_w32(tl_ctx *c, int t)
     const bool prof = prof_start(c, TL_K_CG_W32);
     k_cgg<4, false><<<nb4p(c), TBV, 0, c->stream>>>(
         c->g, c->g.typ4, t, t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gc32, c->hx32, c->hy32,
-        c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 2) : nullptr, t >= 2 ? nbr(c, t - 2) : 0,
-        t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c));
+        c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c));
     c->nb_w = nb4p(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
@@ -1734,7 +1762,7 @@ hipError_t launch_cg_ur32(tl_ctx *c, int t)
     const bool prof = prof_start(c, TL_K_CG_UR32);
     k_cgg<4, true><<<nb4(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty4, t, nullptr, p32(c, t), c->r32, c->x32, c->gc32, c->hx32, c->hy32, c->ctrl, part_w(c),
-        c->nb_w, part_r(c, t - 1), nbr(c, t - 1), nullptr, 0, part_r(c, t));
+        c->nb_w, nullptr, 0, part_r(c, t));
     nbr(c, t) = nb4(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
@@ -1762,7 +1790,7 @@ hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next)
                                                   c->ctrl, part_r(c, t_next - 1));
     nbr(c, t_next - 1) = nbh(c);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_r(c, t_next - 1), nbh(c), 1, 0, FK_IR, 0) : e;
+    return e == hipSuccess ? fin(c, part_r(c, t_next - 1), nbh(c), 1, 0, FK_IR, t_next - 1) : e;
 }
 
 hipError_t launch_apply(tl_ctx *c, const double *src, double *dst)

@@ -43,7 +43,7 @@ extern "C" {
 
 /* stop rules */
 #define TL_STOP_REFERENCE 0 /* each solver's own rule: CG sqrt|rr| < eps (CG.jl:24); others |err| < eps */
-#define TL_STOP_RELATIVE 1  /* CG: ||u0 - A u||_2 <= rel_tol * ||u0||_2 (true FP64 residual at exit) */
+#define TL_STOP_RELATIVE 1  /* CG: ||u0 - A u||_2 <= rel_tol * ||u0||_2, checked on the true FP64 residual */
 
 /* initial guess of the solve */
 #define TL_WARM_NONE 0      /* u = energy*density, the reference's start (CG.jl:52) */
@@ -103,7 +103,9 @@ typedef struct {
     int cheby_iters;      /* Chebyshev / PPCG iterations after the switch */
     int est_iters;        /* Chebyshev iteration estimate (Cheby.calciter) */
     int inner_iters;      /* TL_FP32_IR: FP32 CG iterations */
-    int refreshes;        /* TL_FP32_IR: FP64 residual recomputations */
+    int refreshes;        /* TL_FP32_IR: FP64 residual recomputations; FP64 CG with TL_STOP_RELATIVE:
+                             restarts from the true residual when the recursive one met the rule
+                             but the true one did not */
     int converged;        /* stop rule met (0: ran max_iters) */
     int fallback;         /* warm start rejected in favour of u = energy*density */
     double error;         /* final error as the solver defines it */
@@ -144,7 +146,9 @@ typedef struct {
 
 /* ------------------------------------------------------------ lifecycle */
 void tl_default_params(tl_solve_params *p);
-/* Chunk(settings), chunk.jl:68-89: nx*ny interior cells, domain origin and cell size */
+/* Chunk(settings), chunk.jl:68-89: nx*ny interior cells, domain origin and cell size;
+ * TL_E_INVALID unless nx, ny >= 2 and 2 <= halo_depth <= 16 (with a depth-1 halo the
+ * reference's CG.init! divides by an unset coefficient, CG.jl:59-68) */
 int tl_create(tl_ctx **ctx, int nx, int ny, int halo_depth, double xmin, double ymin, double dx, double dy,
               int device);
 void tl_destroy(tl_ctx *ctx);
@@ -163,7 +167,8 @@ int tl_set_guess(tl_ctx *ctx, const double *host);
 /* ----------------------------------------------------- per-step operations */
 /* setchunkstate! + initialiseapp!, chunk.jl:122-151 and TeaLeaf.jl:35-44, on the device */
 int tl_initialise(tl_ctx *ctx, const tl_state *states, int nstates);
-/* haloupdate!, kernels.jl:146-159 (reflective, depth layers, interior span) */
+/* haloupdate!, kernels.jl:146-159 (reflective, depth layers, interior span);
+ * depth <= min(halo_depth, nx, ny) */
 int tl_halo_update(tl_ctx *ctx, unsigned field_mask, int depth);
 /* set.solver.solve!(chunk, set, rx, ry), TeaLeaf.jl:74 */
 int tl_solve(tl_ctx *ctx, const tl_solve_params *p, double rx, double ry, tl_solve_stats *out);

@@ -94,9 +94,12 @@ extern "C" int tl_create(tl_ctx **out, int nx, int ny, int halo_depth, double xm
                          double dy, int device)
 {
     *out = nullptr;
-    if (nx < 2 || ny < 2 || halo_depth < 1 || halo_depth > 16 || nx < halo_depth || ny < halo_depth ||
-        !(dx > 0) || !(dy > 0)) {
-        g_create_err = "tl_create: invalid grid";
+    // halo_depth >= 2: CG.init! (CG.jl:59-68) builds the face coefficients
+    // from w on halo(chunk, 1), every cell but the outermost ring, and reads
+    // w one cell outside the interior; with a depth-1 halo that cell is the
+    // unset outermost ring and the reference divides by zero
+    if (nx < 2 || ny < 2 || halo_depth < 2 || halo_depth > 16 || !(dx > 0) || !(dy > 0)) {
+        g_create_err = "tl_create: invalid grid (nx, ny >= 2, 2 <= halo_depth <= 16, dx, dy > 0)";
         return TL_E_INVALID;
     }
     int ndev = 0;
@@ -241,7 +244,8 @@ extern "C" int tl_set_guess(tl_ctx *c, const double *host)
 
 extern "C" int tl_halo_update(tl_ctx *c, unsigned mask, int depth)
 {
-    if (!c || depth < 0 || depth > c->g.hd) return TL_E_INVALID;
+    // a reflection of depth d reads d interior cells next to each face
+    if (!c || depth < 0 || depth > c->g.hd || depth > c->g.nx || depth > c->g.ny) return TL_E_INVALID;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, launch_halo(c, mask, depth, nullptr, 0));
     return TL_OK;
@@ -525,14 +529,25 @@ static int solve_cg64(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
     HIPCHK(c, launch_cg_init_resid(c, guess));
     Ctrl k;
     int t = 0;
-    // fused iterations: PW(t) = [end of t-1: beta, stop test, p = r + beta p] + w = A p; UR(t)
-    RETIF(run_until_stop(c, 64, (long)p->max_iters + 16,
-                         [&]() -> hipError_t {
-                             ++t;
-                             hipError_t e = launch_cg_pw64(c, t);
-                             return e == hipSuccess ? launch_cg_urv64(c, t) : e;
-                         },
-                         &k));
+    for (;;) {
+        // fused iterations: PW(t) = [end of t-1: beta, stop test, p = r + beta p] + w = A p; UR(t)
+        RETIF(run_until_stop(c, 64, (long)p->max_iters + 16,
+                             [&]() -> hipError_t {
+                                 ++t;
+                                 hipError_t e = launch_cg_pw64(c, t);
+                                 return e == hipSuccess ? launch_cg_urv64(c, t) : e;
+                             },
+                             &k));
+        if (mode != MODE_CG_REL || k.why != WHY_CONVERGED) break;
+        // the relative rule is met by the recursive residual: confirm it on the
+        // true one, and if rounding has let the two drift apart, continue CG
+        // from the true residual (keeping the search direction)
+        t = k.stop_iter;
+        HIPCHK(c, launch_cg64_true_residual(c, t));
+        RETIF(read_ctrl(c, &k));
+        if (k.stop_iter != INT_MAX) break;
+        ++st->refreshes;
+    }
     const int T = k.stop_iter;
     if (T >= 1) {
         // the last iteration's CG.p!, so p ends as the reference leaves it

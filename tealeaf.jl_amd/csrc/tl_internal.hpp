@@ -178,6 +178,7 @@ hipError_t launch_cg_p64(tl_ctx *c, int t);
 hipError_t launch_cg_w32(tl_ctx *c, int t);
 hipError_t launch_cg_ur32(tl_ctx *c, int t);
 hipError_t launch_residual(tl_ctx *c);
+hipError_t launch_cg64_true_residual(tl_ctx *c, int T);
 hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t);
 hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next);
 hipError_t launch_apply(tl_ctx *c, const double *src, double *dst);

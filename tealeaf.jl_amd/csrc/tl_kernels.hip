@@ -1768,6 +1768,19 @@ hipError_t launch_cg_ur32(tl_ctx *c, int t)
     return hipGetLastError();
 }
 
+// the true FP64 residual r = u0 - A u of the FP64 CG after iteration T
+// replaces the recursive one (r and its partial sums of iteration T); the
+// finalizer then stops on it or lets iteration T+1 run
+hipError_t launch_cg64_true_residual(tl_ctx *c, int T)
+{
+    k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],
+                                                  c->f[TL_F_KY], c->f[TL_F_R], nullptr, nullptr, c->ctrl,
+                                                  part_r(c, T));
+    nbr(c, T) = nbh(c);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? fin(c, part_r(c, T), nbh(c), 1, 0, FK_IR, T) : e;
+}
+
 hipError_t launch_residual(tl_ctx *c)
 {
     k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],

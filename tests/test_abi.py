@@ -102,6 +102,8 @@ def test_invalid_arguments_return_codes():
     ctx = C.c_void_p()
     assert L.tl_create(C.byref(ctx), 0, 10, 2, 0.0, 0.0, 1.0, 1.0, 0) == _lib.TL_E_INVALID
     assert L.tl_create(C.byref(ctx), 10, 10, 0, 0.0, 0.0, 1.0, 1.0, 0) == _lib.TL_E_INVALID
+    assert L.tl_create(C.byref(ctx), 10, 10, 1, 0.0, 0.0, 1.0, 1.0, 0) == _lib.TL_E_INVALID  # depth-1 halo
+    assert L.tl_create(C.byref(ctx), 10, 10, 2, 0.0, 0.0, -1.0, 1.0, 0) == _lib.TL_E_INVALID
 
 
 def test_julia_float_format():

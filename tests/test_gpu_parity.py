@@ -150,8 +150,63 @@ def test_cg_ragged_grid_whole_fields(precision):
     for t in range(2):
         assert _rel(gsum[t], osum[t]) < tol, (t, gsum[t], osum[t])
     fields = ("density", "energy0", "energy", "u") + (("p",) if precision == "fp64" else ())
+    uscale = np.max(np.abs(o.field("u")))
     for f in fields:
         a, b = c.field(f), o.field(f)
         assert a.shape == b.shape
-        scale = np.max(np.abs(b))
+        scale = uscale if f == "p" else np.max(np.abs(b))  # p: on u's scale (see above)
         assert np.max(np.abs(a - b)) <= 1e-8 * scale, (f, np.max(np.abs(a - b)), scale)
+
+
+@pytest.mark.parametrize("nx,ny,hd", [(2, 2, 2), (3, 2, 2), (2, 9, 2), (9, 6, 3), (17, 5, 2), (67, 3, 4)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_small_and_odd_grids(nx, ny, hd, precision):
+    """Grids narrower than one wave's columns, odd shapes and halo depths
+    other than 2: summaries and (FP64) every field, halos included."""
+    print(f"case: CG {precision} {nx}x{ny} halo {hd}, 2 steps", flush=True)
+    extra = "stop_rule=relative\nrel_tol=1e-10\nprecision=fp32_ir\n" if precision == "fp32_ir" else ""
+    text = (decks.reference_deck(nx, 2, extra=extra).replace(f"y_cells={nx}", f"y_cells={ny}")
+            .replace("halo_depth=2", f"halo_depth={hd}"))
+    if precision == "fp32_ir":
+        o, c, s = _pair(text, stop_rule=O.STOP_RELATIVE, rel_tol=1e-12, maxiters=100000)
+    else:
+        o, c, s = _pair(text)
+    assert (c.X, c.Y) == (nx + 2 * hd, ny + 2 * hd)
+    osum, ost = o.run(2)
+    gsum, gst = T.diffuse(c, s, 2)
+    tol = RTOL if precision == "fp64" else 1e-9
+    for t in range(2):
+        assert _rel(gsum[t], osum[t]) < tol, (t, gsum[t], osum[t])
+        if precision == "fp32_ir":
+            assert gst[t]["rel_resid"] <= 1e-10
+    if precision == "fp64":
+        # p is the last search direction, of the size of the final residual
+        # (down to rounding noise on small grids): it is compared on u's scale
+        uscale = np.max(np.abs(o.field("u")))
+        for f in ("density", "energy", "u", "p"):
+            a, b = c.field(f), o.field(f)
+            scale = uscale if f == "p" else np.max(np.abs(b))
+            assert np.max(np.abs(a - b)) <= 1e-9 * scale, (f, np.max(np.abs(a - b)), scale)
+
+
+def test_full_size_8192_one_step_properties():
+    """The benchmark's problem at full size (8192^2, random rectangles seed
+    1001), one timestep with each precision: the FP64 residual meets 1e-10,
+    vol and mass are unchanged, ie = sum vol*u is conserved (1^T A = 1^T, up
+    to the residual's sum), and the two precisions' summaries agree within
+    1e-9."""
+    print("case: 8192^2 random rectangles seed 1001, 1 step FP32-refined and FP64", flush=True)
+    res = {}
+    for precision in ("fp32_ir", "fp64"):
+        s = T.Settings.from_text(decks.random_rect_deck(1001, n=8192, steps=1, precision=precision))
+        c = T.initialiseapp(s)
+        before = T.fieldsummary(c)
+        st = T.step(c, s)
+        after = T.fieldsummary(c)
+        c.close()
+        print(f"  {precision}: {st['iterations']} iterations, rel. residual {st['rel_resid']:.3e}", flush=True)
+        assert st["converged"] == 1 and st["rel_resid"] <= 1e-10, st
+        assert after[0] == before[0] and abs(after[1] / before[1] - 1) < 1e-13
+        assert abs(after[2] / before[2] - 1) < 1e-8, (before, after)
+        res[precision] = after
+    assert _rel(res["fp32_ir"], res["fp64"]) < 1e-9, res

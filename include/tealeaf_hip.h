@@ -221,12 +221,14 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 #define TL_K_CG_W32 3  /* FP32 PW of the inner CG (w = A p only reduced, not stored) */
 #define TL_K_CG_UR32 4 /* FP32 UR of the inner CG (recomputes A p) */
 #define TL_K_CG_P32 5  /* unused: folded into TL_K_CG_W32 */
-/* Bracket every launch of the kernels in kernel_mask (bit = 1 << TL_K_*)
- * with HIP events on the context's stream; tl_profile_read returns one
- * kernel's per-launch durations (ms, launch order; launches made after a
- * solve's stop flag exit at once and are near-empty).  No counterpart in
- * the reference (it has no timers). */
-int tl_profile_begin(tl_ctx *ctx, unsigned kernel_mask);
+/* Bracket every period-th launch of each kernel in kernel_mask (bit =
+ * 1 << TL_K_*; period >= 1) with HIP events on the context's stream (an event
+ * pair costs a few microseconds of GPU time, so a period of about 16 keeps the
+ * measurement out of the timing); tl_profile_read returns one kernel's
+ * sampled per-launch durations (ms, launch order; launches made after a
+ * solve's stop flag exit at once and are near-empty).  No counterpart in the
+ * reference (it has no timers). */
+int tl_profile_begin(tl_ctx *ctx, unsigned kernel_mask, int period);
 int tl_profile_read(tl_ctx *ctx, int kernel_id, double *durations_ms, int max, int *count);
 int tl_profile_end(tl_ctx *ctx);
 /* `reps` back-to-back launches of one kernel on the current fields, for

@@ -805,11 +805,13 @@ static int prof_harvest(tl_ctx *c)
     return TL_OK;
 }
 
-extern "C" int tl_profile_begin(tl_ctx *c, unsigned kernel_mask)
+extern "C" int tl_profile_begin(tl_ctx *c, unsigned kernel_mask, int period)
 {
-    if (!c || (kernel_mask >> (TL_K_CG_P32 + 1))) return TL_E_INVALID;
+    if (!c || (kernel_mask >> (TL_K_CG_P32 + 1)) || period < 1) return TL_E_INVALID;
     RETIF(prof_harvest(c));
     c->prof_mask = kernel_mask;
+    c->prof_period = (unsigned)period;
+    for (auto &n : c->prof_seq) n = 0;
     for (auto &v : c->prof_ms) v.clear();
     return TL_OK;
 }

@@ -142,6 +142,8 @@ struct tl_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};   // batch pipeline
     // per-launch event timing of selected kernels (tl_profile_begin)
     unsigned prof_mask = 0;
+    unsigned prof_period = 1;   // bracket every prof_period-th launch of a kernel
+    unsigned prof_seq[8] = {};  // launches per kernel since tl_profile_begin
     std::vector<hipEvent_t> prof_ev;          // pairs, reused
     std::vector<int> prof_tag;                // kernel id of each recorded pair
     size_t prof_used = 0;                     // events recorded since the last harvest

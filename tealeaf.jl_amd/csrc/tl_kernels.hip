@@ -1597,6 +1597,7 @@ static inline float *p32(tl_ctx *c, int t) { return ((t - 1) & 1) ? c->p32b : c-
 static bool prof_start(tl_ctx *c, int kind)
 {
     if (!(c->prof_mask & (1u << kind))) return false;
+    if (c->prof_seq[kind & 7]++ % c->prof_period != 0) return false;
     if (c->prof_used + 2 > c->prof_ev.size()) {
         hipEvent_t e0, e1;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return false;
@@ -1778,7 +1779,8 @@ hipError_t launch_cg64_true_residual(tl_ctx *c, int T)
                                                   part_r(c, T));
     nbr(c, T) = nbh(c);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? fin(c, part_r(c, T), nbh(c), 1, 0, FK_IR, T) : e;
+    // result in sums[3]: sums[0..1] still hold the initial residual and bb
+    return e == hipSuccess ? fin(c, part_r(c, T), nbh(c), 1, 3, FK_IR, T) : e;
 }
 
 hipError_t launch_residual(tl_ctx *c)

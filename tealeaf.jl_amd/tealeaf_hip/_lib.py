@@ -141,7 +141,7 @@ def load():
     L.tl_unet_train.argtypes = [vp, C.POINTER(TrainParams), fp]
     L.tl_unet_predict.argtypes = [vp, C.c_double, C.c_double, C.c_int]
     L.tl_unet_eval.argtypes = [vp, fp, fp, C.c_int, C.c_int, C.c_int, fp, fp, fp]
-    L.tl_profile_begin.argtypes = [vp, C.c_uint]
+    L.tl_profile_begin.argtypes = [vp, C.c_uint, C.c_int]
     L.tl_profile_read.argtypes = [vp, C.c_int, dp, C.c_int, ip]
     L.tl_profile_end.argtypes = [vp]
     L.tl_kernel_repeat.argtypes = [vp, C.c_int, C.c_int]

@@ -42,6 +42,10 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_VEC_BLOCKS64
 #define TL_VEC_BLOCKS64 8192
 #endif
+// neighbour exchange of the FP32 PW kernel by DPP wave shifts (else LDS shuffles)
+#ifndef TL_DPP
+#define TL_DPP 1
+#endif
 // XCD-aware block-to-tile mapping of the vectorised kernels
 #ifndef TL_XCD_MAP
 #define TL_XCD_MAP 0  // measured at 8192^2: 1.5% less traffic but 5-8% slower kernels

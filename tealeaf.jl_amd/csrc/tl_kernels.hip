@@ -798,14 +798,58 @@ __device__ __forceinline__ VCol<V> vcolumn(const Geo &g, int ty)
     return c;
 }
 
+// v of lane l-1 (up1) / l+1 (down1) in a wave of 64, by LDS shuffles or (DPP)
+// by DPP wave shifts, which move data between VALU lanes without the LDS
+// crossbar (lane 0 / 63 then keep their own value; the callers replace it).
+// Measured at 8192^2: DPP makes the FP32 PW kernel 4% faster and the others
+// no faster, so only that kernel uses it.
+__device__ __forceinline__ int dpp_up1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_down1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+template <bool DPP>
+__device__ __forceinline__ float lane_up1(float v)
+{
+    if (DPP) {
+        return __int_as_float(dpp_up1(__float_as_int(v)));
+    }
+    return __shfl_up(v, 1, 64);
+}
+template <bool DPP>
+__device__ __forceinline__ float lane_down1(float v)
+{
+    if (DPP) {
+        return __int_as_float(dpp_down1(__float_as_int(v)));
+    }
+    return __shfl_down(v, 1, 64);
+}
+template <bool DPP>
+__device__ __forceinline__ double lane_up1(double v)
+{
+    if (DPP) {
+        const long long b = __double_as_longlong(v);
+        const int lo = dpp_up1((int)(b & 0xffffffffLL)), hi = dpp_up1((int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+    return __shfl_up(v, 1, 64);
+}
+template <bool DPP>
+__device__ __forceinline__ double lane_down1(double v)
+{
+    if (DPP) {
+        const long long b = __double_as_longlong(v);
+        const int lo = dpp_down1((int)(b & 0xffffffffLL)), hi = dpp_down1((int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+    return __shfl_down(v, 1, 64);
+}
+
 // values of the columns left and right of the thread's V columns, given the
 // thread's own row values c[] (element index i of column xl)
-template <typename T, int V, class F>
+template <typename T, int V, bool DPP = false, class F>
 __device__ __forceinline__ void vneigh(const Geo &g, const VCol<V> &cl, const T (&c)[V], F &&at_col, T &lv,
                                        T &rv)
 {
-    T l = __shfl_up(c[V - 1], 1, 64);
-    T r = __shfl_down(c[0], 1, 64);
+    T l = lane_up1<DPP>(c[V - 1]);
+    T r = lane_down1<DPP>(c[0]);
     if (cl.lane == 0) l = cl.xl > 0 ? at_col(cl.xl - 1) : c[0];
     if (cl.lane == 63) r = cl.xl + V < g.nx ? at_col(cl.xl + V) : c[V - 1];
     if (cl.xl == 0) l = c[0];
@@ -913,7 +957,7 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
                 auto col_val = [&](int x) -> T { return pnew(ic + (x - cl.xl)); };
                 vneigh<T, V>(g, cl, cur, col_val, lv, rv);
                 // kx of the face right of the last column: next lane's first kx
-                T kxn = __shfl_down(kxv[q][0], 1, 64);
+                T kxn = lane_down1<false>(kxv[q][0]);
                 if (cl.lane == 63) kxn = kx[ic + V];
                 T wv[V];
 #pragma unroll
@@ -1100,8 +1144,8 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 float lv, rvn, glv, grv;
                 auto col_val = [&](int xx) -> float { return pnew(ic + (xx - cl.xl)); };
                 auto col_g = [&](int xx) -> float { return gcol(ic + (xx - cl.xl)); };
-                vneigh<float, V>(g, cl, cur, col_val, lv, rvn);
-                vneigh<float, V>(g, cl, gcur, col_g, glv, grv);
+                vneigh<float, V, TL_DPP && !UR>(g, cl, cur, col_val, lv, rvn);
+                vneigh<float, V, TL_DPP && !UR>(g, cl, gcur, col_g, glv, grv);
                 float wv[V];
 #pragma unroll
                 for (int j = 0; j < V; ++j) {

@@ -272,7 +272,7 @@ def main():
             "iterations": [st["iterations"] for st in flag["stats"]],
             "refreshes": [st["refreshes"] for st in flag["stats"]],
             "fallback": [st["fallback"] for st in flag["stats"]],
-            "start": [["previous", "learned", "smoothed previous"][st["start"]] for st in flag["stats"]],
+            "start": [["previous", "learned"][st["start"]] for st in flag["stats"]],
             "init_rel_resid": [st["init_rel_resid"] for st in flag["stats"]],
             "rel_resid": [st["rel_resid"] for st in flag["stats"]],
         },

@@ -47,8 +47,7 @@ extern "C" {
 
 /* initial guess of the solve */
 #define TL_WARM_NONE 0      /* u = energy*density, the reference's start (CG.jl:52) */
-#define TL_WARM_LEARNED 1   /* U-Net prediction; the start with the smallest residual among it, the
-                               smoothed previous-step start and the previous-step start is used */
+#define TL_WARM_LEARNED 1   /* U-Net prediction; falls back to TL_WARM_NONE when its residual is worse */
 #define TL_WARM_GUESS 2     /* caller-provided guess set with tl_set_guess, same fallback */
 
 /* field ids, in the order of the Chunk struct (chunk.jl:25-38) */
@@ -121,7 +120,6 @@ typedef struct {
 /* tl_solve_stats.start */
 #define TL_START_PREVIOUS 0 /* u = energy*density (the reference's start) */
 #define TL_START_WARM 1     /* the learned prediction or the caller's guess */
-#define TL_START_SMOOTHED 2 /* TL_WARM_LEARNED only: the previous-step start after the same smoothing sweeps */
 
 /* a parsed tea.in deck (settings.jl:90-173); states keep the deck's coordinates */
 #define TL_MAX_STATES 128

@@ -22,7 +22,6 @@ using namespace tl;
 
 namespace tl {
 int unet_predict_guess(tl_ctx *c, double rx, double ry, int coefficient, double *guess_dev);
-hipError_t launch_smooth(tl_ctx *c, const double *x, double *xout, double omega);
 void unet_destroy(tl_ctx *c);
 int unet_record_step(tl_ctx *c, double rx, double ry, int coefficient);
 }  // namespace tl
@@ -155,7 +154,7 @@ extern "C" void tl_destroy(tl_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     unet_destroy(c);
-    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->guess2, c->gc32, c->p32, c->r32, c->x32,
+    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->gc32, c->p32, c->r32, c->x32,
                     c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
     for (int i = 0; i < TL_NFIELDS; ++i)
         if (c->f[i]) (void)hipFree(c->f[i]);
@@ -442,37 +441,19 @@ static int finish_stats(tl_ctx *c, tl_solve_stats *st)
 
 // Warm start: put the candidate guess in c->guess and keep it only if its
 // residual is smaller than that of u = energy*density (the fallback).
-static hipError_t smooth3(tl_ctx *c, const double *x0, double **out)
-{
-    // three damped-Jacobi sweeps on A x = b (b = u), starting from x0
-    hipError_t e = launch_smooth(c, x0, *out, 0.7);
-    for (int s = 1; s < 3 && e == hipSuccess; ++s) {
-        e = launch_smooth(c, *out, c->u_alt, 0.7);
-        std::swap(*out, c->u_alt);
-    }
-    return e;
-}
-
-// The start of a warm-started solve.  TL_WARM_LEARNED: the U-Net prediction
-// and the previous-step start u, both after three damped-Jacobi sweeps (they
-// remove high-frequency error that CG would otherwise spend its first
-// iterations on), and u itself; the one with the smallest residual
-// ||u0 - A x|| is used.  TL_WARM_GUESS: the caller's guess if its residual
-// beats u's.  stats.fallback = 1 when the warm start was not used.
+// The start of a warm-started solve: the U-Net prediction (TL_WARM_LEARNED)
+// or the caller's guess (TL_WARM_GUESS) is used only if its residual
+// ||u0 - A x|| is below that of the previous-step start u; otherwise the
+// solve starts from u and stats.fallback = 1.
 static int choose_guess(tl_ctx *c, const tl_solve_params *p, double rx, double ry, const double **guess,
                         tl_solve_stats *st)
 {
     *guess = nullptr;
     st->start = TL_START_PREVIOUS;
     if (p->warm_start == TL_WARM_NONE) return TL_OK;
-    const bool learned = p->warm_start == TL_WARM_LEARNED;
-    if (learned) {
+    if (p->warm_start == TL_WARM_LEARNED) {
         if (!c->guess) RETIF(alloc_d(c, &c->guess, c->g.elems));
-        if (!c->guess2) RETIF(alloc_d(c, &c->guess2, c->g.elems));
         RETIF(unet_predict_guess(c, rx, ry, p->coefficient, c->guess));
-        HIPCHK(c, launch_copy_full(c, c->guess, c->sd_alt));
-        HIPCHK(c, smooth3(c, c->sd_alt, &c->guess));
-        HIPCHK(c, smooth3(c, c->f[TL_F_U], &c->guess2));
     } else if (p->warm_start == TL_WARM_GUESS) {
         if (!c->guess) {
             c->err = "TL_WARM_GUESS without tl_set_guess";
@@ -483,21 +464,13 @@ static int choose_guess(tl_ctx *c, const tl_solve_params *p, double rx, double r
     }
     HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U], c->guess, 0));
     HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U], c->f[TL_F_U], 1));
-    if (learned) HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U], c->guess2, 2));
     Ctrl k;
     RETIF(read_ctrl(c, &k));
-    const double warm = std::isfinite(k.sums[0]) ? k.sums[0] : INFINITY;
-    const double smoothed = learned && std::isfinite(k.sums[2]) ? k.sums[2] : INFINITY;
-    const double prev = k.sums[1];
-    if (warm < prev && warm < smoothed) {
+    if (std::isfinite(k.sums[0]) && k.sums[0] < k.sums[1]) {
         *guess = c->guess;
         st->start = TL_START_WARM;
-        return TL_OK;
-    }
-    st->fallback = 1;
-    if (smoothed < prev) {
-        *guess = c->guess2;
-        st->start = TL_START_SMOOTHED;
+    } else {
+        st->fallback = 1;
     }
     return TL_OK;
 }

@@ -123,7 +123,6 @@ struct tl_ctx {
     // FP64 fields (reference Chunk fields; u and sd have ping-pong twins)
     double *f[TL_NFIELDS] = {};
     double *u_alt = nullptr, *sd_alt = nullptr, *guess = nullptr;
-    double *guess2 = nullptr;  // smoothed previous-step start (learned warm start)
     // FP32 fields of the mixed-precision solve
     // FP32 inner CG: cell coefficient gc = 1/w and its face scale (rx/2, ry/2),
     // search direction, residual, correction

@@ -1560,30 +1560,6 @@ __global__ __launch_bounds__(TB) void k_jacobi(Geo g, int t, const double *__res
     write_partials<1>(acc, part, g.nbmax);
 }
 
-// damped Jacobi sweep on A x = b with b = u: x' = x + omega (b - A x) / diag(A)
-__global__ __launch_bounds__(TB) void k_smooth(Geo g, const double *__restrict__ xin, double *__restrict__ xout,
-                                               const double *__restrict__ b, const double *__restrict__ kx,
-                                               const double *__restrict__ ky, double omega)
-{
-    const Col c = column(g);
-    if (!c.active) return;
-    Win<double> win;
-    win.start(g, c, xin, ky);
-    for (int yb = c.y0; yb < c.y1; yb += R) {
-        Stn<double> st;
-        double bb[R];
-        load_stn(st, g, c, yb, xin, kx, ky);
-        load_rows(bb, g, c, yb, b);
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const double d = 1.0 + st.kxl[q] + st.kxr[q] + win.kyd + st.kyu[q];
-            const double v = win.cur + omega * (bb[q] - win.apply(st, q)) / d;
-            if (yb + q < c.y1) xout[at(g, c.x, yb + q)] = v;
-            win.advance(st, q);
-        }
-    }
-}
-
 // ------------------------------------------------------ mixed precision
 
 // FP32 cell coefficient of the inner CG over the whole allocation:
@@ -1948,12 +1924,6 @@ hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient)
     k_gc32<<<nb, TB, 0, c->stream>>>(c->f[TL_F_DENSITY], c->gc32, n, coefficient);
     c->hx32 = (float)(0.5 * rx);
     c->hy32 = (float)(0.5 * ry);
-    return hipGetLastError();
-}
-
-hipError_t launch_smooth(tl_ctx *c, const double *x, double *xout, double omega)
-{
-    k_smooth<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, x, xout, c->f[TL_F_U], c->f[TL_F_KX], c->f[TL_F_KY], omega);
     return hipGetLastError();
 }
 

@@ -14,9 +14,8 @@ what it gains.
 3. Evaluation on unseen seeds at --eval-res: the same timesteps solved from
    the previous-step guess and from the learned guess (with its fallback),
    for FP64 CG and for FP32-IR CG; iterations, initial residuals, the start
-   chosen (0 previous-step, 1 learned, 2 smoothed previous-step) and solve
-   times per step are printed as JSON lines.  The "smooth_only" arm runs the
-   learned path with an all-zero network (the smoothing sweeps alone).
+   chosen (0 previous-step, 1 learned) and solve times per step are printed
+   as JSON lines.
 """
 from __future__ import annotations
 
@@ -78,15 +77,13 @@ def evaluate(w, a):
     for seed in a.eval_seeds:
         for prec in ("fp64", "fp32_ir"):
             res = {}
-            # "smooth_only": the learned path with an all-zero network, i.e. the
-            # previous-step guess after the same smoothing sweeps (ablation)
-            for arm in ("none", "learned", "smooth_only"):
-                warm = "none" if arm == "none" else "learned"
+            for arm in ("none", "learned"):
+                warm = arm
                 s = Settings.from_text(decks.random_rect_deck(seed, n=a.eval_res, steps=a.eval_steps, precision=prec,
                                                               warm_start=warm))
                 c = initialiseapp(s)
-                if arm != "none":
-                    unet.set_weights(c, w if arm == "learned" else np.zeros_like(w))
+                if arm == "learned":
+                    unet.set_weights(c, w)
                 its, secs, init_r, fb, summ, starts = [], [], [], [], [], []
                 for t in range(a.eval_steps):
                     t0 = time.time()
@@ -108,10 +105,7 @@ def evaluate(w, a):
                        init_rel_resid_prev_guess=res["none"]["init_rel_resid"],
                        init_rel_resid_learned=res["learned"]["init_rel_resid"],
                        fallback=res["learned"]["fallback"], start_learned=res["learned"]["start"],
-                       summary_max_rel_diff=float(d),
-                       iterations_smooth_only=res["smooth_only"]["iterations"],
-                       init_rel_resid_smooth_only=res["smooth_only"]["init_rel_resid"],
-                       fallback_smooth_only=res["smooth_only"]["fallback"])
+                       summary_max_rel_diff=float(d))
             log(**row)
             rows.append(row)
     return rows

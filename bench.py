@@ -212,6 +212,33 @@ def solvers_run(n, steps):
         print(json.dumps({"solver": solver, "max_rel_summary_diff_vs_cg": d}), flush=True)
 
 
+def compare_run(n, seed, steps):
+    """The benchmark problem for `steps` timesteps with the FP32-refined CG
+    and with FP64 CG side by side: one JSON line per step with both field
+    summaries, their largest relative difference, iterations, true residuals
+    and times."""
+    runs = {}
+    for prec in ("fp32_ir", "fp64"):
+        s = make_settings(n, seed, prec, "none")
+        runs[prec] = (s, T.initialiseapp(s))
+    worst = 0.0
+    for k in range(steps):
+        row = {"step": k + 1}
+        summ = {}
+        for prec, (s, chunk) in runs.items():
+            t0 = time.perf_counter()
+            st = T.step(chunk, s)
+            summ[prec] = T.fieldsummary(chunk)
+            row[prec] = {"s": round(time.perf_counter() - t0, 3), "iterations": st["iterations"],
+                         "refreshes": st["refreshes"], "rel_resid": st["rel_resid"]}
+        d = float(np.max(np.abs(summ["fp32_ir"] - summ["fp64"]) / np.abs(summ["fp64"])))
+        worst = max(worst, d)
+        row["summary_fp64"] = [float(x) for x in summ["fp64"]]
+        row["max_rel_diff"] = d
+        print(json.dumps(row), flush=True)
+    print(json.dumps({"steps": steps, "max_rel_diff_all_steps": worst}), flush=True)
+
+
 def kernel_repeat(n, seed, kid, reps):
     """Back-to-back launches of one kernel on the 8192^2 problem (for
     counter runs); kid -1 is a same-size copy kernel (16 B/cell)."""
@@ -233,6 +260,8 @@ def main():
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-repeat", type=int, nargs=2, metavar=("KERNEL_ID", "REPS"))
+    ap.add_argument("--compare-steps", type=int, default=0,
+                    help="FP32-refined vs FP64 CG side by side for this many timesteps (per-step summaries)")
     ap.add_argument("--solvers", action="store_true",
                     help="CG, Chebyshev and PPCG on the 3-state reference deck at --n (default 4096 here)")
     a = ap.parse_args()
@@ -246,6 +275,9 @@ def main():
         return
     if a.solvers:
         solvers_run(a.n if a.n != 8192 else 4096, a.steps)
+        return
+    if a.compare_steps:
+        compare_run(a.n, a.seed, a.compare_steps)
         return
     learned = os.path.exists(WEIGHTS)
     warm = "learned" if learned else "none"

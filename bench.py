@@ -37,18 +37,21 @@ import tealeaf_hip as T  # noqa: E402
 from tealeaf_hip import _lib, decks  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
-PROFILE_PERIOD = 16
+PROFILE_PERIOD = 15  # odd: both parities of the alternating FP32 UR kernel are sampled
 # algorithmic bytes per interior cell per launch (arrays read once + written once).
 # A CG iteration is two kernels.
 #   FP64:  PW = [p = r + beta p; w = A p] reads p, r, Kx, Ky, writes p', w  (48 B)
 #          UR = [u += alpha p; r -= alpha w] reads u, p, r, w, writes u, r   (48 B)
 #   FP32 inner CG (operator from one cell-coefficient array gc, w not stored):
 #          PW reads p, r, gc, writes p'                                     (16 B)
-#          UR = [x += alpha p; r -= alpha A p] reads x, p, r, gc, writes x, r (24 B)
+#          UR = [r -= alpha A p] reads p, r, gc, writes r                   (16 B)
+#          and at even iterations also x += alpha(t-1) p(t-1) + alpha(t) p(t):
+#          reads x, p(t-1), writes x                                     (+12 B)
+#          i.e. 22 B per launch on average
 # P64 is only the final p update of an FP64 solve.
 KERNEL_BYTES = {
     _lib.TL_K_CG_W64: 48, _lib.TL_K_CG_UR64: 48, _lib.TL_K_CG_P64: 24,
-    _lib.TL_K_CG_W32: 16, _lib.TL_K_CG_UR32: 24, _lib.TL_K_CG_P32: 0,
+    _lib.TL_K_CG_W32: 16, _lib.TL_K_CG_UR32: 22, _lib.TL_K_CG_P32: 0,
 }
 WEIGHTS = os.path.join(ROOT, "tealeaf.jl_amd", "weights", "unet_warmstart.npy")
 # HBM bytes per launch of the CG kernels at 8192^2 from rocprofv3 FETCH_SIZE /
@@ -240,8 +243,9 @@ def compare_run(n, seed, steps):
 
 
 def kernel_repeat(n, seed, kid, reps):
-    """Back-to-back launches of one kernel on the 8192^2 problem (for
-    counter runs); kid -1 is a same-size copy kernel (16 B/cell)."""
+    """For counter runs on the 8192^2 problem: `reps` iterations of the CG
+    that kernel kid belongs to (tl_kernel_repeat), or for kid -1 / -2 that
+    many launches of a same-size copy kernel (16 B/cell)."""
     s = make_settings(n, seed, "fp64", "none")
     chunk = T.initialiseapp(s)
     _lib.check(_lib.load().tl_kernel_repeat(chunk._ctx, kid, reps), chunk._ctx, "kernel_repeat")

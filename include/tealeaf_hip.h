@@ -229,10 +229,13 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 int tl_profile_begin(tl_ctx *ctx, unsigned kernel_mask, int period);
 int tl_profile_read(tl_ctx *ctx, int kernel_id, double *durations_ms, int max, int *count);
 int tl_profile_end(tl_ctx *ctx);
-/* `reps` back-to-back launches of one kernel on the current fields, for
- * counter collection and timing; kernel_id -1 is a copy of one field's
- * (nx+2h)*(ny+2h) cells (8 B read + 8 B written per cell, one cell per lane),
- * -2 a 16-byte-per-lane copy of one field's whole allocation */
+/* For counter collection and timing on the current fields: kernel_id of a
+ * CG kernel runs `reps` iterations of that CG (FP64 or FP32 inner; both of
+ * its kernels per iteration, after three warm-up iterations, with a stop test
+ * that never fires), TL_K_CG_P64 `reps` launches of that kernel; -1 runs a
+ * copy of one field's (nx+2h)*(ny+2h) cells (8 B read + 8 B written per cell,
+ * one cell per lane), -2 a 16-byte-per-lane copy of one field's whole
+ * allocation */
 int tl_kernel_repeat(tl_ctx *ctx, int kernel_id, int reps);
 
 /* --------------------------------------------------- learned warm start */

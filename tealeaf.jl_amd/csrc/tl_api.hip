@@ -746,7 +746,7 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     while (k.stop_iter == INT_MAX || k.why == WHY_REFRESH) {
         if (k.stop_iter != INT_MAX && k.why == WHY_REFRESH) {
             t = k.stop_iter;
-            HIPCHK(c, launch_ir_accumulate(c));
+            HIPCHK(c, launch_ir_accumulate(c, t));
             HIPCHK(c, launch_ir_residual(c, 0, t + 1));
             ++refreshes;
             RETIF(read_ctrl(c, &k));
@@ -754,7 +754,7 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
         }
         RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return cg32_iteration(c, ++t); }, &k));
     }
-    if (k.why == WHY_MAXITERS) HIPCHK(c, launch_ir_accumulate(c));  // keep the partial correction
+    if (k.why == WHY_MAXITERS) HIPCHK(c, launch_ir_accumulate(c, k.stop_iter));  // keep the partial correction
     st->inner_iters = k.iters;
     st->iterations = k.iters;
     st->cg_iters = k.iters;
@@ -830,29 +830,28 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
         HIPCHK(c, launch_ir_prepare(c, 1.0, 1.0, 1));
         HIPCHK(c, launch_ir_residual(c, 1, 1));
     }
-    // two live iterations so every partial-sum array exists, then repeats of
-    // one kernel of iteration 3 (its inputs stay valid: it reads only
-    // buffers that kernel does not write)
-    if (kernel_id >= TL_K_CG_W32) {
-        HIPCHK(c, cg32_iteration(c, 1));
-        HIPCHK(c, cg32_iteration(c, 2));
-    } else {
-        HIPCHK(c, launch_cg_pw64(c, 1));
-        HIPCHK(c, launch_cg_urv64(c, 1));
-        HIPCHK(c, launch_cg_pw64(c, 2));
-        HIPCHK(c, launch_cg_urv64(c, 2));
+    // three live iterations so every partial-sum array and ping-pong buffer
+    // is filled, then `reps` further iterations of the same CG (both kernels
+    // of each; consecutive iterations alternate the FP32 UR kernel's even and
+    // odd forms), or `reps` launches of the FP64 P kernel of iteration 4
+    for (int t = 1; t <= 3; ++t) {
+        if (kernel_id >= TL_K_CG_W32) {
+            HIPCHK(c, cg32_iteration(c, t));
+        } else {
+            HIPCHK(c, launch_cg_pw64(c, t));
+            HIPCHK(c, launch_cg_urv64(c, t));
+        }
     }
     for (int i = 0; i < reps; ++i) {
-        hipError_t e = hipSuccess;
-        switch (kernel_id) {
-        case TL_K_CG_W64: e = launch_cg_pw64(c, 3); break;
-        case TL_K_CG_UR64: e = launch_cg_urv64(c, 3); break;
-        case TL_K_CG_P64: e = launch_cg_p64(c, 3); break;
-        case TL_K_CG_W32: e = launch_cg_w32(c, 3); break;
-        case TL_K_CG_UR32: e = launch_cg_ur32(c, 3); break;
-        default: break;
+        const int t = 4 + i;
+        if (kernel_id >= TL_K_CG_W32) {
+            HIPCHK(c, cg32_iteration(c, t));
+        } else if (kernel_id == TL_K_CG_P64) {
+            HIPCHK(c, launch_cg_p64(c, 4));
+        } else {
+            HIPCHK(c, launch_cg_pw64(c, t));
+            HIPCHK(c, launch_cg_urv64(c, t));
         }
-        HIPCHK(c, e);
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return TL_OK;

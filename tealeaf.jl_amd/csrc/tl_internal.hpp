@@ -102,6 +102,8 @@ struct Ctrl {
     double sums[4];     // results of single-block finalizers (field summary, norms)
     double rrs[2];      // vectorised CG: rr(t) in rrs[t & 1], written once per iteration by
                         // block 0 of the kernel that sums it (or by the init/refresh finalizer)
+    double alphas[2];   // alpha(t) in alphas[t & 1] (the FP32 inner CG applies two steps'
+                        // x updates at once, at even t)
     int iters;          // last completed iteration
     int maxiters;
     int mode;
@@ -197,7 +199,7 @@ hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t);
 hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t);
 hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer);
 hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient);
-hipError_t launch_ir_accumulate(tl_ctx *c);
+hipError_t launch_ir_accumulate(tl_ctx *c, int T);
 
 Geo make_geo(int nx, int ny, int hd);
 

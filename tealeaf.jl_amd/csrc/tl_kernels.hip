@@ -299,6 +299,7 @@ __global__ __launch_bounds__(TB) void k_init_ctrl(Ctrl *c, int mode, int maxiter
 {
     if (threadIdx.x != 0) return;
     c->rro = c->pw = c->alpha = c->beta = 0.0;
+    c->rrs[0] = c->rrs[1] = c->alphas[0] = c->alphas[1] = 0.0;
     c->err = 1e10; /* ERROR_START, kernels.jl:7 */
     c->bb = 0.0;
     c->eps = eps;
@@ -383,6 +384,8 @@ __global__ __launch_bounds__(TB) void k_fin(Ctrl *__restrict__ c, const double *
         const double rr = v[0];
         c->rro = rr;
         c->rrs[t & 1] = rr;
+        // the x update of iteration t is in u now: iteration t+1 must not apply it again
+        c->alphas[t & 1] = 0.0;
         c->err = rr;
         c->refresh_abs = c->ir_delta * sqrt(rr);
         if (!isfinite(rr)) {
@@ -608,6 +611,7 @@ __device__ __forceinline__ bool cg_alpha_slot(const Geo &g, int t, Ctrl *__restr
     if (block_id() == 0 && threadIdx.x == 0) {
         ctrl->pw = pwv;
         ctrl->alpha = a;
+        ctrl->alphas[t & 1] = a;
         if (hist_a) hist_a[t - 1] = a;
         if (!ok) {  // non-positive curvature: stop before this iteration's update
             ctrl->status = TL_E_BREAKDOWN;
@@ -1045,23 +1049,30 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
 //   beta = rr(t-1)/rr(t-2), the stop test (CG.jl:39-42, 24) -- then
 //   p = r + beta p (CG.p!, written to pout; p is ping-ponged because every
 //   thread reads its neighbours' old p); partial sums of p.(A p) (CG.w!).
-// UR = true (iteration t): alpha = rr(t-1)/pw(t), then x += alpha p,
-//   r -= alpha A p and partial sums of r^2 (CG.ur!, CG.jl:93-98).
+// UR = true (iteration t): alpha = rr(t-1)/pw(t), then r -= alpha A p and
+//   partial sums of r^2 (CG.ur!, CG.jl:93-98); the correction x += alpha p
+//   is applied for two iterations at once, at even t, which saves x's read
+//   and write at odd t: UR moves 28 bytes per cell at even t, 16 at odd t.
 template <int V, bool UR>
 __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, const float *__restrict__ pin,
                                             float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
                                             const float *__restrict__ gc, float hx, float hy,
                                             Ctrl *__restrict__ ctrl, const double *__restrict__ pw_in, unsigned nbw,
                                             const double *__restrict__ prcur, unsigned nbc,
-                                            double *__restrict__ part_out)
+                                            double *__restrict__ part_out, const float *__restrict__ pprev)
 {
     if (t > ctrl->stop_iter) return;
     float beta = 0.0f, alpha = 0.0f;
     const bool fuse = !UR && t >= 2;
+    // UR at even t also applies the x update of iteration t-1, whose p is
+    // still in the other ping-pong buffer: x += alpha(t-1) p(t-1) + alpha(t) p(t)
+    const bool xupd = UR && (t & 1) == 0;
+    float alpha_prev = 0.0f;
     if (UR) {
         double ad;
         if (!cg_alpha_slot<TBV>(g, t, ctrl, pw_in, nbw, nullptr, &ad)) return;
         alpha = (float)ad;
+        if (xupd) alpha_prev = (float)ctrl->alphas[(t - 1) & 1];
     } else if (fuse) {
         // rr(t-1) from the partials of UR(t-1) (or of the FP64 refresh);
         // rr(t-2) as this kernel's previous launch summed it
@@ -1110,7 +1121,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
             }
         }
         for (int yb = cl.y0; yb < cl.y1; yb += RV) {
-            float up[RV][V], rup[RV][V], gup[RV][V], xv[RV][V], rv[RV][V];
+            float up[RV][V], rup[RV][V], gup[RV][V], xv[RV][V], rv[RV][V], pv[RV][V];
 #pragma unroll
             for (int q = 0; q < RV; ++q) {
                 const int y = yb + q;
@@ -1120,8 +1131,11 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 vload<float, V>(gup[q], gc + iu);
                 if (UR) {
                     const long ic = atc(g, cl.xl, y);
-                    vload<float, V>(xv[q], x + ic);
                     vload<float, V>(rv[q], r + ic);
+                    if (xupd) {
+                        vload<float, V>(xv[q], x + ic);
+                        vload<float, V>(pv[q], pprev + ic);
+                    }
                 }
             }
 #pragma unroll
@@ -1162,11 +1176,12 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                         float xn[V], rn[V];
 #pragma unroll
                         for (int j = 0; j < V; ++j) {
-                            xn[j] = xv[q][j] + alpha * cur[j];
+                            // the two steps' updates in order, each rounded as if stored
+                            xn[j] = xupd ? (xv[q][j] + alpha_prev * pv[q][j]) + alpha * cur[j] : 0.0f;
                             rn[j] = rv[q][j] - alpha * wv[j];
                             if (j < nv) acc[0] += (double)rn[j] * (double)rn[j];
                         }
-                        vstore_n<float, V>(x + ic, xn, nv);
+                        if (xupd) vstore_n<float, V>(x + ic, xn, nv);
                         vstore_n<float, V>(r + ic, rn, nv);
                     } else {
                         if (fuse) vstore_n<float, V>(pout + ic, cur, nv);
@@ -1574,21 +1589,30 @@ __global__ __launch_bounds__(TB) void k_gc32(const double *__restrict__ rho, flo
     gc[i] = coefficient == 2 ? (float)d : (d != 0.0 ? (float)(1.0 / d) : 0.0f);
 }
 
-// FP64 solution accumulation of the mixed-precision CG: u += x; x = 0
-__global__ __launch_bounds__(TB) void k_ir_accumulate(Geo g, double *__restrict__ u, float *__restrict__ x)
+// FP64 solution accumulation of the mixed-precision CG after iteration T:
+// u += x; x = 0.  x holds the updates up to the last even iteration; for odd
+// T the update of iteration T, alpha(T) p(T), is added first, as the FP32 UR
+// kernel would have.
+__global__ __launch_bounds__(TB) void k_ir_accumulate(Geo g, double *__restrict__ u, float *__restrict__ x,
+                                                      const float *__restrict__ pT, const Ctrl *__restrict__ ctrl,
+                                                      int T)
 {
     const Col c = column(g);
     if (!c.active) return;
+    const bool pend = (T & 1) && pT;
+    const float aT = pend ? (float)ctrl->alphas[T & 1] : 0.0f;
     for (int yb = c.y0; yb < c.y1; yb += R) {
         double uv[R];
-        float xv[R];
+        float xv[R], pv[R];
         load_rows(uv, g, c, yb, u);
         load_rows(xv, g, c, yb, x);
+        if (pend) load_rows(pv, g, c, yb, pT);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             if (yb + q < c.y1) {
                 const long i = at(g, c.x, yb + q);
-                u[i] = uv[q] + (double)xv[q];
+                const float xf = pend ? xv[q] + aT * pv[q] : xv[q];
+                u[i] = uv[q] + (double)xf;
                 x[i] = 0.0f;
             }
         }
@@ -1772,7 +1796,7 @@ hipError_t launch_cg_w32(tl_ctx *c, int t)
     const bool prof = prof_start(c, TL_K_CG_W32);
     k_cgg<4, false><<<nb4p(c), TBV, 0, c->stream>>>(
         c->g, c->g.typ4, t, t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gc32, c->hx32, c->hy32,
-        c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c));
+        c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c), nullptr);
     c->nb_w = nb4p(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
@@ -1783,7 +1807,7 @@ hipError_t launch_cg_ur32(tl_ctx *c, int t)
     const bool prof = prof_start(c, TL_K_CG_UR32);
     k_cgg<4, true><<<nb4(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty4, t, nullptr, p32(c, t), c->r32, c->x32, c->gc32, c->hx32, c->hy32, c->ctrl, part_w(c),
-        c->nb_w, nullptr, 0, part_r(c, t));
+        c->nb_w, nullptr, 0, part_r(c, t), t >= 2 ? p32(c, t - 1) : nullptr);
     nbr(c, t) = nb4(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
@@ -1927,9 +1951,10 @@ hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient)
     return hipGetLastError();
 }
 
-hipError_t launch_ir_accumulate(tl_ctx *c)
+hipError_t launch_ir_accumulate(tl_ctx *c, int T)
 {
-    k_ir_accumulate<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U], c->x32);
+    k_ir_accumulate<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U], c->x32, T >= 1 ? p32(c, T) : nullptr,
+                                                       c->ctrl, T);
     return hipGetLastError();
 }
 

@@ -76,8 +76,9 @@ def make_settings(n, seed, precision, warm):
     return T.Settings.from_text(text)
 
 
-def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0):
+def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, launch="stream"):
     s = make_settings(n, seed, precision, warm)
+    s.set(launch={"stream": T.TL_LAUNCH_STREAM, "graph": T.TL_LAUNCH_GRAPH}[launch])
     chunk = T.initialiseapp(s)
     if warm == "learned":
         from tealeaf_hip import unet
@@ -266,6 +267,8 @@ def main():
     ap.add_argument("--kernel-repeat", type=int, nargs=2, metavar=("KERNEL_ID", "REPS"))
     ap.add_argument("--compare-steps", type=int, default=0,
                     help="FP32-refined vs FP64 CG side by side for this many timesteps (per-step summaries)")
+    ap.add_argument("--launch", choices=("stream", "graph"), default="stream",
+                    help="CG iterations as stream launches or hipGraph replays (graph: no per-kernel timing)")
     ap.add_argument("--solvers", action="store_true",
                     help="CG, Chebyshev and PPCG on the 3-state reference deck at --n (default 4096 here)")
     a = ap.parse_args()
@@ -287,10 +290,10 @@ def main():
     warm = "learned" if learned else "none"
     mask = (1 << _lib.TL_K_CG_W32) | (1 << _lib.TL_K_CG_UR32) | (1 << _lib.TL_K_CG_P32)
     flag = run_method("fp32-ir" + ("+learned" if learned else ""), a.n, a.seed, "fp32_ir", warm, a.steps,
-                      a.warmup, mask)
+                      a.warmup, mask if a.launch == "stream" else 0, a.launch)
     base = None
     if not a.no_baseline:
-        base = run_method("fp64-cg", a.n, a.seed, "fp64", "none", a.steps, a.warmup)
+        base = run_method("fp64-cg", a.n, a.seed, "fp64", "none", a.steps, a.warmup, launch=a.launch)
     value = float(np.mean(flag["times"]))
     out = {
         "metric": "s/timestep at 8192², rel. resid 1e-10: learned+FP32-refined vs plain FP64 CG",
@@ -299,7 +302,7 @@ def main():
         "vs_baseline": None, "dtype": "fp32 inner / fp64 residual+solution+reductions", "data": "synthetic",
         "config": {
             "workload": f"TeaLeaf {a.n}x{a.n} random rectangles (32, seed {a.seed}), dt 0.004, rel. resid 1e-10",
-            "grid": a.n, "seed": a.seed, "warm_start": warm, "parallelism": "single GPU",
+            "grid": a.n, "seed": a.seed, "warm_start": warm, "parallelism": "single GPU", "launch": a.launch,
         },
         "flagship": {
             "s_per_step": [round(t, 4) for t in flag["times"]],

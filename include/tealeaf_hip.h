@@ -50,6 +50,15 @@ extern "C" {
 #define TL_WARM_LEARNED 1   /* U-Net prediction; falls back to TL_WARM_NONE when its residual is worse */
 #define TL_WARM_GUESS 2     /* caller-provided guess set with tl_set_guess, same fallback */
 
+/* tl_solve_params.launch: how the CG iterations of TL_CG solves (FP64 and
+ * TL_FP32_IR) reach the GPU.  Both give bit-identical results.  Other solvers
+ * always use stream launches. */
+#define TL_LAUNCH_STREAM 0  /* kernels enqueued in batches of up to 64 iterations */
+#define TL_LAUNCH_GRAPH 1   /* after a first stream batch, replays of one hipGraph of 64 iterations
+                               (captured on first use, kept with the context, re-captured when
+                               a buffer or coefficient it holds changes); tl_profile_* does not
+                               see the replayed kernels */
+
 /* field ids, in the order of the Chunk struct (chunk.jl:25-38) */
 #define TL_F_DENSITY0 0
 #define TL_F_DENSITY 1
@@ -92,7 +101,8 @@ typedef struct {
     int warm_start;       /* TL_WARM_* */
     int check_result;     /* Settings.checkresult: residual! in tl_solve_finished */
     double ir_delta;      /* TL_FP32_IR: refresh the FP64 residual when the FP32 one falls by this factor */
-    int reserved[8];
+    int launch;           /* TL_LAUNCH_* */
+    int reserved[7];
 } tl_solve_params;
 
 /* replaces the reference's @info lines (CG.jl:27, Cheby.jl:60, PPCG.jl:54, Jacobi.jl:29) */

@@ -164,6 +164,10 @@ extern "C" void tl_destroy(tl_ctx *c)
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
     if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+    for (auto &gb : c->graphs) {
+        if (gb.exec) (void)hipGraphExecDestroy(gb.exec);
+        if (gb.graph) (void)hipGraphDestroy(gb.graph);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -265,19 +269,115 @@ extern "C" int tl_initialise(tl_ctx *c, const tl_state *states, int nstates)
     return TL_OK;
 }
 
+// ------------------------------------------------------- hipGraph batches
+
+#define TL_GRAPH_CACHE 4
+enum GraphKind { GRAPH_CG64 = 0, GRAPH_CG32 = 1 };
+
+// what a captured batch of `kind` holds besides the Ctrl-resident scalars:
+// batch length, buffers, FP32 face scales and the partial-sum block counts
+static std::vector<uintptr_t> graph_key(const tl_ctx *c, int kind, int k)
+{
+    uint32_t hx = 0, hy = 0;
+    std::memcpy(&hx, &c->hx32, sizeof hx);
+    std::memcpy(&hy, &c->hy32, sizeof hy);
+    std::vector<uintptr_t> v = {(uintptr_t)kind, (uintptr_t)k, (uintptr_t)c->ctrl, (uintptr_t)c->partials,
+                                c->nb_r[0], c->nb_r[1], c->nb_w};
+    if (kind == GRAPH_CG64) {
+        for (const void *q : {(const void *)c->f[TL_F_P], (const void *)c->p_alt, (const void *)c->f[TL_F_R],
+                              (const void *)c->f[TL_F_KX], (const void *)c->f[TL_F_KY], (const void *)c->f[TL_F_W],
+                              (const void *)c->f[TL_F_U], (const void *)c->hist_a, (const void *)c->hist_b})
+            v.push_back((uintptr_t)q);
+    } else {
+        for (const void *q : {(const void *)c->p32, (const void *)c->p32b, (const void *)c->r32,
+                              (const void *)c->x32, (const void *)c->gc32})
+            v.push_back((uintptr_t)q);
+        v.push_back(hx);
+        v.push_back(hy);
+    }
+    return v;
+}
+
+static void graph_drop(tl_ctx::GraphBatch &b)
+{
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b = tl_ctx::GraphBatch{};
+}
+
+// The executable graph of k CG iterations of `kind`: node 0 advances
+// Ctrl::tg, then PW and UR of each iteration.  Every replay starts at an odd
+// iteration, so the ping-pong buffers and partial-sum slots of node i are the
+// same in every replay.  Called after a stream batch of at least two
+// iterations, so the block counts read at capture are the steady ones.
+static int graph_get(tl_ctx *c, int kind, int k, hipGraphExec_t *exec)
+{
+    const std::vector<uintptr_t> key = graph_key(c, kind, k);
+    for (auto &gb : c->graphs)
+        if (gb.exec && gb.key == key) {
+            *exec = gb.exec;
+            return TL_OK;
+        }
+    if (c->graphs.size() < TL_GRAPH_CACHE) c->graphs.emplace_back();
+    tl_ctx::GraphBatch &b = c->graphs.size() < TL_GRAPH_CACHE || !c->graphs.back().exec
+                                ? c->graphs.back()
+                                : c->graphs[c->graph_next++ % TL_GRAPH_CACHE];
+    graph_drop(b);
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    const int tref = 3;  // an odd iteration >= 2 (t only selects buffers and slots here)
+    hipError_t e = launch_graph_base(c, k);
+    for (int i = 0; i < k && e == hipSuccess; ++i) {
+        const int t = tref + i;
+        e = kind == GRAPH_CG64 ? launch_cg_pw64(c, t, tref) : launch_cg_w32(c, t, tref);
+        if (e == hipSuccess) e = kind == GRAPH_CG64 ? launch_cg_urv64(c, t, tref) : launch_cg_ur32(c, t, tref);
+    }
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    b.graph = g;
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        graph_drop(b);
+        c->err = std::string("hipGraph batch: ") + hipGetErrorString(e);
+        return TL_E_HIP;
+    }
+    b.key = key;
+    *exec = b.exec;
+    return TL_OK;
+}
+
 // ------------------------------------------------------- batch pipeline
 
 // Enqueues `body` (one iteration's kernels) in batches, one batch ahead of
 // the host's check, until the device control block reports a stop.  Returns
 // with the stream synchronised and *out holding the final control block.
+// With a graph kind (TL_LAUNCH_GRAPH) only the first batch is enqueued
+// through `body`; it ends at an even iteration, and every later batch is one
+// replay of a graph of kmax iterations.  *tlast is the last iteration
+// enqueued (the counter `body` advances).
 template <class Body>
-static int run_until_stop(tl_ctx *c, int kmax, long max_batches, Body &&body, Ctrl *out)
+static int run_until_stop(tl_ctx *c, int kmax, long max_batches, Body &&body, Ctrl *out, int gkind = -1,
+                          int *tlast = nullptr)
 {
     hipEvent_t *ev = events(c);
     Ctrl *snap = c->ctrl_h;
+    const bool graph = gkind >= 0 && tlast;
     int k = 4;
+    if (graph && ((*tlast + k) & 1)) ++k;
+    hipGraphExec_t exec = nullptr;
     for (long b = 0;; ++b) {
-        for (int i = 0; i < k; ++i) HIPCHK(c, body());
+        if (graph && b > 0) {
+            if (!exec) {
+                RETIF(graph_get(c, gkind, kmax, &exec));
+                HIPCHK(c, launch_graph_seed(c, *tlast + 1));
+            }
+            HIPCHK(c, hipGraphLaunch(exec, c->stream));
+            *tlast += kmax;
+        } else {
+            for (int i = 0; i < k; ++i) HIPCHK(c, body());
+        }
         HIPCHK(c, hipMemcpyAsync(&snap[b & 1], c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipEventRecord(ev[b & 1], c->stream));
         if (k < kmax) k *= 2;
@@ -510,7 +610,7 @@ static int solve_cg64(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
                                  hipError_t e = launch_cg_pw64(c, t);
                                  return e == hipSuccess ? launch_cg_urv64(c, t) : e;
                              },
-                             &k));
+                             &k, p->launch == TL_LAUNCH_GRAPH ? GRAPH_CG64 : -1, &t));
         if (mode != MODE_CG_REL || k.why != WHY_CONVERGED) break;
         // the relative rule is met by the recursive residual: confirm it on the
         // true one, and if rounding has let the two drift apart, continue CG
@@ -752,7 +852,8 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
             RETIF(read_ctrl(c, &k));
             continue;
         }
-        RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return cg32_iteration(c, ++t); }, &k));
+        RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return cg32_iteration(c, ++t); }, &k,
+                             p->launch == TL_LAUNCH_GRAPH ? GRAPH_CG32 : -1, &t));
     }
     if (k.why == WHY_MAXITERS) HIPCHK(c, launch_ir_accumulate(c, k.stop_iter));  // keep the partial correction
     st->inner_iters = k.iters;
@@ -871,6 +972,10 @@ extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double r
     }
     if (p->precision == TL_FP32_IR && (p->solver != TL_CG || p->stop_rule != TL_STOP_RELATIVE)) {
         c->err = "TL_FP32_IR needs the CG solver and the relative stop rule";
+        return TL_E_INVALID;
+    }
+    if (p->launch != TL_LAUNCH_STREAM && p->launch != TL_LAUNCH_GRAPH) {
+        c->err = "tl_solve: launch must be TL_LAUNCH_STREAM or TL_LAUNCH_GRAPH";
         return TL_E_INVALID;
     }
     if (p->warm_start != TL_WARM_NONE && p->solver != TL_CG) {

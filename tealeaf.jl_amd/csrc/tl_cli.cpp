@@ -1,7 +1,8 @@
 // tl_cli.cpp -- command-line driver over libtealeaf_hip, the counterpart of
 // run.jl (Laura7089/TeaLeaf.jl run.jl:1-47):
 //   tealeaf [-s cg|cheby|ppcg|jacobi] [-x N] [-y N] [-i tea.in] [--steps N]
-//           [--device D] [-p tea.problems]
+//           [--device D] [-p tea.problems] [--graph]
+// --graph runs CG iterations as hipGraph replays (TL_LAUNCH_GRAPH).
 // -x/-y re-derive dx/dy from the domain (run.jl:35-40 leaves them stale).
 // Each timestep prints one JSON line with the solver statistics and the
 // field summary (vol, mass, ie, temp); at the end, if check_result is on, the
@@ -46,7 +47,7 @@ static void on_step(int step, const double s[4], const tl_solve_stats *st, void 
 int main(int argc, char **argv)
 {
     std::string infile = "tea.in", solver, extra, problems = "tea.problems";
-    int device = 0, steps = 0;
+    int device = 0, steps = 0, graph = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -70,9 +71,11 @@ int main(int argc, char **argv)
             device = std::atoi(next().c_str());
         else if (a == "-p" || a == "--problems")
             problems = next();
+        else if (a == "--graph")
+            graph = 1;
         else if (a == "-h" || a == "--help") {
             std::printf("usage: tealeaf [-s cg|cheby|ppcg|jacobi] [-x N] [-y N] [-i tea.in] [--steps N] "
-                        "[--device D] [-p tea.problems]\n");
+                        "[--device D] [-p tea.problems] [--graph]\n");
             return 0;
         } else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
@@ -99,6 +102,7 @@ int main(int argc, char **argv)
         std::fprintf(stderr, "could not parse %s\n", infile.c_str());
         return 1;
     }
+    if (graph) deck.params.launch = TL_LAUNCH_GRAPH;
     tl_ctx *ctx = nullptr;
     int rc = tl_deck_create(&ctx, &deck, device);
     if (rc != TL_OK) {

@@ -104,6 +104,7 @@ struct Ctrl {
                         // block 0 of the kernel that sums it (or by the init/refresh finalizer)
     double alphas[2];   // alpha(t) in alphas[t & 1] (the FP32 inner CG applies two steps'
                         // x updates at once, at even t)
+    int tg, tgn;        // hipGraph batches: first iteration of the running / next replay
     int iters;          // last completed iteration
     int maxiters;
     int mode;
@@ -155,6 +156,16 @@ struct tl_ctx {
     std::vector<double> prof_ms[8];
     tl::Unet *unet = nullptr;
     bool recording = false;
+    // hipGraph batches of CG iterations (TL_LAUNCH_GRAPH), keyed by solve kind,
+    // batch length and every launch argument that can change between solves
+    struct GraphBatch {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<uintptr_t> key;
+    };
+    std::vector<GraphBatch> graphs;  // at most TL_GRAPH_CACHE entries
+    unsigned graph_next = 0;         // entry replaced next when the cache is full
+    bool capturing = false;
 };
 
 namespace tl {
@@ -176,14 +187,19 @@ hipError_t launch_init_coef(tl_ctx *c, double rx, double ry, int coefficient, in
 hipError_t launch_halo(tl_ctx *c, unsigned mask, int depth, double *const *extra, int nextra);
 hipError_t launch_cg_init_resid(tl_ctx *c, const double *guess);
 hipError_t launch_resid_norm_slot(tl_ctx *c, const double *b, const double *x, int slot);
-hipError_t launch_cg_pw64(tl_ctx *c, int t);
-hipError_t launch_cg_urv64(tl_ctx *c, int t);
+// tref > 0: the launch is being captured into a hipGraph batch whose first
+// iteration is tref (t only selects the buffers; the kernel takes its
+// iteration number from Ctrl::tg)
+hipError_t launch_cg_pw64(tl_ctx *c, int t, int tref = 0);
+hipError_t launch_cg_urv64(tl_ctx *c, int t, int tref = 0);
 hipError_t launch_cg_pfinal64(tl_ctx *c, int t);
 hipError_t launch_cg_w64(tl_ctx *c, int t);
 hipError_t launch_cg_ur64(tl_ctx *c, int t);
 hipError_t launch_cg_p64(tl_ctx *c, int t);
-hipError_t launch_cg_w32(tl_ctx *c, int t);
-hipError_t launch_cg_ur32(tl_ctx *c, int t);
+hipError_t launch_cg_w32(tl_ctx *c, int t, int tref = 0);
+hipError_t launch_cg_ur32(tl_ctx *c, int t, int tref = 0);
+hipError_t launch_graph_base(tl_ctx *c, int k);
+hipError_t launch_graph_seed(tl_ctx *c, int t0);
 hipError_t launch_residual(tl_ctx *c);
 hipError_t launch_cg64_true_residual(tl_ctx *c, int T);
 hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t);

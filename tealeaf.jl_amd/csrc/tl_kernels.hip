@@ -292,6 +292,28 @@ __device__ void record_stop(Ctrl *c, int t, int why)
     c->stop_iter = t;
 }
 
+// Iteration number of a CG iteration kernel.  Stream launches pass t >= 1.
+// The launches of a hipGraph batch pass -i for the i-th iteration of the
+// batch and take t = Ctrl::tg + i, tg being set by the batch's first node
+// (k_graph_base), so one captured graph serves every batch.
+__device__ __forceinline__ int iter_of(int targ, const Ctrl *c) { return targ > 0 ? targ : c->tg - targ; }
+
+// first node of a hipGraph batch of k iterations: tg = first iteration of
+// this replay, tgn = first iteration of the next one
+__global__ __launch_bounds__(64) void k_graph_base(Ctrl *c, int k)
+{
+    if (threadIdx.x != 0) return;
+    const int t0 = c->tgn;
+    c->tg = t0;
+    c->tgn = t0 + k;
+}
+
+// before the first replay of a solve: the first batch starts at iteration t0
+__global__ __launch_bounds__(64) void k_graph_seed(Ctrl *c, int t0)
+{
+    if (threadIdx.x == 0) c->tgn = t0;
+}
+
 // -------------------------------------------------------- init kernels
 
 __global__ __launch_bounds__(TB) void k_init_ctrl(Ctrl *c, int mode, int maxiters, double eps, double rel_tol,
@@ -885,6 +907,7 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
                                                double *__restrict__ pw, const double *__restrict__ prcur, unsigned nbc,
                                                double *__restrict__ hist_b)
 {
+    t = iter_of(t, ctrl);
     if (t > ctrl->stop_iter) return;
     T beta = (T)0;
     const bool fuse = t >= 2;
@@ -996,6 +1019,7 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
                                                const double *__restrict__ pw, unsigned nbw,
                                                double *__restrict__ prcur, double *__restrict__ hist_a)
 {
+    t = iter_of(t, ctrl);
     if (t > ctrl->stop_iter) return;
     double ad;
     if (!cg_alpha_slot<TBV>(g, t, ctrl, pw, nbw, hist_a, &ad)) return;
@@ -1061,6 +1085,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                                             const double *__restrict__ prcur, unsigned nbc,
                                             double *__restrict__ part_out, const float *__restrict__ pprev)
 {
+    t = iter_of(t, ctrl);
     if (t > ctrl->stop_iter) return;
     float beta = 0.0f, alpha = 0.0f;
     const bool fuse = !UR && t >= 2;
@@ -1637,10 +1662,16 @@ static inline unsigned &nbr(tl_ctx *c, int t) { return c->nb_r[t & 1]; }
 static inline double *p64(tl_ctx *c, int t) { return ((t - 1) & 1) ? c->p_alt : c->f[TL_F_P]; }
 static inline float *p32(tl_ctx *c, int t) { return ((t - 1) & 1) ? c->p32b : c->p32; }
 
-// event pair around one launch when this kernel is being profiled
+// kernel argument for iteration t: t itself, or, in a launch captured into a
+// hipGraph batch whose first iteration is tref, -(t - tref) (the kernel adds
+// the replay's first iteration, Ctrl::tg)
+static inline int targ(int t, int tref) { return tref > 0 ? -(t - tref) : t; }
+
+// event pair around one launch when this kernel is being profiled (never
+// inside a graph capture)
 static bool prof_start(tl_ctx *c, int kind)
 {
-    if (!(c->prof_mask & (1u << kind))) return false;
+    if (c->capturing || !(c->prof_mask & (1u << kind))) return false;
     if (c->prof_seq[kind & 7]++ % c->prof_period != 0) return false;
     if (c->prof_used + 2 > c->prof_ev.size()) {
         hipEvent_t e0, e1;
@@ -1759,11 +1790,11 @@ hipError_t launch_cg_p64(tl_ctx *c, int t)
 }
 
 // fused CG, FP64: PW(t) then UR(t)
-hipError_t launch_cg_pw64(tl_ctx *c, int t)
+hipError_t launch_cg_pw64(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_W64);
     k_cg_pwv<double, 2><<<nb2(c), TBV, 0, c->stream>>>(
-        c->g, c->g.ty2, t, t >= 2 ? p64(c, t - 1) : nullptr, p64(c, t), c->f[TL_F_R], c->f[TL_F_KX], c->f[TL_F_KY],
+        c->g, c->g.ty2, targ(t, tref), t >= 2 ? p64(c, t - 1) : nullptr, p64(c, t), c->f[TL_F_R], c->f[TL_F_KX], c->f[TL_F_KY],
         c->f[TL_F_W], c->ctrl, part_w(c), t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0,
         c->hist_b);
     c->nb_w = nb2(c);
@@ -1771,11 +1802,11 @@ hipError_t launch_cg_pw64(tl_ctx *c, int t)
     return hipGetLastError();
 }
 
-hipError_t launch_cg_urv64(tl_ctx *c, int t)
+hipError_t launch_cg_urv64(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_UR64);
     k_cg_urv<double, 2><<<nb2(c), TBV, 0, c->stream>>>(
-        c->g, c->g.ty2, t, c->f[TL_F_U], p64(c, t), c->f[TL_F_R], c->f[TL_F_W], c->ctrl, part_w(c), c->nb_w,
+        c->g, c->g.ty2, targ(t, tref), c->f[TL_F_U], p64(c, t), c->f[TL_F_R], c->f[TL_F_W], c->ctrl, part_w(c), c->nb_w,
         part_r(c, t), c->hist_a);
     nbr(c, t) = nb2(c);
     if (prof) prof_stop(c);
@@ -1791,25 +1822,37 @@ hipError_t launch_cg_pfinal64(tl_ctx *c, int T)
     return hipGetLastError();
 }
 
-hipError_t launch_cg_w32(tl_ctx *c, int t)
+hipError_t launch_cg_w32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_W32);
     k_cgg<4, false><<<nb4p(c), TBV, 0, c->stream>>>(
-        c->g, c->g.typ4, t, t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gc32, c->hx32, c->hy32,
+        c->g, c->g.typ4, targ(t, tref), t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gc32, c->hx32, c->hy32,
         c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c), nullptr);
     c->nb_w = nb4p(c);
     if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_ur32(tl_ctx *c, int t)
+hipError_t launch_cg_ur32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_UR32);
     k_cgg<4, true><<<nb4(c), TBV, 0, c->stream>>>(
-        c->g, c->g.ty4, t, nullptr, p32(c, t), c->r32, c->x32, c->gc32, c->hx32, c->hy32, c->ctrl, part_w(c),
+        c->g, c->g.ty4, targ(t, tref), nullptr, p32(c, t), c->r32, c->x32, c->gc32, c->hx32, c->hy32, c->ctrl, part_w(c),
         c->nb_w, nullptr, 0, part_r(c, t), t >= 2 ? p32(c, t - 1) : nullptr);
     nbr(c, t) = nb4(c);
     if (prof) prof_stop(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_graph_base(tl_ctx *c, int k)
+{
+    k_graph_base<<<1, 64, 0, c->stream>>>(c->ctrl, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_graph_seed(tl_ctx *c, int t0)
+{
+    k_graph_seed<<<1, 64, 0, c->stream>>>(c->ctrl, t0);
     return hipGetLastError();
 }
 

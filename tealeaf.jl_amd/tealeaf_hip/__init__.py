@@ -20,7 +20,8 @@ from typing import Callable, List, Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (FID, FIELDS, TL_CG, TL_CHEBY, TL_FP32_IR, TL_FP64, TL_JACOBI, TL_PPCG, TL_STOP_REFERENCE,
+from ._lib import (FID, FIELDS, TL_CG, TL_CHEBY, TL_FP32_IR, TL_FP64, TL_JACOBI, TL_LAUNCH_GRAPH,
+                   TL_LAUNCH_STREAM, TL_PPCG, TL_STOP_REFERENCE,
                    TL_STOP_RELATIVE, TL_WARM_GUESS, TL_WARM_LEARNED, TL_WARM_NONE, X_DENSITY, X_ENERGY,
                    X_ENERGY0, X_P, X_SD, X_U, TeaLeafError, check, load)
 

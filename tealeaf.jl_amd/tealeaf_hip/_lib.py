@@ -18,6 +18,7 @@ TL_CG, TL_CHEBY, TL_PPCG, TL_JACOBI = 0, 1, 2, 3
 TL_FP64, TL_FP32_IR = 0, 1
 TL_STOP_REFERENCE, TL_STOP_RELATIVE = 0, 1
 TL_WARM_NONE, TL_WARM_LEARNED, TL_WARM_GUESS = 0, 1, 2
+TL_LAUNCH_STREAM, TL_LAUNCH_GRAPH = 0, 1
 FIELDS = ["density0", "density", "energy0", "energy", "u", "u0", "p", "r", "mi", "w", "kx", "ky", "sd"]
 FID = {n: i for i, n in enumerate(FIELDS)}
 X_DENSITY, X_P, X_ENERGY0, X_ENERGY, X_U, X_SD = 1, 2, 4, 8, 16, 32
@@ -45,7 +46,7 @@ class SolveParams(C.Structure):
         ("solver", C.c_int), ("coefficient", C.c_int), ("max_iters", C.c_int), ("presteps", C.c_int),
         ("ppcg_inner_steps", C.c_int), ("error_switch", C.c_int), ("eps", C.c_double), ("epslim", C.c_double),
         ("precision", C.c_int), ("stop_rule", C.c_int), ("rel_tol", C.c_double), ("warm_start", C.c_int),
-        ("check_result", C.c_int), ("ir_delta", C.c_double), ("reserved", C.c_int * 8),
+        ("check_result", C.c_int), ("ir_delta", C.c_double), ("launch", C.c_int), ("reserved", C.c_int * 7),
     ]
 
 

@@ -42,16 +42,16 @@ PROFILE_PERIOD = 15  # odd: both parities of the alternating FP32 UR kernel are 
 # A CG iteration is two kernels.
 #   FP64:  PW = [p = r + beta p; w = A p] reads p, r, Kx, Ky, writes p', w  (48 B)
 #          UR = [u += alpha p; r -= alpha w] reads u, p, r, w, writes u, r   (48 B)
-#   FP32 inner CG (operator from one cell-coefficient array gc, w not stored):
-#          PW reads p, r, gc, writes p'                                     (16 B)
-#          UR = [r -= alpha A p] reads p, r, gc, writes r                   (16 B)
+#   FP32 inner CG (operator from one bf16 cell-coefficient array gc, w not stored):
+#          PW reads p, r (4 B each), gc (2 B), writes p'                    (14 B)
+#          UR = [r -= alpha A p] reads p, r, gc, writes r                   (14 B)
 #          and at even iterations also x += alpha(t-1) p(t-1) + alpha(t) p(t):
 #          reads x, p(t-1), writes x                                     (+12 B)
-#          i.e. 22 B per launch on average
+#          i.e. 20 B per launch on average
 # P64 is only the final p update of an FP64 solve.
 KERNEL_BYTES = {
     _lib.TL_K_CG_W64: 48, _lib.TL_K_CG_UR64: 48, _lib.TL_K_CG_P64: 24,
-    _lib.TL_K_CG_W32: 16, _lib.TL_K_CG_UR32: 22, _lib.TL_K_CG_P32: 0,
+    _lib.TL_K_CG_W32: 14, _lib.TL_K_CG_UR32: 20, _lib.TL_K_CG_P32: 0,
 }
 WEIGHTS = os.path.join(ROOT, "tealeaf.jl_amd", "weights", "unet_warmstart.npy")
 # HBM bytes per launch of the CG kernels at 8192^2 from rocprofv3 FETCH_SIZE /

@@ -154,7 +154,7 @@ extern "C" void tl_destroy(tl_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     unet_destroy(c);
-    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->gc32, c->p32, c->r32, c->x32,
+    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->gcell, c->p32, c->r32, c->x32,
                     c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
     for (int i = 0; i < TL_NFIELDS; ++i)
         if (c->f[i]) (void)hipFree(c->f[i]);
@@ -290,7 +290,7 @@ static std::vector<uintptr_t> graph_key(const tl_ctx *c, int kind, int k)
             v.push_back((uintptr_t)q);
     } else {
         for (const void *q : {(const void *)c->p32, (const void *)c->p32b, (const void *)c->r32,
-                              (const void *)c->x32, (const void *)c->gc32})
+                              (const void *)c->x32, (const void *)c->gcell})
             v.push_back((uintptr_t)q);
         v.push_back(hx);
         v.push_back(hy);
@@ -418,9 +418,13 @@ static int ensure_history(tl_ctx *c, int n)
 
 static int ensure_fp32(tl_ctx *c)
 {
-    if (c->gc32) return TL_OK;
+    if (c->gcell) return TL_OK;
     const size_t n = c->g.elems;
-    RETIF(alloc_f(c, &c->gc32, n));
+    if (hipMalloc((void **)&c->gcell, n * sizeof(tl_gc_t)) != hipSuccess) {
+        c->err = "device allocation failed";
+        c->gcell = nullptr;
+        return TL_E_NOMEM;
+    }
     RETIF(alloc_f(c, &c->p32, n));
     RETIF(alloc_f(c, &c->p32b, n));
     RETIF(alloc_f(c, &c->r32, n));

@@ -56,6 +56,19 @@ constexpr int TBV = TL_TBV;
 #else
 #define TL_CGG_ATTR
 #endif
+// storage of the FP32 inner CG's cell coefficient gc: bfloat16 (1) or FP32 (0).
+// The operator built from bf16 gc has every face conductivity within a
+// relative 2^-8 of the FP64 one, so it is spectrally equivalent to it within
+// that factor and the FP64 residual refresh still converges to the FP64
+// operator's solution; the kernels read 2 bytes per cell instead of 4.
+#ifndef TL_GC_BF16
+#define TL_GC_BF16 1
+#endif
+#if TL_GC_BF16
+typedef unsigned short tl_gc_t;  // bfloat16 bits
+#else
+typedef float tl_gc_t;
+#endif
 // non-temporal (streaming) vector loads / stores in those kernels
 #ifndef TL_NT_LOAD
 #define TL_NT_LOAD 0
@@ -127,9 +140,10 @@ struct tl_ctx {
     double *f[TL_NFIELDS] = {};
     double *u_alt = nullptr, *sd_alt = nullptr, *guess = nullptr;
     // FP32 fields of the mixed-precision solve
-    // FP32 inner CG: cell coefficient gc = 1/w and its face scale (rx/2, ry/2),
-    // search direction, residual, correction
-    float *gc32 = nullptr, *p32 = nullptr, *r32 = nullptr, *x32 = nullptr;
+    // FP32 inner CG: cell coefficient gc = 1/w (tl_gc_t) and its face scale
+    // (rx/2, ry/2), search direction, residual, correction
+    tl::tl_gc_t *gcell = nullptr;
+    float *p32 = nullptr, *r32 = nullptr, *x32 = nullptr;
     float hx32 = 0.0f, hy32 = 0.0f;
     // reductions
     double *partials = nullptr;

@@ -1057,17 +1057,41 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
     write_partials<1, TBV>(acc, prcur, g.nbmax);
 }
 
+// cell coefficients as FP32: bf16 -> FP32 is exact (the bits shifted up)
+__device__ __forceinline__ float gc_val(float v) { return v; }
+__device__ __forceinline__ float gc_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <int V>
+__device__ __forceinline__ void gc_load(float (&d)[V], const float *__restrict__ p)
+{
+    vload<float, V>(d, p);
+}
+
+// V = 4 bf16 values (8 bytes, 8-byte aligned like the FP32 vector at the same index)
+template <int V>
+__device__ __forceinline__ void gc_load(float (&d)[V], const unsigned short *__restrict__ p)
+{
+    static_assert(V == 4, "bf16 coefficients are loaded 4 at a time");
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    const u2 v = *reinterpret_cast<const u2 *>(p);
+    d[0] = __uint_as_float(v[0] << 16);
+    d[1] = __uint_as_float(v[0] & 0xffff0000u);
+    d[2] = __uint_as_float(v[1] << 16);
+    d[3] = __uint_as_float(v[1] & 0xffff0000u);
+}
+
 // ------------------------------- FP32 inner CG on a cell-coefficient array
 //
 // The inner CG of the mixed-precision solver builds the operator from one
-// FP32 array per cell, gc = 1/w (w = density or 1/density, CG.jl:56-58),
-// instead of the two face arrays: kx(x) = hx (gc(x-1) + gc(x)) with
-// hx = rx/2 equals rx (w(x-1) + w(x)) / (2 w(x-1) w(x)) of CG.jl:61-68 up to
-// FP32 rounding (ky likewise with hy = ry/2).  At the grid edge the clamped
-// neighbour supplies what the reflective halo holds (kernels.jl:191-210).
-// w = A p is never stored: UR recomputes it from p, so an iteration moves 40
-// bytes per cell -- PW reads p, r, gc and writes p'; UR reads x, p, r, gc and
-// writes x, r.
+// array per cell, gc = 1/w (w = density or 1/density, CG.jl:56-58), stored
+// as bfloat16 (tl_gc_t), instead of the two FP64 face arrays:
+// kx(x) = hx (gc(x-1) + gc(x)) with hx = rx/2 equals
+// rx (w(x-1) + w(x)) / (2 w(x-1) w(x)) of CG.jl:61-68 up to the rounding of
+// gc (ky likewise with hy = ry/2), computed in FP32.  At the grid edge the
+// clamped neighbour supplies what the reflective halo holds
+// (kernels.jl:191-210).  w = A p is never stored: UR recomputes it from p.
+// Bytes per cell: PW reads p, r (4 each), gc (2) and writes p' (4): 14;
+// UR 14 or 26 (below).
 //
 // UR = false (PW, iteration t): for t >= 2 first the end of iteration t-1 --
 //   beta = rr(t-1)/rr(t-2), the stop test (CG.jl:39-42, 24) -- then
@@ -1076,11 +1100,11 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
 // UR = true (iteration t): alpha = rr(t-1)/pw(t), then r -= alpha A p and
 //   partial sums of r^2 (CG.ur!, CG.jl:93-98); the correction x += alpha p
 //   is applied for two iterations at once, at even t, which saves x's read
-//   and write at odd t: UR moves 28 bytes per cell at even t, 16 at odd t.
+//   and write at odd t: UR moves 26 bytes per cell at even t, 14 at odd t.
 template <int V, bool UR>
 __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, const float *__restrict__ pin,
                                             float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
-                                            const float *__restrict__ gc, float hx, float hy,
+                                            const tl_gc_t *__restrict__ gc, float hx, float hy,
                                             Ctrl *__restrict__ ctrl, const double *__restrict__ pw_in, unsigned nbw,
                                             const double *__restrict__ prcur, unsigned nbc,
                                             double *__restrict__ part_out, const float *__restrict__ pprev)
@@ -1121,7 +1145,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
     const float *pp = fuse ? pin : pout;
     // p (after this launch's update) at element i
     auto pnew = [&](long i) -> float { return fuse ? r[i] + beta * pp[i] : pp[i]; };
-    auto gcol = [&](long i) -> float { return gc[i]; };
+    auto gcol = [&](long i) -> float { return gc_val(gc[i]); };
     double acc[1] = {0.0};
     if (cl.y0 < g.ny) {
         float dn[V], cur[V], gdn[V], gcur[V];
@@ -1129,8 +1153,8 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
             const long i0 = at(g, cl.xl, cl.y0), id = at(g, cl.xl, max(cl.y0 - 1, 0));
             float a[V], b2[V];
             vload<float, V>(a, pp + i0);
-            vload<float, V>(gcur, gc + i0);
-            vload<float, V>(gdn, gc + id);
+            gc_load<V>(gcur, gc + i0);
+            gc_load<V>(gdn, gc + id);
             if (fuse) {
                 vload<float, V>(b2, r + i0);
 #pragma unroll
@@ -1153,7 +1177,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 const long iu = atc(g, cl.xl, y + 1);
                 vload<float, V>(up[q], pp + iu);
                 if (fuse) vload<float, V>(rup[q], r + iu);
-                vload<float, V>(gup[q], gc + iu);
+                gc_load<V>(gup[q], gc + iu);
                 if (UR) {
                     const long ic = atc(g, cl.xl, y);
                     vload<float, V>(rv[q], r + ic);
@@ -1602,16 +1626,28 @@ __global__ __launch_bounds__(TB) void k_jacobi(Geo g, int t, const double *__res
 
 // ------------------------------------------------------ mixed precision
 
-// FP32 cell coefficient of the inner CG over the whole allocation:
-// gc = 1/w with w = density (coefficient 1) or 1/density (CG.jl:56-58);
-// cells without a density (padding) get 0
-__global__ __launch_bounds__(TB) void k_gc32(const double *__restrict__ rho, float *__restrict__ gc, size_t n,
-                                             int coefficient)
+// cell coefficient of the inner CG over the whole allocation:
+// gc = 1/w with w = density (coefficient 1) or 1/density (CG.jl:56-58),
+// rounded to FP32 and then (bf16 storage) to the nearest bfloat16, ties to
+// even; cells without a density (padding) get 0
+__device__ __forceinline__ tl_gc_t gc_store(float f)
+{
+#if TL_GC_BF16
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (tl_gc_t)(u >> 16);
+#else
+    return f;
+#endif
+}
+
+__global__ __launch_bounds__(TB) void k_gcell(const double *__restrict__ rho, tl_gc_t *__restrict__ gc, size_t n,
+                                              int coefficient)
 {
     const size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     const double d = rho[i];
-    gc[i] = coefficient == 2 ? (float)d : (d != 0.0 ? (float)(1.0 / d) : 0.0f);
+    gc[i] = gc_store(coefficient == 2 ? (float)d : (d != 0.0 ? (float)(1.0 / d) : 0.0f));
 }
 
 // FP64 solution accumulation of the mixed-precision CG after iteration T:
@@ -1826,7 +1862,7 @@ hipError_t launch_cg_w32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_W32);
     k_cgg<4, false><<<nb4p(c), TBV, 0, c->stream>>>(
-        c->g, c->g.typ4, targ(t, tref), t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gc32, c->hx32, c->hy32,
+        c->g, c->g.typ4, targ(t, tref), t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gcell, c->hx32, c->hy32,
         c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c), nullptr);
     c->nb_w = nb4p(c);
     if (prof) prof_stop(c);
@@ -1837,7 +1873,7 @@ hipError_t launch_cg_ur32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_UR32);
     k_cgg<4, true><<<nb4(c), TBV, 0, c->stream>>>(
-        c->g, c->g.ty4, targ(t, tref), nullptr, p32(c, t), c->r32, c->x32, c->gc32, c->hx32, c->hy32, c->ctrl, part_w(c),
+        c->g, c->g.ty4, targ(t, tref), nullptr, p32(c, t), c->r32, c->x32, c->gcell, c->hx32, c->hy32, c->ctrl, part_w(c),
         c->nb_w, nullptr, 0, part_r(c, t), t >= 2 ? p32(c, t - 1) : nullptr);
     nbr(c, t) = nb4(c);
     if (prof) prof_stop(c);
@@ -1988,7 +2024,7 @@ hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient)
 {
     const size_t n = c->g.elems;
     const unsigned nb = (unsigned)((n + TB - 1) / TB);
-    k_gc32<<<nb, TB, 0, c->stream>>>(c->f[TL_F_DENSITY], c->gc32, n, coefficient);
+    k_gcell<<<nb, TB, 0, c->stream>>>(c->f[TL_F_DENSITY], c->gcell, n, coefficient);
     c->hx32 = (float)(0.5 * rx);
     c->hy32 = (float)(0.5 * ry);
     return hipGetLastError();

@@ -33,6 +33,13 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_RV
 #define TL_RV 4
 #endif
+// rows per load batch of the FP32 inner-CG kernels (PW, UR)
+#ifndef TL_RV_PW32
+#define TL_RV_PW32 4
+#endif
+#ifndef TL_RV_UR32
+#define TL_RV_UR32 8
+#endif
 #ifndef TL_VEC_BLOCKS32
 #define TL_VEC_BLOCKS32 2048
 #endif
@@ -45,6 +52,11 @@ constexpr int TBV = TL_TBV;
 // neighbour exchange of the FP32 PW kernel by DPP wave shifts (else LDS shuffles)
 #ifndef TL_DPP
 #define TL_DPP 1
+#endif
+// FP32 inner CG: lanes 0/63 load the neighbour column with the batched row
+// loads instead of when the row is used
+#ifndef TL_EDGE_PF
+#define TL_EDGE_PF 1
 #endif
 // XCD-aware block-to-tile mapping of the vectorised kernels
 #ifndef TL_XCD_MAP

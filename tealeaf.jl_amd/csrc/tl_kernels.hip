@@ -55,24 +55,24 @@ Geo make_geo(int nx, int ny, int hd)
     // vector kernels: the same target block count, 4 (FP32) or 2 (FP64) columns per thread
     // (measured at 8192^2: the FP64 kernels run fastest with about 8192
     // blocks, the FP32 UR with about 2048 and the FP32 PW with about 1024)
-    auto vec = [&](int v, long target, int &gxv, int &gyv, int &tyv) {
+    auto vec = [&](int v, long target, int rv, int &gxv, int &gyv, int &tyv) {
         gxv = (nx + v * TBV - 1) / (v * TBV);
         const long r2 = (long)ny * gxv;
         int t2 = (int)((r2 + target - 1) / target);
-        t2 = (t2 + TL_RV - 1) / TL_RV * TL_RV;  // a multiple of the vector kernels' row chunk
+        t2 = (t2 + rv - 1) / rv * rv;  // a multiple of the kernel's row batch
         // bands of at least 16 rows (each re-reads the row above and below
         // it) unless that leaves fewer than about 1024 blocks
-        const long tcap = std::min<long>(16, std::max<long>(TL_RV, r2 / 1024));
-        const int tmin = (int)((tcap + TL_RV - 1) / TL_RV * TL_RV);
+        const long tcap = std::min<long>(16, std::max<long>(rv, r2 / 1024));
+        const int tmin = (int)((tcap + rv - 1) / rv * rv);
         if (t2 < tmin) t2 = tmin;
         if (t2 > 1024) t2 = 1024;
         tyv = t2;
         gyv = (ny + t2 - 1) / t2;
     };
-    vec(4, TL_VEC_BLOCKS32, g.gx4, g.gy4, g.ty4);
+    vec(4, TL_VEC_BLOCKS32, TL_RV_UR32, g.gx4, g.gy4, g.ty4);
     int gxp;
-    vec(4, TL_VEC_BLOCKS32_PW, gxp, g.gyp4, g.typ4);  // FP32 PW: same columns, its own bands
-    vec(2, TL_VEC_BLOCKS64, g.gx2, g.gy2, g.ty2);
+    vec(4, TL_VEC_BLOCKS32_PW, TL_RV_PW32, gxp, g.gyp4, g.typ4);  // FP32 PW: same columns, its own bands
+    vec(2, TL_VEC_BLOCKS64, TL_RV, g.gx2, g.gy2, g.ty2);
     g.nbmax = std::max(std::max(g.gx * g.gy, g.gx4 * g.gyp4), std::max(g.gx4 * g.gy4, g.gx2 * g.gy2));
     return g;
 }
@@ -933,8 +933,10 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
     }
     const VCol<V> cl = vcolumn<V>(g, ty);
     const T *pp = fuse ? pin : pout;
+#if !TL_EDGE_PF
     // p after the update at column x of row element index i
     auto pnew = [&](long i) -> T { return fuse ? r[i] + beta * pp[i] : pp[i]; };
+#endif
     double acc[1] = {0.0};
     if (cl.y0 < g.ny) {
         T dn[V], cur[V], kyd[V];
@@ -957,8 +959,23 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
             }
             vload<T, V>(kyd, ky + i0);
         }
+#if TL_EDGE_PF
+        // lanes 0 / 63 of a wave read the column left / right of the wave's
+        // columns (and lane 63 the face coefficient right of it) together
+        // with the row loads below, not when the row is used
+        const bool eneed = (cl.lane == 0 && cl.xl > 0) || (cl.lane == 63 && cl.xl + V < g.nx);
+        const long eoff = cl.lane == 0 ? -1 : V;
+        T ecur = (T)0;
+        if (eneed) {
+            const long ie = at(g, cl.xl, cl.y0) + eoff;
+            ecur = fuse ? r[ie] + beta * pp[ie] : pp[ie];
+        }
+#endif
         for (int yb = cl.y0; yb < cl.y1; yb += RV) {
             T up[RV][V], kxv[RV][V], kyu[RV][V], rup[RV][V];
+#if TL_EDGE_PF
+            T eup_p[RV], eup_r[RV], ekx[RV];
+#endif
 #pragma unroll
             for (int q = 0; q < RV; ++q) {
                 const int y = yb + q;
@@ -967,6 +984,14 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
                 if (fuse) vload<T, V>(rup[q], r + iu);
                 vload<T, V>(kxv[q], kx + ic);
                 vload<T, V>(kyu[q], ky + at(g, cl.xl, min(y + 1, g.ny)));
+#if TL_EDGE_PF
+                eup_p[q] = eup_r[q] = ekx[q] = (T)0;
+                if (eneed) {
+                    eup_p[q] = pp[iu + eoff];
+                    if (fuse) eup_r[q] = r[iu + eoff];
+                }
+                if (cl.lane == 63) ekx[q] = kx[ic + V];
+#endif
             }
 #pragma unroll
             for (int q = 0; q < RV; ++q) {
@@ -981,11 +1006,19 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
                     for (int j = 0; j < V; ++j) u2[j] = cur[j];
                 }
                 T lv, rv;
+#if TL_EDGE_PF
+                auto col_val = [&](int) -> T { return ecur; };
+#else
                 auto col_val = [&](int x) -> T { return pnew(ic + (x - cl.xl)); };
+#endif
                 vneigh<T, V>(g, cl, cur, col_val, lv, rv);
                 // kx of the face right of the last column: next lane's first kx
                 T kxn = lane_down1<false>(kxv[q][0]);
+#if TL_EDGE_PF
+                if (cl.lane == 63) kxn = ekx[q];
+#else
                 if (cl.lane == 63) kxn = kx[ic + V];
+#endif
                 T wv[V];
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
@@ -1005,6 +1038,11 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
                     cur[j] = u2[j];
                     kyd[j] = kyu[q][j];
                 }
+#if TL_EDGE_PF
+                // the row above becomes the current row (at the top edge the
+                // clamped load read the current row itself)
+                ecur = fuse ? eup_r[q] + beta * eup_p[q] : eup_p[q];
+#endif
             }
         }
     }
@@ -1101,7 +1139,7 @@ __device__ __forceinline__ void gc_load(float (&d)[V], const unsigned short *__r
 //   partial sums of r^2 (CG.ur!, CG.jl:93-98); the correction x += alpha p
 //   is applied for two iterations at once, at even t, which saves x's read
 //   and write at odd t: UR moves 26 bytes per cell at even t, 14 at odd t.
-template <int V, bool UR>
+template <int V, bool UR, int RB>
 __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, const float *__restrict__ pin,
                                             float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
                                             const tl_gc_t *__restrict__ gc, float hx, float hy,
@@ -1143,9 +1181,11 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
     }
     const VCol<V> cl = vcolumn<V>(g, ty);
     const float *pp = fuse ? pin : pout;
+#if !TL_EDGE_PF
     // p (after this launch's update) at element i
     auto pnew = [&](long i) -> float { return fuse ? r[i] + beta * pp[i] : pp[i]; };
     auto gcol = [&](long i) -> float { return gc_val(gc[i]); };
+#endif
     double acc[1] = {0.0};
     if (cl.y0 < g.ny) {
         float dn[V], cur[V], gdn[V], gcur[V];
@@ -1169,15 +1209,40 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 vload<float, V>(dn, pp + id);
             }
         }
-        for (int yb = cl.y0; yb < cl.y1; yb += RV) {
-            float up[RV][V], rup[RV][V], gup[RV][V], xv[RV][V], rv[RV][V], pv[RV][V];
+#if TL_EDGE_PF
+        // lanes 0 / 63 of a wave read the column left / right of the wave's
+        // columns together with the row loads below, not when the row is used
+        const bool eneed = (cl.lane == 0 && cl.xl > 0) || (cl.lane == 63 && cl.xl + V < g.nx);
+        const long eoff = cl.lane == 0 ? -1 : V;
+        float ecur_p = 0.0f, ecur_g = 0.0f;
+        if (eneed) {
+            const long ie = at(g, cl.xl, cl.y0) + eoff;
+            ecur_p = fuse ? r[ie] + beta * pp[ie] : pp[ie];
+            ecur_g = gc_val(gc[ie]);
+        }
+#endif
+        for (int yb = cl.y0; yb < cl.y1; yb += RB) {
+            float up[RB][V], rup[RB][V], gup[RB][V], xv[RB][V], rv[RB][V], pv[RB][V];
+#if TL_EDGE_PF
+            float eup_p[RB], eup_r[RB];
+            tl_gc_t eup_g[RB];
+#endif
 #pragma unroll
-            for (int q = 0; q < RV; ++q) {
+            for (int q = 0; q < RB; ++q) {
                 const int y = yb + q;
                 const long iu = atc(g, cl.xl, y + 1);
                 vload<float, V>(up[q], pp + iu);
                 if (fuse) vload<float, V>(rup[q], r + iu);
                 gc_load<V>(gup[q], gc + iu);
+#if TL_EDGE_PF
+                eup_p[q] = eup_r[q] = 0.0f;
+                eup_g[q] = 0;
+                if (eneed) {
+                    eup_p[q] = pp[iu + eoff];
+                    if (fuse) eup_r[q] = r[iu + eoff];
+                    eup_g[q] = gc[iu + eoff];
+                }
+#endif
                 if (UR) {
                     const long ic = atc(g, cl.xl, y);
                     vload<float, V>(rv[q], r + ic);
@@ -1188,7 +1253,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 }
             }
 #pragma unroll
-            for (int q = 0; q < RV; ++q) {
+            for (int q = 0; q < RB; ++q) {
                 const int y = yb + q;
                 const long ic = atc(g, cl.xl, y);
                 float u2[V], gu[V];
@@ -1205,8 +1270,13 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                     }
                 }
                 float lv, rvn, glv, grv;
+#if TL_EDGE_PF
+                auto col_val = [&](int) -> float { return ecur_p; };
+                auto col_g = [&](int) -> float { return ecur_g; };
+#else
                 auto col_val = [&](int xx) -> float { return pnew(ic + (xx - cl.xl)); };
                 auto col_g = [&](int xx) -> float { return gcol(ic + (xx - cl.xl)); };
+#endif
                 vneigh<float, V, TL_DPP && !UR>(g, cl, cur, col_val, lv, rvn);
                 vneigh<float, V, TL_DPP && !UR>(g, cl, gcur, col_g, glv, grv);
                 float wv[V];
@@ -1246,6 +1316,12 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                     gdn[j] = gcur[j];
                     gcur[j] = gu[j];
                 }
+#if TL_EDGE_PF
+                // the row above becomes the current row (at the top edge the
+                // clamped load read the current row itself)
+                ecur_p = fuse ? eup_r[q] + beta * eup_p[q] : eup_p[q];
+                ecur_g = gc_val(eup_g[q]);
+#endif
             }
         }
     }
@@ -1861,7 +1937,7 @@ hipError_t launch_cg_pfinal64(tl_ctx *c, int T)
 hipError_t launch_cg_w32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_W32);
-    k_cgg<4, false><<<nb4p(c), TBV, 0, c->stream>>>(
+    k_cgg<4, false, TL_RV_PW32><<<nb4p(c), TBV, 0, c->stream>>>(
         c->g, c->g.typ4, targ(t, tref), t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gcell, c->hx32, c->hy32,
         c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c), nullptr);
     c->nb_w = nb4p(c);
@@ -1872,7 +1948,7 @@ hipError_t launch_cg_w32(tl_ctx *c, int t, int tref)
 hipError_t launch_cg_ur32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_UR32);
-    k_cgg<4, true><<<nb4(c), TBV, 0, c->stream>>>(
+    k_cgg<4, true, TL_RV_UR32><<<nb4(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty4, targ(t, tref), nullptr, p32(c, t), c->r32, c->x32, c->gcell, c->hx32, c->hy32, c->ctrl, part_w(c),
         c->nb_w, nullptr, 0, part_r(c, t), t >= 2 ? p32(c, t - 1) : nullptr);
     nbr(c, t) = nb4(c);

@@ -1098,6 +1098,15 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
 // cell coefficients as FP32: bf16 -> FP32 is exact (the bits shifted up)
 __device__ __forceinline__ float gc_val(float v) { return v; }
 __device__ __forceinline__ float gc_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+// a coefficient held in a 32-bit register as loaded (bf16: zero-extended)
+// and converted only where it is used, so that no instruction between the
+// load and the use waits for it
+__device__ __forceinline__ unsigned gc_raw(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ unsigned gc_raw(unsigned short h) { return (unsigned)h; }
+__device__ __forceinline__ float gc_from_raw(unsigned b)
+{
+    return __uint_as_float(TL_GC_BF16 ? b << 16 : b);
+}
 
 template <int V>
 __device__ __forceinline__ void gc_load(float (&d)[V], const float *__restrict__ p)
@@ -1225,7 +1234,9 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
             float up[RB][V], rup[RB][V], gup[RB][V], xv[RB][V], rv[RB][V], pv[RB][V];
 #if TL_EDGE_PF
             float eup_p[RB], eup_r[RB];
-            tl_gc_t eup_g[RB];
+            // a full register per value: bf16 halves packed into one register
+            // would make the compiler wait for each load before the next row's
+            unsigned eup_g[RB];
 #endif
 #pragma unroll
             for (int q = 0; q < RB; ++q) {
@@ -1240,7 +1251,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 if (eneed) {
                     eup_p[q] = pp[iu + eoff];
                     if (fuse) eup_r[q] = r[iu + eoff];
-                    eup_g[q] = gc[iu + eoff];
+                    eup_g[q] = gc_raw(gc[iu + eoff]);
                 }
 #endif
                 if (UR) {
@@ -1320,7 +1331,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 // the row above becomes the current row (at the top edge the
                 // clamped load read the current row itself)
                 ecur_p = fuse ? eup_r[q] + beta * eup_p[q] : eup_p[q];
-                ecur_g = gc_val(eup_g[q]);
+                ecur_g = gc_from_raw(eup_g[q]);
 #endif
             }
         }

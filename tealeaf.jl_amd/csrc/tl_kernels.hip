@@ -1098,6 +1098,37 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
 // cell coefficients as FP32: bf16 -> FP32 is exact (the bits shifted up)
 __device__ __forceinline__ float gc_val(float v) { return v; }
 __device__ __forceinline__ float gc_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+// V coefficients as loaded (bf16: two per 32-bit word), unpacked where used
+constexpr int GCW = TL_GC_BF16 ? 2 : 1;
+template <int V>
+struct GcRaw {
+    unsigned w[V / GCW];
+};
+
+template <int V>
+__device__ __forceinline__ void gc_load_raw(GcRaw<V> &d, const tl_gc_t *__restrict__ p)
+{
+    typedef unsigned uv __attribute__((ext_vector_type(V / GCW)));
+    const uv v = *reinterpret_cast<const uv *>(p);
+#pragma unroll
+    for (int k = 0; k < V / GCW; ++k) d.w[k] = v[k];
+}
+
+template <int V>
+__device__ __forceinline__ void gc_unpack(float (&d)[V], const GcRaw<V> &s)
+{
+#if TL_GC_BF16
+#pragma unroll
+    for (int k = 0; k < V / 2; ++k) {
+        d[2 * k] = __uint_as_float(s.w[k] << 16);
+        d[2 * k + 1] = __uint_as_float(s.w[k] & 0xffff0000u);
+    }
+#else
+#pragma unroll
+    for (int k = 0; k < V; ++k) d[k] = __uint_as_float(s.w[k]);
+#endif
+}
+
 // a coefficient held in a 32-bit register as loaded (bf16: zero-extended)
 // and converted only where it is used, so that no instruction between the
 // load and the use waits for it
@@ -1148,7 +1179,7 @@ __device__ __forceinline__ void gc_load(float (&d)[V], const unsigned short *__r
 //   partial sums of r^2 (CG.ur!, CG.jl:93-98); the correction x += alpha p
 //   is applied for two iterations at once, at even t, which saves x's read
 //   and write at odd t: UR moves 26 bytes per cell at even t, 14 at odd t.
-template <int V, bool UR, int RB>
+template <int V, bool UR, int RB, bool PIPE>
 __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, const float *__restrict__ pin,
                                             float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
                                             const tl_gc_t *__restrict__ gc, float hx, float hy,
@@ -1230,49 +1261,54 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
             ecur_g = gc_val(gc[ie]);
         }
 #endif
-        for (int yb = cl.y0; yb < cl.y1; yb += RB) {
-            float up[RB][V], rup[RB][V], gup[RB][V], xv[RB][V], rv[RB][V], pv[RB][V];
+        // the loads of RB rows, issued together
+        struct Batch {
+            float up[RB][V], rup[RB][V], xv[RB][V], rv[RB][V], pv[RB][V];
+            GcRaw<V> gup[RB];
 #if TL_EDGE_PF
             float eup_p[RB], eup_r[RB];
             // a full register per value: bf16 halves packed into one register
             // would make the compiler wait for each load before the next row's
             unsigned eup_g[RB];
 #endif
+        };
+        auto load_batch = [&](Batch &B, int yb) {
 #pragma unroll
             for (int q = 0; q < RB; ++q) {
                 const int y = yb + q;
                 const long iu = atc(g, cl.xl, y + 1);
-                vload<float, V>(up[q], pp + iu);
-                if (fuse) vload<float, V>(rup[q], r + iu);
-                gc_load<V>(gup[q], gc + iu);
+                vload<float, V>(B.up[q], pp + iu);
+                if (fuse) vload<float, V>(B.rup[q], r + iu);
+                gc_load_raw<V>(B.gup[q], gc + iu);
 #if TL_EDGE_PF
-                eup_p[q] = eup_r[q] = 0.0f;
-                eup_g[q] = 0;
+                B.eup_p[q] = B.eup_r[q] = 0.0f;
+                B.eup_g[q] = 0;
                 if (eneed) {
-                    eup_p[q] = pp[iu + eoff];
-                    if (fuse) eup_r[q] = r[iu + eoff];
-                    eup_g[q] = gc_raw(gc[iu + eoff]);
+                    B.eup_p[q] = pp[iu + eoff];
+                    if (fuse) B.eup_r[q] = r[iu + eoff];
+                    B.eup_g[q] = gc_raw(gc[iu + eoff]);
                 }
 #endif
                 if (UR) {
                     const long ic = atc(g, cl.xl, y);
-                    vload<float, V>(rv[q], r + ic);
+                    vload<float, V>(B.rv[q], r + ic);
                     if (xupd) {
-                        vload<float, V>(xv[q], x + ic);
-                        vload<float, V>(pv[q], pprev + ic);
+                        vload<float, V>(B.xv[q], x + ic);
+                        vload<float, V>(B.pv[q], pprev + ic);
                     }
                 }
             }
+        };
+        // rows yb .. yb+RB-1 from a batch's loads (and the rows carried below)
+        auto compute_batch = [&](const Batch &B, int yb) {
 #pragma unroll
             for (int q = 0; q < RB; ++q) {
                 const int y = yb + q;
                 const long ic = atc(g, cl.xl, y);
                 float u2[V], gu[V];
+                gc_unpack<V>(gu, B.gup[q]);
 #pragma unroll
-                for (int j = 0; j < V; ++j) {
-                    u2[j] = fuse ? rup[q][j] + beta * up[q][j] : up[q][j];
-                    gu[j] = gup[q][j];
-                }
+                for (int j = 0; j < V; ++j) u2[j] = fuse ? B.rup[q][j] + beta * B.up[q][j] : B.up[q][j];
                 if (y + 1 >= g.ny) {  // reflection at the top edge: the row itself
 #pragma unroll
                     for (int j = 0; j < V; ++j) {
@@ -1307,8 +1343,8 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
 #pragma unroll
                         for (int j = 0; j < V; ++j) {
                             // the two steps' updates in order, each rounded as if stored
-                            xn[j] = xupd ? (xv[q][j] + alpha_prev * pv[q][j]) + alpha * cur[j] : 0.0f;
-                            rn[j] = rv[q][j] - alpha * wv[j];
+                            xn[j] = xupd ? (B.xv[q][j] + alpha_prev * B.pv[q][j]) + alpha * cur[j] : 0.0f;
+                            rn[j] = B.rv[q][j] - alpha * wv[j];
                             if (j < nv) acc[0] += (double)rn[j] * (double)rn[j];
                         }
                         if (xupd) vstore_n<float, V>(x + ic, xn, nv);
@@ -1330,9 +1366,29 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
 #if TL_EDGE_PF
                 // the row above becomes the current row (at the top edge the
                 // clamped load read the current row itself)
-                ecur_p = fuse ? eup_r[q] + beta * eup_p[q] : eup_p[q];
-                ecur_g = gc_from_raw(eup_g[q]);
+                ecur_p = fuse ? B.eup_r[q] + beta * B.eup_p[q] : B.eup_p[q];
+                ecur_g = gc_from_raw(B.eup_g[q]);
 #endif
+            }
+        };
+        if constexpr (PIPE) {
+            // software pipeline: the next batch's loads are in flight while a
+            // batch is computed (two register sets, used alternately)
+            Batch b0, b1;
+            load_batch(b0, cl.y0);
+            for (int yb = cl.y0; yb < cl.y1; yb += 2 * RB) {
+                const bool more = yb + RB < cl.y1;
+                if (more) load_batch(b1, yb + RB);
+                compute_batch(b0, yb);
+                if (!more) break;
+                if (yb + 2 * RB < cl.y1) load_batch(b0, yb + 2 * RB);
+                compute_batch(b1, yb + RB);
+            }
+        } else {
+            for (int yb = cl.y0; yb < cl.y1; yb += RB) {
+                Batch b0;
+                load_batch(b0, yb);
+                compute_batch(b0, yb);
             }
         }
     }
@@ -1948,7 +2004,7 @@ hipError_t launch_cg_pfinal64(tl_ctx *c, int T)
 hipError_t launch_cg_w32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_W32);
-    k_cgg<4, false, TL_RV_PW32><<<nb4p(c), TBV, 0, c->stream>>>(
+    k_cgg<4, false, TL_RV_PW32, TL_PIPE_PW32 != 0><<<nb4p(c), TBV, 0, c->stream>>>(
         c->g, c->g.typ4, targ(t, tref), t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gcell, c->hx32, c->hy32,
         c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c), nullptr);
     c->nb_w = nb4p(c);
@@ -1959,7 +2015,7 @@ hipError_t launch_cg_w32(tl_ctx *c, int t, int tref)
 hipError_t launch_cg_ur32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_UR32);
-    k_cgg<4, true, TL_RV_UR32><<<nb4(c), TBV, 0, c->stream>>>(
+    k_cgg<4, true, TL_RV_UR32, TL_PIPE_UR32 != 0><<<nb4(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty4, targ(t, tref), nullptr, p32(c, t), c->r32, c->x32, c->gcell, c->hx32, c->hy32, c->ctrl, part_w(c),
         c->nb_w, nullptr, 0, part_r(c, t), t >= 2 ? p32(c, t - 1) : nullptr);
     nbr(c, t) = nb4(c);

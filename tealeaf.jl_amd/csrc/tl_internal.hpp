@@ -50,6 +50,20 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_PIPE_UR32
 #define TL_PIPE_UR32 1
 #endif
+// the same for the FP64 CG kernels (measured: PW 4 rows pipelined 2.6%
+// faster than plain; the streaming UR gains nothing from it)
+#ifndef TL_RV_PW64
+#define TL_RV_PW64 TL_RV
+#endif
+#ifndef TL_RV_UR64
+#define TL_RV_UR64 TL_RV
+#endif
+#ifndef TL_PIPE_PW64
+#define TL_PIPE_PW64 1
+#endif
+#ifndef TL_PIPE_UR64
+#define TL_PIPE_UR64 0
+#endif
 #ifndef TL_VEC_BLOCKS32
 #define TL_VEC_BLOCKS32 2048
 #endif

@@ -737,7 +737,6 @@ __global__ __launch_bounds__(TB) void k_cg_p(Geo g, int t, T *__restrict__ p, co
 // valid clamped vector and store nothing.  When nx is not a multiple of V the
 // last vector of a row reads into the right halo/padding (inside the
 // allocation) and stores only its interior columns.
-constexpr int RV = TL_RV;
 
 template <typename T, int V>
 __device__ __forceinline__ void vload(T (&d)[V], const T *__restrict__ p)
@@ -895,12 +894,39 @@ __device__ __forceinline__ T vsmvp(const Geo &g, int xl, int j, const T (&c)[V],
     return smvp5(c[j], l, r, dn, up, kxl, kxr, kyd, kyu);
 }
 
+// Rows y0 .. y1-1 in batches of RB: load(batch, first row) issues a batch's
+// loads, compute(batch, first row) uses them.  PIPE: software pipeline --
+// the next batch's loads are issued before the current batch is computed,
+// into a second register set, so a wave keeps loads in flight.
+template <bool PIPE, int RB, class Batch, class Load, class Compute>
+__device__ __forceinline__ void row_batches(int y0, int y1, Load &&load, Compute &&compute)
+{
+    if constexpr (PIPE) {
+        Batch b0, b1;
+        load(b0, y0);
+        for (int yb = y0; yb < y1; yb += 2 * RB) {
+            const bool more = yb + RB < y1;
+            if (more) load(b1, yb + RB);
+            compute(b0, yb);
+            if (!more) break;
+            if (yb + 2 * RB < y1) load(b0, yb + 2 * RB);
+            compute(b1, yb + RB);
+        }
+    } else {
+        for (int yb = y0; yb < y1; yb += RB) {
+            Batch b0;
+            load(b0, yb);
+            compute(b0, yb);
+        }
+    }
+}
+
 // Iteration t of the fused CG: for t >= 2, first the end of iteration t-1 --
 // beta = rr(t-1)/rr(t-2) and the stop test (CG.jl:39-42, 24) -- then
 // p = r + beta p (CG.p!, written to pout; p is ping-ponged because every
 // thread reads its neighbours' old p), then w = A p and the partial sums of
 // p.w (CG.w!).  For t = 1, p comes from CG.init!.
-template <typename T, int V>
+template <typename T, int V, int RB, bool PIPE>
 __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *__restrict__ pin, T *__restrict__ pout,
                                                const T *__restrict__ r, const T *__restrict__ kx,
                                                const T *__restrict__ ky, T *__restrict__ w, Ctrl *__restrict__ ctrl,
@@ -971,35 +997,40 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
             ecur = fuse ? r[ie] + beta * pp[ie] : pp[ie];
         }
 #endif
-        for (int yb = cl.y0; yb < cl.y1; yb += RV) {
-            T up[RV][V], kxv[RV][V], kyu[RV][V], rup[RV][V];
+        // the loads of RB rows, issued together
+        struct Batch {
+            T up[RB][V], kxv[RB][V], kyu[RB][V], rup[RB][V];
 #if TL_EDGE_PF
-            T eup_p[RV], eup_r[RV], ekx[RV];
+            T eup_p[RB], eup_r[RB], ekx[RB];
 #endif
+        };
+        auto load_batch = [&](Batch &B, int yb) {
 #pragma unroll
-            for (int q = 0; q < RV; ++q) {
+            for (int q = 0; q < RB; ++q) {
                 const int y = yb + q;
                 const long ic = atc(g, cl.xl, y), iu = atc(g, cl.xl, y + 1);
-                vload<T, V>(up[q], pp + iu);
-                if (fuse) vload<T, V>(rup[q], r + iu);
-                vload<T, V>(kxv[q], kx + ic);
-                vload<T, V>(kyu[q], ky + at(g, cl.xl, min(y + 1, g.ny)));
+                vload<T, V>(B.up[q], pp + iu);
+                if (fuse) vload<T, V>(B.rup[q], r + iu);
+                vload<T, V>(B.kxv[q], kx + ic);
+                vload<T, V>(B.kyu[q], ky + at(g, cl.xl, min(y + 1, g.ny)));
 #if TL_EDGE_PF
-                eup_p[q] = eup_r[q] = ekx[q] = (T)0;
+                B.eup_p[q] = B.eup_r[q] = B.ekx[q] = (T)0;
                 if (eneed) {
-                    eup_p[q] = pp[iu + eoff];
-                    if (fuse) eup_r[q] = r[iu + eoff];
+                    B.eup_p[q] = pp[iu + eoff];
+                    if (fuse) B.eup_r[q] = r[iu + eoff];
                 }
-                if (cl.lane == 63) ekx[q] = kx[ic + V];
+                if (cl.lane == 63) B.ekx[q] = kx[ic + V];
 #endif
             }
+        };
+        auto compute_batch = [&](const Batch &B, int yb) {
 #pragma unroll
-            for (int q = 0; q < RV; ++q) {
+            for (int q = 0; q < RB; ++q) {
                 const int y = yb + q;
                 const long ic = atc(g, cl.xl, y);
                 T u2[V];
 #pragma unroll
-                for (int j = 0; j < V; ++j) u2[j] = fuse ? rup[q][j] + beta * up[q][j] : up[q][j];
+                for (int j = 0; j < V; ++j) u2[j] = fuse ? B.rup[q][j] + beta * B.up[q][j] : B.up[q][j];
                 // clamp the row above at the top edge (reflection: the row itself)
                 if (y + 1 >= g.ny) {
 #pragma unroll
@@ -1013,17 +1044,18 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
 #endif
                 vneigh<T, V>(g, cl, cur, col_val, lv, rv);
                 // kx of the face right of the last column: next lane's first kx
-                T kxn = lane_down1<false>(kxv[q][0]);
+                T kxn = lane_down1<false>(B.kxv[q][0]);
 #if TL_EDGE_PF
-                if (cl.lane == 63) kxn = ekx[q];
+                if (cl.lane == 63) kxn = B.ekx[q];
 #else
                 if (cl.lane == 63) kxn = kx[ic + V];
 #endif
                 T wv[V];
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
-                    const T kxr = j < V - 1 ? kxv[q][j + 1] : kxn;
-                    wv[j] = vsmvp<T, V>(g, cl.xl, j, cur, lv, rv, dn[j], u2[j], kxv[q][j], kxr, kyd[j], kyu[q][j]);
+                    const T kxr = j < V - 1 ? B.kxv[q][j + 1] : kxn;
+                    wv[j] = vsmvp<T, V>(g, cl.xl, j, cur, lv, rv, dn[j], u2[j], B.kxv[q][j], kxr, kyd[j],
+                                        B.kyu[q][j]);
                 }
                 if (y < cl.y1 && cl.active) {
                     vstore_n<T, V>(w + ic, wv, g.nx - cl.x0);
@@ -1036,22 +1068,23 @@ __global__ __launch_bounds__(TBV) void k_cg_pwv(Geo g, int ty, int t, const T *_
                 for (int j = 0; j < V; ++j) {
                     dn[j] = cur[j];
                     cur[j] = u2[j];
-                    kyd[j] = kyu[q][j];
+                    kyd[j] = B.kyu[q][j];
                 }
 #if TL_EDGE_PF
                 // the row above becomes the current row (at the top edge the
                 // clamped load read the current row itself)
-                ecur = fuse ? eup_r[q] + beta * eup_p[q] : eup_p[q];
+                ecur = fuse ? B.eup_r[q] + beta * B.eup_p[q] : B.eup_p[q];
 #endif
             }
-        }
+        };
+        row_batches<PIPE, RB, Batch>(cl.y0, cl.y1, load_batch, compute_batch);
     }
     write_partials<1, TBV>(acc, pw, g.nbmax);
 }
 
 // CG.ur!, CG.jl:93-98 on V-vectors: alpha = rro/pw from the partials, then
 // u += alpha p; r -= alpha w; partial sums of r^2
-template <typename T, int V>
+template <typename T, int V, int RB, bool PIPE>
 __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restrict__ u, const T *__restrict__ p,
                                                T *__restrict__ r, const T *__restrict__ w, Ctrl *__restrict__ ctrl,
                                                const double *__restrict__ pw, unsigned nbw,
@@ -1065,32 +1098,37 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
     const VCol<V> cl = vcolumn<V>(g, ty);
     double acc[1] = {0.0};
     if (cl.active && cl.y0 < g.ny) {
-        for (int yb = cl.y0; yb < cl.y1; yb += RV) {
-            T uv[RV][V], pv[RV][V], rv[RV][V], wv[RV][V];
+        struct Batch {
+            T uv[RB][V], pv[RB][V], rv[RB][V], wv[RB][V];
+        };
+        auto load_batch = [&](Batch &B, int yb) {
 #pragma unroll
-            for (int q = 0; q < RV; ++q) {
+            for (int q = 0; q < RB; ++q) {
                 const long i = atc(g, cl.xl, yb + q);
-                vload<T, V>(uv[q], u + i);
-                vload<T, V>(pv[q], p + i);
-                vload<T, V>(rv[q], r + i);
-                vload<T, V>(wv[q], w + i);
+                vload<T, V>(B.uv[q], u + i);
+                vload<T, V>(B.pv[q], p + i);
+                vload<T, V>(B.rv[q], r + i);
+                vload<T, V>(B.wv[q], w + i);
             }
+        };
+        auto compute_batch = [&](const Batch &B, int yb) {
 #pragma unroll
-            for (int q = 0; q < RV; ++q) {
+            for (int q = 0; q < RB; ++q) {
                 if (yb + q < cl.y1) {
                     const long i = at(g, cl.xl, yb + q);
                     T un[V], rn[V];
 #pragma unroll
                     for (int j = 0; j < V; ++j) {
-                        un[j] = uv[q][j] + alpha * pv[q][j];
-                        rn[j] = rv[q][j] - alpha * wv[q][j];
+                        un[j] = B.uv[q][j] + alpha * B.pv[q][j];
+                        rn[j] = B.rv[q][j] - alpha * B.wv[q][j];
                         if (cl.x0 + j < g.nx) acc[0] += (double)rn[j] * (double)rn[j];
                     }
                     vstore_n<T, V>(u + i, un, g.nx - cl.x0);
                     vstore_n<T, V>(r + i, rn, g.nx - cl.x0);
                 }
             }
-        }
+        };
+        row_batches<PIPE, RB, Batch>(cl.y0, cl.y1, load_batch, compute_batch);
     }
     write_partials<1, TBV>(acc, prcur, g.nbmax);
 }
@@ -1098,7 +1136,19 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
 // cell coefficients as FP32: bf16 -> FP32 is exact (the bits shifted up)
 __device__ __forceinline__ float gc_val(float v) { return v; }
 __device__ __forceinline__ float gc_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-// V coefficients as loaded (bf16: two per 32-bit word), unpacked where used
+
+// a coefficient held in a 32-bit register as loaded (bf16: zero-extended)
+// and converted only where it is used, so that no instruction between the
+// load and the use waits for it
+__device__ __forceinline__ unsigned gc_raw(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ unsigned gc_raw(unsigned short h) { return (unsigned)h; }
+__device__ __forceinline__ float gc_from_raw(unsigned b)
+{
+    return __uint_as_float(TL_GC_BF16 ? b << 16 : b);
+}
+
+// V coefficients as loaded (bf16: two per 32-bit word; the vector is aligned
+// like the FP32 vector at the same element index), unpacked where used
 constexpr int GCW = TL_GC_BF16 ? 2 : 1;
 template <int V>
 struct GcRaw {
@@ -1129,33 +1179,12 @@ __device__ __forceinline__ void gc_unpack(float (&d)[V], const GcRaw<V> &s)
 #endif
 }
 
-// a coefficient held in a 32-bit register as loaded (bf16: zero-extended)
-// and converted only where it is used, so that no instruction between the
-// load and the use waits for it
-__device__ __forceinline__ unsigned gc_raw(float v) { return __float_as_uint(v); }
-__device__ __forceinline__ unsigned gc_raw(unsigned short h) { return (unsigned)h; }
-__device__ __forceinline__ float gc_from_raw(unsigned b)
-{
-    return __uint_as_float(TL_GC_BF16 ? b << 16 : b);
-}
-
 template <int V>
-__device__ __forceinline__ void gc_load(float (&d)[V], const float *__restrict__ p)
+__device__ __forceinline__ void gc_load(float (&d)[V], const tl_gc_t *__restrict__ p)
 {
-    vload<float, V>(d, p);
-}
-
-// V = 4 bf16 values (8 bytes, 8-byte aligned like the FP32 vector at the same index)
-template <int V>
-__device__ __forceinline__ void gc_load(float (&d)[V], const unsigned short *__restrict__ p)
-{
-    static_assert(V == 4, "bf16 coefficients are loaded 4 at a time");
-    typedef unsigned u2 __attribute__((ext_vector_type(2)));
-    const u2 v = *reinterpret_cast<const u2 *>(p);
-    d[0] = __uint_as_float(v[0] << 16);
-    d[1] = __uint_as_float(v[0] & 0xffff0000u);
-    d[2] = __uint_as_float(v[1] << 16);
-    d[3] = __uint_as_float(v[1] & 0xffff0000u);
+    GcRaw<V> w;
+    gc_load_raw<V>(w, p);
+    gc_unpack<V>(d, w);
 }
 
 // ------------------------------- FP32 inner CG on a cell-coefficient array
@@ -1371,26 +1400,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
 #endif
             }
         };
-        if constexpr (PIPE) {
-            // software pipeline: the next batch's loads are in flight while a
-            // batch is computed (two register sets, used alternately)
-            Batch b0, b1;
-            load_batch(b0, cl.y0);
-            for (int yb = cl.y0; yb < cl.y1; yb += 2 * RB) {
-                const bool more = yb + RB < cl.y1;
-                if (more) load_batch(b1, yb + RB);
-                compute_batch(b0, yb);
-                if (!more) break;
-                if (yb + 2 * RB < cl.y1) load_batch(b0, yb + 2 * RB);
-                compute_batch(b1, yb + RB);
-            }
-        } else {
-            for (int yb = cl.y0; yb < cl.y1; yb += RB) {
-                Batch b0;
-                load_batch(b0, yb);
-                compute_batch(b0, yb);
-            }
-        }
+        row_batches<PIPE, RB, Batch>(cl.y0, cl.y1, load_batch, compute_batch);
     }
     write_partials<1, TBV>(acc, part_out, g.nbmax);
 }
@@ -1972,7 +1982,7 @@ hipError_t launch_cg_p64(tl_ctx *c, int t)
 hipError_t launch_cg_pw64(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_W64);
-    k_cg_pwv<double, 2><<<nb2(c), TBV, 0, c->stream>>>(
+    k_cg_pwv<double, 2, TL_RV_PW64, TL_PIPE_PW64 != 0><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, targ(t, tref), t >= 2 ? p64(c, t - 1) : nullptr, p64(c, t), c->f[TL_F_R], c->f[TL_F_KX], c->f[TL_F_KY],
         c->f[TL_F_W], c->ctrl, part_w(c), t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0,
         c->hist_b);
@@ -1984,7 +1994,7 @@ hipError_t launch_cg_pw64(tl_ctx *c, int t, int tref)
 hipError_t launch_cg_urv64(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_UR64);
-    k_cg_urv<double, 2><<<nb2(c), TBV, 0, c->stream>>>(
+    k_cg_urv<double, 2, TL_RV_UR64, TL_PIPE_UR64 != 0><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, targ(t, tref), c->f[TL_F_U], p64(c, t), c->f[TL_F_R], c->f[TL_F_W], c->ctrl, part_w(c), c->nb_w,
         part_r(c, t), c->hist_a);
     nbr(c, t) = nb2(c);

@@ -14,6 +14,7 @@ computes a field.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 from typing import Callable, List, Optional
 
@@ -45,7 +46,9 @@ class _SolverModule:
         p = settings.params(solver=self.code)
         st = _lib.SolveStats()
         check(load().tl_solve(chunk._ctx, C.byref(p), rx, ry, C.byref(st)), chunk._ctx, "solve")
-        return st.as_dict()
+        out = st.as_dict()
+        chunk._record_solve(p, out)
+        return out
 
     def __repr__(self):
         return f"<solver {self.name}>"
@@ -71,6 +74,7 @@ class Settings:
         self.rel_tol = deck.params.rel_tol
         self.warm_start = deck.params.warm_start
         self.ir_delta = deck.params.ir_delta
+        self.debugfile = ""  # Settings.debugfile (settings.jl:77, run.jl --debug-out)
 
     @classmethod
     def from_text(cls, text: str) -> "Settings":
@@ -132,7 +136,7 @@ class Settings:
         for k, v in kw.items():
             if k == "solver":
                 self.solver = SOLVERS[v] if isinstance(v, str) else v
-            elif k in ("precision", "stop_rule", "rel_tol", "warm_start", "ir_delta"):
+            elif k in ("precision", "stop_rule", "rel_tol", "warm_start", "ir_delta", "debugfile"):
                 setattr(self, k, v)
             else:
                 setattr(self._deck.params, k, v)
@@ -152,6 +156,52 @@ class Chunk:
         X, Y = C.c_int(), C.c_int()
         L.tl_dims(ctx, C.byref(X), C.byref(Y))
         self.X, self.Y, self.hd = X.value, Y.value, d.halo_depth
+        self._init_host_state(settings)
+
+    def _init_host_state(self, settings: Settings):
+        """The reference Chunk's members that are not device fields
+        (chunk.jl:40-60), built as Chunk(settings) builds them (chunk.jl:68-89)
+        and kept on the host for debugrecord: mesh vectors, volume and areas,
+        the Chebyshev scalars and the CG / Chebyshev coefficient histories
+        (length maxiters), which solves update as the reference's solve! does."""
+        d = settings._deck
+        x, y, h = self.X, self.Y, self.hd
+        self.vertexx = d.xmin + d.dx * (np.arange(1, x + 2) - h - 1)
+        self.vertexy = d.ymin + d.dy * (np.arange(1, y + 2) - h - 1)
+        self.cellx = 0.5 * (self.vertexx[:-1] + self.vertexx[1:])
+        # chunk.jl:44 adds vertexx to vertexy (as written, reproduced here);
+        # with x != y that broadcast fails in the reference, and vertexy is used
+        self.celly = 0.5 * (self.vertexy[:-1] + (self.vertexx if x == y else self.vertexy)[1:])
+        self.volume = np.full((y, x), d.dx * d.dy)
+        self.xarea = np.full((y, x), d.dy)
+        self.yarea = np.full((y, x), d.dx)
+        self.theta = self.eigmin = self.eigmax = 0.0
+        n = d.params.max_iters
+        self.cgalpha, self.cgbeta = np.zeros(n), np.zeros(n)
+        self.chebyalpha, self.chebybeta = np.zeros(n), np.zeros(n)
+
+    def _record_solve(self, p: _lib.SolveParams, st: dict):
+        """What an FP64 solve! leaves in the reference's chunk besides the
+        fields: cgα/cgβ of its CG iterations (CG.jl:37-38), and for
+        Chebyshev/PPCG after the switch eigmin/eigmax (kernels.jl:47-48), θ and
+        chebyα/chebyβ (Cheby.coef!, Cheby.jl:121-135; PPCG.jl:42).  The
+        mixed-precision solver has no counterpart and records nothing."""
+        if p.precision != TL_FP64:
+            return
+        n = len(self.cgalpha)
+        if p.solver in (TL_CG, TL_CHEBY, TL_PPCG):
+            k = min(int(st["cg_iters"]), n)
+            if k > 0:
+                a, b = self.cg_history(k)
+                self.cgalpha[:k], self.cgbeta[:k] = a, b
+        if p.solver in (TL_CHEBY, TL_PPCG) and st["eigmax"] > 0.0:
+            self.eigmin, self.eigmax = st["eigmin"], st["eigmax"]
+            m = p.max_iters - int(st["cg_iters"]) if p.solver == TL_CHEBY else p.ppcg_inner_steps
+            m = max(0, min(m, n))
+            a, b, th = np.zeros(max(m, 1)), np.zeros(max(m, 1)), C.c_double()
+            load().tl_cheby_coef(self.eigmin, self.eigmax, m, _dp(a), _dp(b), C.byref(th))
+            self.chebyalpha[:m], self.chebybeta[:m] = a[:m], b[:m]
+            self.theta = th.value
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -262,19 +312,39 @@ def _julia_float(x: float) -> str:
     return f"{m}e{int(e)}"
 
 
+# the members of the reference's Chunk in declaration order (chunk.jl:19-60),
+# as debugrecord writes them (x and y excluded), and where this host keeps them
+DEBUG_MEMBERS = (list(FIELDS) + ["vertexx", "vertexy", "cellx", "celly", "volume", "xarea", "yarea",
+                                 "\u03b8", "eigmin", "eigmax", "cg\u03b1", "cg\u03b2", "cheby\u03b1",
+                                 "cheby\u03b2"])
+_HOST_MEMBER = {"\u03b8": "theta", "cg\u03b1": "cgalpha", "cg\u03b2": "cgbeta", "cheby\u03b1": "chebyalpha",
+                "cheby\u03b2": "chebybeta"}
+
+
+def _julia_string(v) -> str:
+    """getstring, TeaLeaf.jl:104-107: a matrix as one line per column (the
+    values of k = 1..X separated by spaces), a vector as one line, a scalar
+    as Julia's string()."""
+    if np.ndim(v) == 0:
+        return _julia_float(float(v))
+    if np.ndim(v) == 1:
+        return " ".join(_julia_float(float(x)) for x in v)
+    return "\n".join(" ".join(_julia_float(float(x)) for x in row) for row in v)
+
+
 def debugrecord(chunk: Chunk, path: str):
-    """debugrecord, TeaLeaf.jl:90-102: append every field of the chunk to
-    `path` -- the field name, then one line per column j (the values of
-    k = 1..X separated by spaces, getstring of TeaLeaf.jl:104-106), then an
-    empty line; two newlines after the chunk.  The device holds the 13
-    fields of chunk.jl:25-38 (not the mesh vectors or the solver history,
-    which the reference also writes).  The number format follows Julia's
-    print; it is not checked against Julia (none here)."""
-    with open(path, "a") as f:
-        for name in FIELDS:
-            a = chunk.field(name)  # [j, k]
+    """debugrecord, TeaLeaf.jl:90-102: append every member of the chunk
+    except x and y to `path`, in declaration order (chunk.jl:19-60): the
+    member name, its getstring (TeaLeaf.jl:104-107), then an empty line; two
+    newlines after the chunk.  Device fields are read back; the other members
+    come from the host mirror the solves keep (Chunk._record_solve).  The
+    number format follows Julia's print; it is not checked against Julia
+    (none here)."""
+    with open(path, "a", encoding="utf-8") as f:
+        for name in DEBUG_MEMBERS:
+            v = chunk.field(name) if name in FIELDS else getattr(chunk, _HOST_MEMBER.get(name, name))
             f.write(name + "\n")
-            f.write("\n".join(" ".join(_julia_float(float(v)) for v in row) for row in a))
+            f.write(_julia_string(v))
             f.write("\n\n")
         f.write("\n\n")
 
@@ -284,15 +354,23 @@ def step(chunk: Chunk, settings: Settings) -> dict:
     p = settings.params()
     st = _lib.SolveStats()
     check(load().tl_step(chunk._ctx, C.byref(p), settings.dtinit, C.byref(st)), chunk._ctx, "tl_step")
-    return st.as_dict()
+    out = st.as_dict()
+    chunk._record_solve(p, out)
+    return out
 
 
 def diffuse(chunk: Chunk, settings: Settings, nsteps: Optional[int] = None,
             callback: Optional[Callable[[int, np.ndarray, dict], None]] = None):
-    """diffuse!, TeaLeaf.jl:62-83: returns (summaries[n, 4], stats list)."""
+    """diffuse!, TeaLeaf.jl:62-83: returns (summaries[n, 4], stats list).
+    With settings.debugfile set, the file is replaced and the chunk is dumped
+    to it at the start of every timestep (TeaLeaf.jl:63-68)."""
     n = int(nsteps if nsteps is not None else settings.endstep)
+    if settings.debugfile and os.path.isfile(settings.debugfile):
+        os.remove(settings.debugfile)
     summ, stats = [], []
     for t in range(1, n + 1):
+        if settings.debugfile:
+            debugrecord(chunk, settings.debugfile)
         st = step(chunk, settings)
         s = fieldsummary(chunk)
         summ.append(s)

@@ -95,3 +95,18 @@ def test_debugrecord_layout(tmp_path):
     got = np.array([[float(v) for v in r.split(" ")] for r in rows])
     assert np.array_equal(got, c.field("energy"))
     assert [ln for ln in lines if ln in T.FIELDS] == list(T.FIELDS)
+
+
+def test_diffuse_writes_debugfile_each_step(tmp_path):
+    """diffuse! with a debug file (TeaLeaf.jl:63-68): an old file is replaced
+    and the chunk is dumped at the start of every timestep."""
+    print("case: diffuse with debugfile, 8x8, 2 steps", flush=True)
+    s = T.Settings.from_text(decks.reference_deck(8, 2, max_iters=100))
+    path = tmp_path / "dbg.txt"
+    path.write_text("stale\n")
+    s.set(debugfile=str(path))
+    c = T.initialiseapp(s)
+    T.diffuse(c, s, 2)
+    lines = path.read_text(encoding="utf-8").split("\n")
+    assert "stale" not in lines
+    assert lines.count("density0") == 2 and lines.count("chebyβ") == 2

@@ -210,3 +210,61 @@ def test_full_size_8192_one_step_properties():
         assert abs(after[2] / before[2] - 1) < 1e-8, (before, after)
         res[precision] = after
     assert _rel(res["fp32_ir"], res["fp64"]) < 1e-9, res
+
+
+@pytest.mark.parametrize("solver", ["cg", "cheby", "ppcg"])
+def test_debugrecord_members_match_oracle(tmp_path, solver):
+    """debugrecord (TeaLeaf.jl:90-107) writes every member of the reference's
+    Chunk in declaration order: device fields, mesh vectors and areas as
+    Chunk(settings) builds them, and what solve! leaves in the chunk -- the
+    CG history, eigenvalue bounds, theta and the Chebyshev coefficients --
+    checked against the oracle's chunk after the same step."""
+    print(f"case: debugrecord after one {solver} step, 48^2 reference deck", flush=True)
+    text = decks.reference_deck(48, 1, solver=solver, max_iters=300, presteps=8)
+    o, c, s = _pair(text)
+    ost = o.step()
+    st = T.step(c, s)
+    assert st["cg_iters"] == ost["cg_iters"]
+    path = tmp_path / "debug.txt"
+    T.debugrecord(c, str(path))
+    lines = path.read_text(encoding="utf-8").split("\n")
+    heads = [i for i, ln in enumerate(lines) if ln in T.DEBUG_MEMBERS]
+    assert [lines[i] for i in heads] == T.DEBUG_MEMBERS
+
+    def member(name):
+        i = lines.index(name)
+        rows = []
+        for ln in lines[i + 1:]:
+            if ln == "":
+                break
+            rows.append([float(v) for v in ln.split(" ")])
+        return np.array(rows)
+
+    X, Y, hd = c.X, c.Y, c.hd
+    d = O.parse_deck(text)
+    assert np.array_equal(member("energy"), c.field("energy"))
+    # Chunk(settings), chunk.jl:76-77: vertex i (1-based) at xmin + dx (i - hd - 1)
+    vx, vy = member("vertexx")[0], member("vertexy")[0]
+    assert vx.shape == (X + 1,) and vy.shape == (Y + 1,)
+    assert np.allclose(vx, d.xmin + d.dx * (np.arange(X + 1) - hd), rtol=0, atol=1e-12)
+    assert np.allclose(member("cellx")[0], 0.5 * (vx[:-1] + vx[1:]), rtol=0, atol=1e-12)
+    assert np.array_equal(member("volume"), np.full((Y, X), d.dx * d.dy))
+    assert np.array_equal(member("xarea"), np.full((Y, X), d.dy))
+    # solver history of this step against the oracle's (1-based there)
+    k = st["cg_iters"]
+    m = min(k, 12)  # the first iterations, before the residual reaches rounding level
+    for name, which in (("cgα", 0), ("cgβ", 1)):
+        got, ref = member(name)[0], o.history(which, k)[1:k + 1]
+        assert got.shape == (300,)
+        assert np.allclose(got[:m], ref[:m], rtol=1e-9, atol=0), name
+    if solver == "cg":
+        assert member("θ")[0, 0] == 0.0 and member("eigmax")[0, 0] == 0.0
+    else:
+        assert abs(member("eigmin")[0, 0] / ost["eigmin"] - 1) < 1e-9
+        assert abs(member("eigmax")[0, 0] / ost["eigmax"] - 1) < 1e-9
+        assert abs(member("θ")[0, 0] / (0.5 * (ost["eigmin"] + ost["eigmax"])) - 1) < 1e-9
+        n = 300 - k if solver == "cheby" else 10
+        for name, which in (("chebyα", 2), ("chebyβ", 3)):
+            got, ref = member(name)[0], o.history(which, n)[1:n + 1]
+            assert np.allclose(got[:n], ref, rtol=1e-8, atol=0), name
+            assert np.all(got[n:] == 0.0)

@@ -189,6 +189,28 @@ def test_small_and_odd_grids(nx, ny, hd, precision):
             assert np.max(np.abs(a - b)) <= 1e-9 * scale, (f, np.max(np.abs(a - b)), scale)
 
 
+@pytest.mark.timeout(400)
+@pytest.mark.parametrize("solver", ["cg", "cheby"])
+def test_reference_deck_4096_one_step_vs_oracle(solver):
+    """The 3-state reference deck at 4096^2 (the FP64 solver configuration),
+    one timestep with each solver's own stop rule against the oracle run on
+    the host cores: every field-summary quantity within 1e-10, iteration
+    counts as in the other parity tests."""
+    print(f"case: {solver} 4096^2 reference deck, 1 step vs the oracle", flush=True)
+    text = decks.reference_deck(4096, 1, solver=solver)
+    o = O.Oracle(O.parse_deck(text))
+    o.initialise()
+    osum, ost = o.run(1)
+    s = T.Settings.from_text(text)
+    c = T.initialiseapp(s)
+    gsum, gst = T.diffuse(c, s, 1)
+    c.close()
+    print(f"  oracle {ost[0]['iterations']} / library {gst[0]['iterations']} iterations", flush=True)
+    assert _rel(gsum[0], osum[0]) < RTOL, (gsum[0], osum[0])
+    assert gst[0]["cg_iters"] == ost[0]["cg_iters"] or solver == "cg"
+    assert abs(gst[0]["iterations"] - ost[0]["iterations"]) <= max(3, ost[0]["iterations"] // 50)
+
+
 def test_full_size_8192_one_step_properties():
     """The benchmark's problem at full size (8192^2, random rectangles seed
     1001), one timestep with each precision: the FP64 residual meets 1e-10,

@@ -8,6 +8,14 @@
 // fetched from HBM once per sweep.  Left/right neighbours are re-read through
 // L1/L2.
 //
+// The CG iteration kernels (k_cg_pwv, k_cg_urv, k_cgg) are vectorised: a
+// thread owns V = 4 (FP32) or 2 (FP64) consecutive columns loaded as one
+// 16-byte vector, left/right neighbours come from the adjacent lanes (lanes
+// 0 and 63 load the column outside the wave together with the row loads),
+// and the rows of a band are loaded in batches that are software-pipelined
+// (row_batches): the next batch's loads are issued before the current batch
+// is computed, so a wave keeps loads in flight while it computes.
+//
 // Halo.  The depth-1 reflective halo of kernels.jl:191-210 (with the index
 // correction of the intended algorithm) is applied by clamping neighbour
 // indices to the interior: a reflected ghost cell holds exactly the value of

@@ -1,9 +1,14 @@
 """Train the warm-start U-Net on the solver's own trajectories, then measure
 what it gains.
 
-    python -m tealeaf_hip.train_warmstart --res 1024 --problems 60 --steps 20 \\
+    python -m tealeaf_hip.train_warmstart --res 1024 --problems 200 --steps 50 --crops 1 \\
         --train-steps 3000 --out weights/unet_warmstart.npy \\
-        --eval-res 4096 --eval-seeds 1000 1002 --eval-steps 3
+        --eval-res 4096 --eval-seeds 1000 --eval-steps 50
+
+(the defaults: 200 problems x 50 timesteps at 1024^2, one 128x128 crop per
+solve, i.e. 10,000 samples).  With --out, the weights, the loss curve and a
+JSON record of the arguments, the dataset size, the final loss and the
+evaluation rows are written next to each other.
 
 1. Trajectories: seeded random-rectangle problems (the benchmark family:
    background rho=100, e=1e-4 plus 32 rectangles; dt 0.004) at --res, solved
@@ -114,9 +119,9 @@ def evaluate(w, a):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--res", type=int, default=1024)
-    ap.add_argument("--problems", type=int, default=60)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--crops", type=int, default=6)
+    ap.add_argument("--problems", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--crops", type=int, default=1)
     ap.add_argument("--seed0", type=int, default=0)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=16)
@@ -126,8 +131,9 @@ def main(argv=None):
     ap.add_argument("--weights", default="", help="skip training and evaluate these weights")
     ap.add_argument("--eval-res", type=int, default=4096)
     ap.add_argument("--eval-seeds", type=int, nargs="*", default=[1000])
-    ap.add_argument("--eval-steps", type=int, default=3)
+    ap.add_argument("--eval-steps", type=int, default=50)
     a = ap.parse_args(argv)
+    losses = None
     if a.weights:
         w = np.load(a.weights, allow_pickle=False)
     else:
@@ -137,8 +143,13 @@ def main(argv=None):
             np.save(a.out, w, allow_pickle=False)
             np.save(os.path.splitext(a.out)[0] + "_loss.npy", np.asarray(losses, dtype=np.float32),
                     allow_pickle=False)
-    if a.eval_seeds:
-        evaluate(w, a)
+    rows = evaluate(w, a) if a.eval_seeds else []
+    if a.out and losses is not None:
+        rec = dict(args=vars(a), samples=a.problems * a.steps * a.crops,
+                   loss_first=float(np.mean(losses[:20])), loss_final=float(np.mean(losses[-50:])),
+                   eval=[{k: v for k, v in r.items() if k != "summary"} for r in rows])
+        with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
+            json.dump(rec, f, indent=1)
 
 
 if __name__ == "__main__":

@@ -267,6 +267,8 @@ def main():
     ap.add_argument("--kernel-repeat", type=int, nargs=2, metavar=("KERNEL_ID", "REPS"))
     ap.add_argument("--compare-steps", type=int, default=0,
                     help="FP32-refined vs FP64 CG side by side for this many timesteps (per-step summaries)")
+    ap.add_argument("--only", choices=("flagship", "fp64"), default=None,
+                    help="run one method only (long runs); per-step field summaries are in the JSON")
     ap.add_argument("--launch", choices=("stream", "graph"), default="stream",
                     help="CG iterations as stream launches or hipGraph replays (graph: no per-kernel timing)")
     ap.add_argument("--solvers", action="store_true",
@@ -288,11 +290,19 @@ def main():
         return
     learned = os.path.exists(WEIGHTS)
     warm = "learned" if learned else "none"
+    if a.only == "fp64":
+        base = run_method("fp64-cg", a.n, a.seed, "fp64", "none", a.steps, a.warmup, launch=a.launch)
+        print(json.dumps({"method": "fp64-cg", "grid": a.n, "seed": a.seed, "steps": a.steps,
+                          "s_per_step": [round(t, 4) for t in base["times"]],
+                          "iterations": [st["iterations"] for st in base["stats"]],
+                          "rel_resid": [st["rel_resid"] for st in base["stats"]],
+                          "summaries": [list(map(float, x)) for x in base["summaries"]]}))
+        return
     mask = (1 << _lib.TL_K_CG_W32) | (1 << _lib.TL_K_CG_UR32) | (1 << _lib.TL_K_CG_P32)
     flag = run_method("fp32-ir" + ("+learned" if learned else ""), a.n, a.seed, "fp32_ir", warm, a.steps,
                       a.warmup, mask if a.launch == "stream" else 0, a.launch)
     base = None
-    if not a.no_baseline:
+    if not a.no_baseline and a.only is None:
         base = run_method("fp64-cg", a.n, a.seed, "fp64", "none", a.steps, a.warmup, launch=a.launch)
     value = float(np.mean(flag["times"]))
     out = {
@@ -328,6 +338,8 @@ def main():
         out["speedup_vs_fp64_cg"] = round(bval / value, 3)
         diffs = [float(np.max(np.abs(f - b) / np.abs(b))) for f, b in zip(flag["summaries"], base["summaries"])]
         out["field_summary_max_rel_diff_vs_fp64"] = max(diffs)
+    if a.only == "flagship":
+        out["flagship"]["summaries"] = [list(map(float, x)) for x in flag["summaries"]]
     out["roofline"] = roofline_from(flag["prof"], a.n)
     # every profiled kernel of the flagship's timed steps, and the share of the
     # timed wall time they cover (launches past a solve's stop excluded; the

@@ -39,7 +39,8 @@ extern "C" {
 
 /* arithmetic of the solve */
 #define TL_FP64 0           /* everything in FP64 (the reference's Float64) */
-#define TL_FP32_IR 1        /* FP32 inner CG; FP64 residual, solution and reduction accumulators */
+#define TL_FP32_IR 1        /* FP32 inner CG (FP32 vectors and arithmetic, the cell coefficient stored
+                               as bfloat16); FP64 residual, solution and reduction accumulators */
 
 /* stop rules */
 #define TL_STOP_REFERENCE 0 /* each solver's own rule: CG sqrt|rr| < eps (CG.jl:24); others |err| < eps */

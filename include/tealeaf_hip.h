@@ -40,7 +40,8 @@ extern "C" {
 /* arithmetic of the solve */
 #define TL_FP64 0           /* everything in FP64 (the reference's Float64) */
 #define TL_FP32_IR 1        /* FP32 inner CG (FP32 vectors and arithmetic, the cell coefficient stored
-                               as bfloat16); FP64 residual, solution and reduction accumulators */
+                               in 16 bits: see TL_COEF_*); FP64 residual, solution and reduction
+                               accumulators */
 
 /* stop rules */
 #define TL_STOP_REFERENCE 0 /* each solver's own rule: CG sqrt|rr| < eps (CG.jl:24); others |err| < eps */
@@ -124,9 +125,15 @@ typedef struct {
     double init_rel_resid;/* same for the initial guess that was used */
     double solve_ms;      /* wall time of tl_solve */
     int start;            /* initial guess used: TL_START_* */
-    int reserved_i;
+    int coef_format;      /* TL_FP32_IR: storage of the inner CG's cell coefficient, TL_COEF_*; 0 otherwise */
     double reserved[3];
 } tl_solve_stats;
+
+/* tl_solve_stats.coef_format: half with a power-of-two scale when the coefficient's range
+ * fits half's normal range (max/min <= 2^28), else bfloat16 */
+#define TL_COEF_FP16 1
+#define TL_COEF_BF16 2
+#define TL_COEF_FP32 3
 
 /* tl_solve_stats.start */
 #define TL_START_PREVIOUS 0 /* u = energy*density (the reference's start) */

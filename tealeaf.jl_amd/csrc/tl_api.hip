@@ -154,7 +154,7 @@ extern "C" void tl_destroy(tl_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     unet_destroy(c);
-    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->gcell, c->p32, c->r32, c->x32,
+    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->gcell, c->gc_range, c->p32, c->r32, c->x32,
                     c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
     for (int i = 0; i < TL_NFIELDS; ++i)
         if (c->f[i]) (void)hipFree(c->f[i]);
@@ -282,11 +282,12 @@ static std::vector<uintptr_t> graph_key(const tl_ctx *c, int kind, int k)
     std::memcpy(&hx, &c->hx32, sizeof hx);
     std::memcpy(&hy, &c->hy32, sizeof hy);
     std::vector<uintptr_t> v = {(uintptr_t)kind, (uintptr_t)k, (uintptr_t)c->ctrl, (uintptr_t)c->partials,
-                                c->nb_r[0], c->nb_r[1], c->nb_w};
+                                c->nb_r[0], c->nb_r[1], c->nb_w, (uintptr_t)c->gc_fmt};
     if (kind == GRAPH_CG64) {
         for (const void *q : {(const void *)c->f[TL_F_P], (const void *)c->p_alt, (const void *)c->f[TL_F_R],
                               (const void *)c->f[TL_F_KX], (const void *)c->f[TL_F_KY], (const void *)c->f[TL_F_W],
-                              (const void *)c->f[TL_F_U], (const void *)c->hist_a, (const void *)c->hist_b})
+                              (const void *)(c->cg_u ? c->cg_u : c->f[TL_F_U]), (const void *)c->hist_a,
+                              (const void *)c->hist_b})
             v.push_back((uintptr_t)q);
     } else {
         for (const void *q : {(const void *)c->p32, (const void *)c->p32b, (const void *)c->r32,
@@ -420,9 +421,9 @@ static int ensure_fp32(tl_ctx *c)
 {
     if (c->gcell) return TL_OK;
     const size_t n = c->g.elems;
-    if (hipMalloc((void **)&c->gcell, n * sizeof(tl_gc_t)) != hipSuccess) {
+    if (hipMalloc((void **)&c->gcell, n * sizeof(tl_gc_t)) != hipSuccess ||
+        hipMalloc((void **)&c->gc_range, 2 * TL_RANGE_BLOCKS * sizeof(float)) != hipSuccess) {
         c->err = "device allocation failed";
-        c->gcell = nullptr;
         return TL_E_NOMEM;
     }
     RETIF(alloc_f(c, &c->p32, n));
@@ -604,6 +605,16 @@ static int solve_cg64(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
     const double *guess = nullptr;
     RETIF(choose_guess(c, p, rx, ry, &guess, st));
     HIPCHK(c, launch_cg_init_resid(c, guess));
+    // Under the relative rule the iterations add alpha p to a correction d
+    // (u_alt, free during CG) instead of to u, and d is added to u at every
+    // true-residual check ("reliable updates" with group-wise solution
+    // updates): the rounding of the updates is then relative to the small
+    // correction, not to u, so the true residual can reach tolerances that
+    // CG updating u directly stalls above (measured at 8192^2 from the 69th
+    // timestep of a random-rectangle problem: 2-3e-9 against 1e-10).
+    const bool group = mode == MODE_CG_REL;
+    if (group) HIPCHK(c, hipMemsetAsync(c->u_alt, 0, c->g.elems * sizeof(double), c->stream));
+    c->cg_u = group ? c->u_alt : nullptr;
     Ctrl k;
     int t = 0;
     for (;;) {
@@ -620,11 +631,14 @@ static int solve_cg64(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
         // true one, and if rounding has let the two drift apart, continue CG
         // from the true residual (keeping the search direction)
         t = k.stop_iter;
+        HIPCHK(c, launch_acc64(c, c->f[TL_F_U], c->u_alt));
         HIPCHK(c, launch_cg64_true_residual(c, t));
         RETIF(read_ctrl(c, &k));
         if (k.stop_iter != INT_MAX) break;
         ++st->refreshes;
     }
+    if (group) HIPCHK(c, launch_acc64(c, c->f[TL_F_U], c->u_alt));  // pending updates (max_iters, breakdown)
+    c->cg_u = nullptr;  // other callers of the UR launcher update u
     const int T = k.stop_iter;
     if (T >= 1) {
         // the last iteration's CG.p!, so p ends as the reference leaves it
@@ -864,6 +878,7 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     st->iterations = k.iters;
     st->cg_iters = k.iters;
     st->refreshes = refreshes;
+    st->coef_format = !TL_GC16 ? TL_COEF_FP32 : c->gc_fmt == GC_F16 ? TL_COEF_FP16 : TL_COEF_BF16;
     fill_stats(k, st);
     HIPCHK(c, launch_halo(c, TL_X_U, 1, nullptr, 0));
     return status_of(c, k);

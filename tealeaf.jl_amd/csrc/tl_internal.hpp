@@ -92,19 +92,25 @@ constexpr int TBV = TL_TBV;
 #else
 #define TL_CGG_ATTR
 #endif
-// storage of the FP32 inner CG's cell coefficient gc: bfloat16 (1) or FP32 (0).
-// The operator built from bf16 gc has every face conductivity within a
-// relative 2^-8 of the FP64 one, so it is spectrally equivalent to it within
-// that factor and the FP64 residual refresh still converges to the FP64
-// operator's solution; the kernels read 2 bytes per cell instead of 4.
-#ifndef TL_GC_BF16
-#define TL_GC_BF16 1
+// storage of the FP32 inner CG's cell coefficient gc: 16 bits (1) or FP32
+// (0).  16-bit storage is IEEE half with a power-of-two scale when the
+// coefficient's range fits half's normal range (ratio <= 2^28), else
+// bfloat16.  Rounding a coefficient perturbs each face conductivity by a
+// relative 2^-11 (half) or 2^-8 (bfloat16) at most, so the FP32 operator is
+// spectrally equivalent to the FP64 one within that factor and the FP64
+// residual refresh still converges to the FP64 operator's solution; the
+// kernels read 2 bytes per cell instead of 4.
+#ifndef TL_GC16
+#define TL_GC16 1
 #endif
-#if TL_GC_BF16
-typedef unsigned short tl_gc_t;  // bfloat16 bits
+#if TL_GC16
+typedef unsigned short tl_gc_t;  // half or bfloat16 bits
 #else
 typedef float tl_gc_t;
 #endif
+enum GcFmt : int { GC_BF16 = 0, GC_F16 = 1 };
+// blocks of the coefficient range reduction (k_gc_range)
+#define TL_RANGE_BLOCKS 256
 // non-temporal (streaming) vector loads / stores in those kernels
 #ifndef TL_NT_LOAD
 #define TL_NT_LOAD 0
@@ -179,6 +185,9 @@ struct tl_ctx {
     // FP32 inner CG: cell coefficient gc = 1/w (tl_gc_t) and its face scale
     // (rx/2, ry/2), search direction, residual, correction
     tl::tl_gc_t *gcell = nullptr;
+    int gc_fmt = 0;           // tl::GcFmt of gcell (16-bit storage)
+    float gc_scale = 1.0f;    // gcell holds gc * gc_scale (a power of two)
+    float *gc_range = nullptr;  // per-block min / max of gc (device, 2 * TL_RANGE_BLOCKS)
     float *p32 = nullptr, *r32 = nullptr, *x32 = nullptr;
     float hx32 = 0.0f, hy32 = 0.0f;
     // reductions
@@ -186,6 +195,8 @@ struct tl_ctx {
     size_t partials_len = 0;
     unsigned nb_w = 0, nb_r[2] = {0, 0}, nb_m = 0;  // block counts of the last producer of each partial array
     double *p_alt = nullptr;   // ping-pong twin of p for the fused CG kernels (FP64)
+    double *cg_u = nullptr;    // the vector the fused FP64 UR adds alpha p to: u, or (relative stop
+                               // rule) a correction added to u at every true-residual check
     float *p32b = nullptr;     // same for FP32
     tl::Ctrl *ctrl = nullptr;        // device
     tl::Ctrl *ctrl_h = nullptr;      // pinned host mirror
@@ -266,6 +277,7 @@ hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in,
 hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer);
 hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient);
 hipError_t launch_ir_accumulate(tl_ctx *c, int T);
+hipError_t launch_acc64(tl_ctx *c, double *u, double *d);
 
 Geo make_geo(int nx, int ny, int hd);
 

@@ -1141,23 +1141,37 @@ __global__ __launch_bounds__(TBV) void k_cg_urv(Geo g, int ty, int t, T *__restr
     write_partials<1, TBV>(acc, prcur, g.nbmax);
 }
 
-// cell coefficients as FP32: bf16 -> FP32 is exact (the bits shifted up)
-__device__ __forceinline__ float gc_val(float v) { return v; }
-__device__ __forceinline__ float gc_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-
-// a coefficient held in a 32-bit register as loaded (bf16: zero-extended)
-// and converted only where it is used, so that no instruction between the
-// load and the use waits for it
+// Cell coefficients.  A coefficient is held in a 32-bit register as loaded
+// (16-bit storage: zero-extended) and converted to FP32 only where it is
+// used, so that no instruction between the load and the use waits for it.
+// FMT (GcFmt) selects half or bfloat16 for 16-bit storage; both convert to
+// FP32 exactly.
 __device__ __forceinline__ unsigned gc_raw(float v) { return __float_as_uint(v); }
 __device__ __forceinline__ unsigned gc_raw(unsigned short h) { return (unsigned)h; }
-__device__ __forceinline__ float gc_from_raw(unsigned b)
+
+template <int FMT>
+__device__ __forceinline__ float gc_half(unsigned b)  // one 16-bit value in the low bits of b
 {
-    return __uint_as_float(TL_GC_BF16 ? b << 16 : b);
+    if constexpr (FMT == GC_F16) {
+        return (float)__builtin_bit_cast(_Float16, (unsigned short)(b & 0xffffu));
+    } else {
+        return __uint_as_float(b << 16);
+    }
 }
 
-// V coefficients as loaded (bf16: two per 32-bit word; the vector is aligned
-// like the FP32 vector at the same element index), unpacked where used
-constexpr int GCW = TL_GC_BF16 ? 2 : 1;
+template <int FMT>
+__device__ __forceinline__ float gc_from_raw(unsigned b)
+{
+#if TL_GC16
+    return gc_half<FMT>(b);
+#else
+    return __uint_as_float(b);
+#endif
+}
+
+// V coefficients as loaded (16-bit: two per 32-bit word; the vector is
+// aligned like the FP32 vector at the same element index), unpacked where used
+constexpr int GCW = TL_GC16 ? 2 : 1;
 template <int V>
 struct GcRaw {
     unsigned w[V / GCW];
@@ -1172,14 +1186,19 @@ __device__ __forceinline__ void gc_load_raw(GcRaw<V> &d, const tl_gc_t *__restri
     for (int k = 0; k < V / GCW; ++k) d.w[k] = v[k];
 }
 
-template <int V>
+template <int V, int FMT>
 __device__ __forceinline__ void gc_unpack(float (&d)[V], const GcRaw<V> &s)
 {
-#if TL_GC_BF16
+#if TL_GC16
 #pragma unroll
     for (int k = 0; k < V / 2; ++k) {
-        d[2 * k] = __uint_as_float(s.w[k] << 16);
-        d[2 * k + 1] = __uint_as_float(s.w[k] & 0xffff0000u);
+        if constexpr (FMT == GC_F16) {
+            d[2 * k] = gc_half<FMT>(s.w[k]);
+            d[2 * k + 1] = gc_half<FMT>(s.w[k] >> 16);
+        } else {
+            d[2 * k] = __uint_as_float(s.w[k] << 16);
+            d[2 * k + 1] = __uint_as_float(s.w[k] & 0xffff0000u);
+        }
     }
 #else
 #pragma unroll
@@ -1187,12 +1206,12 @@ __device__ __forceinline__ void gc_unpack(float (&d)[V], const GcRaw<V> &s)
 #endif
 }
 
-template <int V>
+template <int V, int FMT>
 __device__ __forceinline__ void gc_load(float (&d)[V], const tl_gc_t *__restrict__ p)
 {
     GcRaw<V> w;
     gc_load_raw<V>(w, p);
-    gc_unpack<V>(d, w);
+    gc_unpack<V, FMT>(d, w);
 }
 
 // ------------------------------- FP32 inner CG on a cell-coefficient array
@@ -1216,7 +1235,7 @@ __device__ __forceinline__ void gc_load(float (&d)[V], const tl_gc_t *__restrict
 //   partial sums of r^2 (CG.ur!, CG.jl:93-98); the correction x += alpha p
 //   is applied for two iterations at once, at even t, which saves x's read
 //   and write at odd t: UR moves 26 bytes per cell at even t, 14 at odd t.
-template <int V, bool UR, int RB, bool PIPE>
+template <int V, bool UR, int RB, bool PIPE, int FMT>
 __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, const float *__restrict__ pin,
                                             float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
                                             const tl_gc_t *__restrict__ gc, float hx, float hy,
@@ -1261,7 +1280,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
 #if !TL_EDGE_PF
     // p (after this launch's update) at element i
     auto pnew = [&](long i) -> float { return fuse ? r[i] + beta * pp[i] : pp[i]; };
-    auto gcol = [&](long i) -> float { return gc_val(gc[i]); };
+    auto gcol = [&](long i) -> float { return gc_from_raw<FMT>(gc_raw(gc[i])); };
 #endif
     double acc[1] = {0.0};
     if (cl.y0 < g.ny) {
@@ -1270,8 +1289,8 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
             const long i0 = at(g, cl.xl, cl.y0), id = at(g, cl.xl, max(cl.y0 - 1, 0));
             float a[V], b2[V];
             vload<float, V>(a, pp + i0);
-            gc_load<V>(gcur, gc + i0);
-            gc_load<V>(gdn, gc + id);
+            gc_load<V, FMT>(gcur, gc + i0);
+            gc_load<V, FMT>(gdn, gc + id);
             if (fuse) {
                 vload<float, V>(b2, r + i0);
 #pragma unroll
@@ -1295,7 +1314,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
         if (eneed) {
             const long ie = at(g, cl.xl, cl.y0) + eoff;
             ecur_p = fuse ? r[ie] + beta * pp[ie] : pp[ie];
-            ecur_g = gc_val(gc[ie]);
+            ecur_g = gc_from_raw<FMT>(gc_raw(gc[ie]));
         }
 #endif
         // the loads of RB rows, issued together
@@ -1343,7 +1362,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 const int y = yb + q;
                 const long ic = atc(g, cl.xl, y);
                 float u2[V], gu[V];
-                gc_unpack<V>(gu, B.gup[q]);
+                gc_unpack<V, FMT>(gu, B.gup[q]);
 #pragma unroll
                 for (int j = 0; j < V; ++j) u2[j] = fuse ? B.rup[q][j] + beta * B.up[q][j] : B.up[q][j];
                 if (y + 1 >= g.ny) {  // reflection at the top edge: the row itself
@@ -1404,7 +1423,7 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
                 // the row above becomes the current row (at the top edge the
                 // clamped load read the current row itself)
                 ecur_p = fuse ? B.eup_r[q] + beta * B.eup_p[q] : B.eup_p[q];
-                ecur_g = gc_from_raw(B.eup_g[q]);
+                ecur_g = gc_from_raw<FMT>(B.eup_g[q]);
 #endif
             }
         };
@@ -1789,26 +1808,73 @@ __global__ __launch_bounds__(TB) void k_jacobi(Geo g, int t, const double *__res
 
 // cell coefficient of the inner CG over the whole allocation:
 // gc = 1/w with w = density (coefficient 1) or 1/density (CG.jl:56-58),
-// rounded to FP32 and then (bf16 storage) to the nearest bfloat16, ties to
-// even; cells without a density (padding) get 0
+// rounded to FP32, multiplied by scale (a power of two: exact) and rounded to
+// the storage format, to nearest with ties to even; cells without a density
+// (padding) get 0
+template <int FMT>
 __device__ __forceinline__ tl_gc_t gc_store(float f)
 {
-#if TL_GC_BF16
-    unsigned u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (tl_gc_t)(u >> 16);
+#if TL_GC16
+    if constexpr (FMT == GC_F16) {
+        return __builtin_bit_cast(unsigned short, (_Float16)f);
+    } else {
+        unsigned u = __float_as_uint(f);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (tl_gc_t)(u >> 16);
+    }
 #else
     return f;
 #endif
 }
 
+__device__ __forceinline__ float gc_value(double d, int coefficient)
+{
+    return coefficient == 2 ? (float)d : (d != 0.0 ? (float)(1.0 / d) : 0.0f);
+}
+
+template <int FMT>
 __global__ __launch_bounds__(TB) void k_gcell(const double *__restrict__ rho, tl_gc_t *__restrict__ gc, size_t n,
-                                              int coefficient)
+                                              int coefficient, float scale)
 {
     const size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
-    const double d = rho[i];
-    gc[i] = gc_store(coefficient == 2 ? (float)d : (d != 0.0 ? (float)(1.0 / d) : 0.0f));
+    gc[i] = gc_store<FMT>(gc_value(rho[i], coefficient) * scale);
+}
+
+// per-block minimum and maximum of gc over the cells with a density (grid-
+// stride over the allocation; TL_RANGE_BLOCKS blocks): out[2b], out[2b + 1]
+__global__ __launch_bounds__(TB) void k_gc_range(const double *__restrict__ rho, size_t n, int coefficient,
+                                                 float *__restrict__ out)
+{
+    float lo = INFINITY, hi = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * TB + threadIdx.x; i < n; i += (size_t)gridDim.x * TB) {
+        const double d = rho[i];
+        if (d != 0.0) {
+            const float f = gc_value(d, coefficient);
+            lo = fminf(lo, f);
+            hi = fmaxf(hi, f);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    __shared__ float slo[TB / 64], shi[TB / 64];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        slo[wave] = lo;
+        shi[wave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < TB / 64; ++w) {
+            lo = fminf(lo, slo[w]);
+            hi = fmaxf(hi, shi[w]);
+        }
+        out[2 * blockIdx.x] = lo;
+        out[2 * blockIdx.x + 1] = hi;
+    }
 }
 
 // FP64 solution accumulation of the mixed-precision CG after iteration T:
@@ -2003,7 +2069,8 @@ hipError_t launch_cg_urv64(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_UR64);
     k_cg_urv<double, 2, TL_RV_UR64, TL_PIPE_UR64 != 0><<<nb2(c), TBV, 0, c->stream>>>(
-        c->g, c->g.ty2, targ(t, tref), c->f[TL_F_U], p64(c, t), c->f[TL_F_R], c->f[TL_F_W], c->ctrl, part_w(c), c->nb_w,
+        c->g, c->g.ty2, targ(t, tref), c->cg_u ? c->cg_u : c->f[TL_F_U], p64(c, t), c->f[TL_F_R], c->f[TL_F_W], c->ctrl,
+        part_w(c), c->nb_w,
         part_r(c, t), c->hist_a);
     nbr(c, t) = nb2(c);
     if (prof) prof_stop(c);
@@ -2022,7 +2089,9 @@ hipError_t launch_cg_pfinal64(tl_ctx *c, int T)
 hipError_t launch_cg_w32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_W32);
-    k_cgg<4, false, TL_RV_PW32, TL_PIPE_PW32 != 0><<<nb4p(c), TBV, 0, c->stream>>>(
+    auto kern = c->gc_fmt == GC_F16 ? k_cgg<4, false, TL_RV_PW32, TL_PIPE_PW32 != 0, GC_F16>
+                                    : k_cgg<4, false, TL_RV_PW32, TL_PIPE_PW32 != 0, GC_BF16>;
+    kern<<<nb4p(c), TBV, 0, c->stream>>>(
         c->g, c->g.typ4, targ(t, tref), t >= 2 ? p32(c, t - 1) : nullptr, p32(c, t), c->r32, nullptr, c->gcell, c->hx32, c->hy32,
         c->ctrl, nullptr, 0, t >= 2 ? part_r(c, t - 1) : nullptr, t >= 2 ? nbr(c, t - 1) : 0, part_w(c), nullptr);
     c->nb_w = nb4p(c);
@@ -2033,7 +2102,9 @@ hipError_t launch_cg_w32(tl_ctx *c, int t, int tref)
 hipError_t launch_cg_ur32(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CG_UR32);
-    k_cgg<4, true, TL_RV_UR32, TL_PIPE_UR32 != 0><<<nb4(c), TBV, 0, c->stream>>>(
+    auto kern = c->gc_fmt == GC_F16 ? k_cgg<4, true, TL_RV_UR32, TL_PIPE_UR32 != 0, GC_F16>
+                                    : k_cgg<4, true, TL_RV_UR32, TL_PIPE_UR32 != 0, GC_BF16>;
+    kern<<<nb4(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty4, targ(t, tref), nullptr, p32(c, t), c->r32, c->x32, c->gcell, c->hx32, c->hy32, c->ctrl, part_w(c),
         c->nb_w, nullptr, 0, part_r(c, t), t >= 2 ? p32(c, t - 1) : nullptr);
     nbr(c, t) = nb4(c);
@@ -2181,13 +2252,61 @@ hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, in
     return (e == hipSuccess && !defer) ? fin(c, part_m(c), nbh(c), 1, 0, FK_JACOBI, t) : e;
 }
 
+// u += d; d = 0 over the whole allocation (d is zero outside the interior)
+__global__ __launch_bounds__(TB) void k_acc64(double *__restrict__ u, double *__restrict__ d, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * TB + threadIdx.x; i < n; i += (size_t)gridDim.x * TB) {
+        const double v = d[i];
+        u[i] += v;
+        d[i] = 0.0;
+    }
+}
+
+hipError_t launch_acc64(tl_ctx *c, double *u, double *d)
+{
+    const size_t n = c->g.elems;
+    const unsigned nb = (unsigned)std::min<size_t>((n + TB - 1) / TB, 8192);
+    k_acc64<<<nb, TB, 0, c->stream>>>(u, d, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient)
 {
     const size_t n = c->g.elems;
     const unsigned nb = (unsigned)((n + TB - 1) / TB);
-    k_gcell<<<nb, TB, 0, c->stream>>>(c->f[TL_F_DENSITY], c->gcell, n, coefficient);
-    c->hx32 = (float)(0.5 * rx);
-    c->hy32 = (float)(0.5 * ry);
+    int fmt = GC_BF16;
+    float scale = 1.0f;
+#if TL_GC16
+    // half with a power-of-two scale when the coefficient's range fits:
+    // max * scale <= 2^14 and min * scale >= 2^-14 (half's smallest normal)
+    k_gc_range<<<TL_RANGE_BLOCKS, TB, 0, c->stream>>>(c->f[TL_F_DENSITY], n, coefficient, c->gc_range);
+    hipError_t e = hipGetLastError();
+    float h[2 * TL_RANGE_BLOCKS];
+    if (e == hipSuccess) e = hipMemcpyAsync(h, c->gc_range, sizeof h, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    float lo = INFINITY, hi = 0.0f;
+    for (int b = 0; b < TL_RANGE_BLOCKS; ++b) {
+        lo = std::min(lo, h[2 * b]);
+        hi = std::max(hi, h[2 * b + 1]);
+    }
+    if (hi > 0.0f && std::isfinite(hi) && lo > 0.0f) {
+        const float s = std::ldexp(1.0f, (int)std::floor(std::log2(16384.0 / (double)hi)));
+        if ((double)lo * s >= std::ldexp(1.0, -14)) {
+            fmt = GC_F16;
+            scale = s;
+        }
+    }
+#endif
+    if (fmt == GC_F16)
+        k_gcell<GC_F16><<<nb, TB, 0, c->stream>>>(c->f[TL_F_DENSITY], c->gcell, n, coefficient, scale);
+    else
+        k_gcell<GC_BF16><<<nb, TB, 0, c->stream>>>(c->f[TL_F_DENSITY], c->gcell, n, coefficient, 1.0f);
+    c->gc_fmt = fmt;
+    c->gc_scale = scale;
+    // kx = hx (gc(x-1) + gc(x)): the scale of the stored gc is divided out of hx, hy
+    c->hx32 = (float)(0.5 * rx / scale);
+    c->hy32 = (float)(0.5 * ry / scale);
     return hipGetLastError();
 }
 

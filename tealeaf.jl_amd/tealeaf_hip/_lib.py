@@ -19,6 +19,7 @@ TL_FP64, TL_FP32_IR = 0, 1
 TL_STOP_REFERENCE, TL_STOP_RELATIVE = 0, 1
 TL_WARM_NONE, TL_WARM_LEARNED, TL_WARM_GUESS = 0, 1, 2
 TL_LAUNCH_STREAM, TL_LAUNCH_GRAPH = 0, 1
+TL_COEF_FP16, TL_COEF_BF16, TL_COEF_FP32 = 1, 2, 3
 FIELDS = ["density0", "density", "energy0", "energy", "u", "u0", "p", "r", "mi", "w", "kx", "ky", "sd"]
 FID = {n: i for i, n in enumerate(FIELDS)}
 X_DENSITY, X_P, X_ENERGY0, X_ENERGY, X_U, X_SD = 1, 2, 4, 8, 16, 32
@@ -56,7 +57,7 @@ class SolveStats(C.Structure):
         ("iterations", C.c_int), ("cg_iters", C.c_int), ("cheby_iters", C.c_int), ("est_iters", C.c_int),
         ("inner_iters", C.c_int), ("refreshes", C.c_int), ("converged", C.c_int), ("fallback", C.c_int),
         ("error", C.c_double), ("eigmin", C.c_double), ("eigmax", C.c_double), ("rel_resid", C.c_double),
-        ("init_rel_resid", C.c_double), ("solve_ms", C.c_double), ("start", C.c_int), ("reserved_i", C.c_int),
+        ("init_rel_resid", C.c_double), ("solve_ms", C.c_double), ("start", C.c_int), ("coef_format", C.c_int),
         ("reserved", C.c_double * 3),
     ]
 

@@ -290,3 +290,40 @@ def test_debugrecord_members_match_oracle(tmp_path, solver):
             got, ref = member(name)[0], o.history(which, n)[1:n + 1]
             assert np.allclose(got[:n], ref, rtol=1e-8, atol=0), name
             assert np.all(got[n:] == 0.0)
+
+
+@pytest.mark.parametrize("densities,fmt", [((100.0, 0.1), 1), ((1e5, 1e-5), 2)])
+def test_fp32_ir_coefficient_format(densities, fmt):
+    """The inner CG stores its cell coefficient as half with a power-of-two
+    scale when the coefficient's range fits half's normal range, as bfloat16
+    when it does not (here 1e10 > 2^28); either way the solve meets the FP64
+    tolerance and the summaries agree with the oracle."""
+    print(f"case: FP32-IR 96^2 reference deck, densities {densities}", flush=True)
+    text = decks.reference_deck(96, 2, extra="stop_rule=relative\nrel_tol=1e-10\nprecision=fp32_ir\n")
+    text = text.replace("density=100.0", f"density={densities[0]!r}").replace("density=0.1",
+                                                                            f"density={densities[1]!r}")
+    o, c, s = _pair(text, stop_rule=O.STOP_RELATIVE, rel_tol=1e-12, maxiters=100000)
+    osum, ost = o.run(2)
+    gsum, gst = T.diffuse(c, s, 2)
+    for t in range(2):
+        assert gst[t]["coef_format"] == fmt, gst[t]
+        assert gst[t]["rel_resid"] <= 1e-10, gst[t]
+        assert _rel(gsum[t], osum[t]) < 1e-9, (t, gsum[t], osum[t])
+
+
+def test_fp64_relative_rule_beyond_plain_cg_accuracy():
+    """FP64 CG under the relative rule adds its updates to a correction that
+    joins u at every true-residual check.  On this problem (256^2, rx = 2687
+    as at 8192^2, tolerance 3e-11) CG updating u directly stalls with a true
+    residual of about 6e-10 (measured with a CPU model: 60,000 iterations);
+    the library must meet the tolerance on the true residual."""
+    print("case: FP64 CG 256^2 random rectangles, dt 4.1, rel_tol 3e-11", flush=True)
+    text = (decks.random_rect_deck(1001, n=256, steps=1, rel_tol=3e-11, max_iters=60000)
+            .replace("initial_timestep=0.004", "initial_timestep=4.1"))
+    s = T.Settings.from_text(text)
+    c = T.initialiseapp(s)
+    st = T.step(c, s)
+    print(f"  {st['iterations']} iterations, {st['refreshes']} true-residual restarts, "
+          f"rel. residual {st['rel_resid']:.3e}", flush=True)
+    assert st["converged"] == 1 and st["rel_resid"] <= 3e-11, st
+    assert st["iterations"] < 60000

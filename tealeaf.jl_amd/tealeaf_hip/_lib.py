@@ -38,8 +38,8 @@ EXPORTS = [
 TL_START_PREVIOUS, TL_START_WARM = 0, 1
 TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32 = range(6)
 KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, false>", 2: "k_cg_p<double>",
-                3: "k_cgg<4, false, 2, true>", 4: "k_cgg<4, true, 4, true>",
-                5: "(fused into k_cgg<4, false, 2, true>)",
+                3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
+                5: "(fused into k_cgg<4, false, 2, true, 1>)",
                 -1: "k_copy_full", -2: "k_copy_vec"}
 
 

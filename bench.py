@@ -123,7 +123,7 @@ def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, la
     return {"times": times, "total": t_all1 - t_all0, "stats": stats, "summaries": summaries, "prof": prof}
 
 
-def roofline_from(prof, n):
+def roofline_from(prof, n, coef_format=None):
     """Dominant kernel by total time; average over launches that ran (launches
     made after a solve's stop flag exit at once and are excluded)."""
     best = None
@@ -143,8 +143,8 @@ def roofline_from(prof, n):
     achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(_lib.KERNEL_NAMES[kid], n),
-        "kernel": _lib.KERNEL_NAMES[kid], "avg_ms": round(avg_ms, 5), "timed_launches": int(len(live)),
+        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(_lib.kernel_name(kid, coef_format), n),
+        "kernel": _lib.kernel_name(kid, coef_format), "avg_ms": round(avg_ms, 5), "timed_launches": int(len(live)),
         "sampling": f"1 launch in {PROFILE_PERIOD}",
         "stopped_launches_excluded": int(skipped), "algorithmic_bytes_per_launch": algo_bytes,
         "bytes_per_cell": KERNEL_BYTES[kid],
@@ -340,7 +340,8 @@ def main():
         out["field_summary_max_rel_diff_vs_fp64"] = max(diffs)
     if a.only == "flagship":
         out["flagship"]["summaries"] = [list(map(float, x)) for x in flag["summaries"]]
-    out["roofline"] = roofline_from(flag["prof"], a.n)
+    fmt = flag["stats"][0].get("coef_format") if flag["stats"] else None
+    out["roofline"] = roofline_from(flag["prof"], a.n, fmt)
     # every profiled kernel of the flagship's timed steps, and the share of the
     # timed wall time they cover (launches past a solve's stop excluded; the
     # totals scale the sampled launches by the sampling period)
@@ -349,7 +350,7 @@ def main():
         if len(d) == 0:
             continue
         live = d[d > 0.25 * float(np.median(d))]
-        kern[_lib.KERNEL_NAMES[kid]] = {"timed_launches": int(len(live)), "avg_ms": round(float(live.mean()), 5),
+        kern[_lib.kernel_name(kid, fmt)] = {"timed_launches": int(len(live)), "avg_ms": round(float(live.mean()), 5),
                                          "total_s_est": round(float(live.sum()) * 1e-3 * PROFILE_PERIOD, 4),
                                          "GBps": round(KERNEL_BYTES[kid] * a.n * a.n / (float(live.mean()) * 1e-3) / 1e9, 1)}
     out["flagship_kernels"] = kern

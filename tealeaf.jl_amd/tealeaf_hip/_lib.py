@@ -43,6 +43,17 @@ KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, fa
                 -1: "k_copy_full", -2: "k_copy_vec"}
 
 
+def kernel_name(kid, coef_format=None):
+    """Symbol of profiled kernel `kid`.  The FP32 kernels' last template
+    argument is the cell coefficient's 16-bit format (1 = scaled half,
+    0 = bfloat16), which the solver picks per solve (`SolveStats.coef_format`);
+    without a format the half instantiation is named."""
+    name = KERNEL_NAMES[kid]
+    if coef_format == TL_COEF_BF16 and kid in (TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32):
+        name = name.replace(", 1>", ", 0>")
+    return name
+
+
 class SolveParams(C.Structure):
     _fields_ = [
         ("solver", C.c_int), ("coefficient", C.c_int), ("max_iters", C.c_int), ("presteps", C.c_int),

@@ -116,3 +116,12 @@ def test_julia_float_format():
         assert j(x) == want, (x, j(x), want)
     for x in np.random.default_rng(0).standard_normal(200) * 10.0 ** np.random.default_rng(1).integers(-8, 9, 200):
         assert float(j(float(x))) == x  # round trip
+
+
+def test_kernel_names_follow_coefficient_format():
+    # the FP32 kernels' last template argument is the 16-bit coefficient format
+    assert _lib.kernel_name(_lib.TL_K_CG_UR32) == "k_cgg<4, true, 4, true, 1>"
+    assert _lib.kernel_name(_lib.TL_K_CG_UR32, _lib.TL_COEF_FP16) == "k_cgg<4, true, 4, true, 1>"
+    assert _lib.kernel_name(_lib.TL_K_CG_UR32, _lib.TL_COEF_BF16) == "k_cgg<4, true, 4, true, 0>"
+    assert _lib.kernel_name(_lib.TL_K_CG_W32, _lib.TL_COEF_BF16) == "k_cgg<4, false, 2, true, 0>"
+    assert _lib.kernel_name(_lib.TL_K_CG_UR64, _lib.TL_COEF_BF16) == _lib.KERNEL_NAMES[_lib.TL_K_CG_UR64]

@@ -76,9 +76,11 @@ def make_settings(n, seed, precision, warm):
     return T.Settings.from_text(text)
 
 
-def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, launch="stream"):
+def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, launch="stream", ir_delta=None):
     s = make_settings(n, seed, precision, warm)
     s.set(launch={"stream": T.TL_LAUNCH_STREAM, "graph": T.TL_LAUNCH_GRAPH}[launch])
+    if ir_delta is not None:
+        s.set(ir_delta=ir_delta)
     chunk = T.initialiseapp(s)
     if warm == "learned":
         from tealeaf_hip import unet
@@ -120,7 +122,8 @@ def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, la
                 prof[kid] = buf
         L.tl_profile_end(chunk._ctx)
     chunk.close()
-    return {"times": times, "total": t_all1 - t_all0, "stats": stats, "summaries": summaries, "prof": prof}
+    return {"times": times, "total": t_all1 - t_all0, "stats": stats, "summaries": summaries, "prof": prof,
+            "ir_delta": s.ir_delta}
 
 
 def roofline_from(prof, n, coef_format=None):
@@ -271,6 +274,8 @@ def main():
                     help="run one method only (long runs); per-step field summaries are in the JSON")
     ap.add_argument("--launch", choices=("stream", "graph"), default="stream",
                     help="CG iterations as stream launches or hipGraph replays (graph: no per-kernel timing)")
+    ap.add_argument("--ir-delta", type=float, default=None,
+                    help="FP32-refined CG: refresh the FP64 residual when the FP32 one falls by this factor")
     ap.add_argument("--solvers", action="store_true",
                     help="CG, Chebyshev and PPCG on the 3-state reference deck at --n (default 4096 here)")
     a = ap.parse_args()
@@ -300,7 +305,7 @@ def main():
         return
     mask = (1 << _lib.TL_K_CG_W32) | (1 << _lib.TL_K_CG_UR32) | (1 << _lib.TL_K_CG_P32)
     flag = run_method("fp32-ir" + ("+learned" if learned else ""), a.n, a.seed, "fp32_ir", warm, a.steps,
-                      a.warmup, mask if a.launch == "stream" else 0, a.launch)
+                      a.warmup, mask if a.launch == "stream" else 0, a.launch, a.ir_delta)
     base = None
     if not a.no_baseline and a.only is None:
         base = run_method("fp64-cg", a.n, a.seed, "fp64", "none", a.steps, a.warmup, launch=a.launch)
@@ -320,6 +325,7 @@ def main():
             "p90": round(float(np.percentile(flag["times"], 90)), 4),
             "iterations": [st["iterations"] for st in flag["stats"]],
             "refreshes": [st["refreshes"] for st in flag["stats"]],
+            "ir_delta": flag["ir_delta"],
             "fallback": [st["fallback"] for st in flag["stats"]],
             "start": [["previous", "learned"][st["start"]] for st in flag["stats"]],
             "init_rel_resid": [st["init_rel_resid"] for st in flag["stats"]],

@@ -169,6 +169,19 @@ __device__ __forceinline__ T smvp5(T c, T l, T r, T d, T u, T kxl, T kxr, T kyd,
     return (consum * c - (kxr * r + kxl * l)) - (kyu * u + kyd * d);
 }
 
+// FP32 (inner CG of the mixed-precision solver): the same operator in flux
+// form, c + sum k (c - nbr).  The differences of neighbouring values are
+// formed before the face coefficients scale them, so the rounding error of
+// A p is relative to the differences rather than to (1 + sum k) |p|, which
+// for smooth p (the slowest CG modes here, where sum k reaches 1e6) is the
+// size of A p itself in the diagonal form.
+template <>
+__device__ __forceinline__ float smvp5<float>(float c, float l, float r, float d, float u, float kxl, float kxr,
+                                              float kyd, float kyu)
+{
+    return c + ((kxr * (c - r) + kxl * (c - l)) + (kyu * (c - u) + kyd * (c - d)));
+}
+
 // One thread owns one column of a tile: rows [y0, y1) of column x.  Rows are
 // processed in chunks of R whose loads are all issued before any arithmetic,
 // so every wave keeps R rows of loads in flight.  Row indices are clamped

@@ -42,7 +42,8 @@ PROFILE_PERIOD = 15  # odd: both parities of the alternating FP32 UR kernel are 
 # A CG iteration is two kernels.
 #   FP64:  PW = [p = r + beta p; w = A p] reads p, r, Kx, Ky, writes p', w  (48 B)
 #          UR = [u += alpha p; r -= alpha w] reads u, p, r, w, writes u, r   (48 B)
-#   FP32 inner CG (operator from one bf16 cell-coefficient array gc, w not stored):
+#   FP32 inner CG (operator from one 16-bit cell-coefficient array gc: half with a
+#   power-of-two scale, or bfloat16 when the range does not fit; w not stored):
 #          PW reads p, r (4 B each), gc (2 B), writes p'                    (14 B)
 #          UR = [r -= alpha A p] reads p, r, gc, writes r                   (14 B)
 #          and at even iterations also x += alpha(t-1) p(t-1) + alpha(t) p(t):

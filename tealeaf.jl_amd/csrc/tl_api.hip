@@ -411,25 +411,30 @@ static int ensure_history(tl_ctx *c, int n)
     if (c->hist_a) (void)hipFree(c->hist_a);
     if (c->hist_b) (void)hipFree(c->hist_b);
     c->hist_a = c->hist_b = nullptr;
+    c->hist_len = 0;  // nothing held until both allocations succeed
     RETIF(alloc_d(c, &c->hist_a, (size_t)n + 2));
     RETIF(alloc_d(c, &c->hist_b, (size_t)n + 2));
     c->hist_len = n + 2;
     return TL_OK;
 }
 
+// The mixed-precision solver's arrays, allocated on its first use.  Each one
+// is allocated only while missing, so a call after a failed allocation
+// retries the missing ones instead of running with a null array.
 static int ensure_fp32(tl_ctx *c)
 {
-    if (c->gcell) return TL_OK;
     const size_t n = c->g.elems;
-    if (hipMalloc((void **)&c->gcell, n * sizeof(tl_gc_t)) != hipSuccess ||
-        hipMalloc((void **)&c->gc_range, 2 * TL_RANGE_BLOCKS * sizeof(float)) != hipSuccess) {
+    if (!c->gcell && hipMalloc((void **)&c->gcell, n * sizeof(tl_gc_t)) != hipSuccess) c->gcell = nullptr;
+    if (!c->gc_range && hipMalloc((void **)&c->gc_range, 2 * TL_RANGE_BLOCKS * sizeof(float)) != hipSuccess)
+        c->gc_range = nullptr;
+    if (!c->gcell || !c->gc_range) {
         c->err = "device allocation failed";
         return TL_E_NOMEM;
     }
-    RETIF(alloc_f(c, &c->p32, n));
-    RETIF(alloc_f(c, &c->p32b, n));
-    RETIF(alloc_f(c, &c->r32, n));
-    RETIF(alloc_f(c, &c->x32, n));
+    if (!c->p32) RETIF(alloc_f(c, &c->p32, n));
+    if (!c->p32b) RETIF(alloc_f(c, &c->p32b, n));
+    if (!c->r32) RETIF(alloc_f(c, &c->r32, n));
+    if (!c->x32) RETIF(alloc_f(c, &c->x32, n));
     return TL_OK;
 }
 

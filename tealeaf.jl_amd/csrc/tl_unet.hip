@@ -625,24 +625,34 @@ static int ualloc(tl_ctx *c, void **p, size_t bytes)
     return TL_OK;
 }
 
+void unet_destroy(tl_ctx *c);
+
+static int unet_alloc(tl_ctx *c, Unet *u)
+{
+    for (int l = 0; l < NL; ++l) {
+        const size_t nw = wcount(l), nb = LAY[l].coutp;
+        float **fw[] = {&u->w[l], &u->mw[l], &u->vw[l], &u->gw[l]};
+        for (float **q : fw)
+            if (int rc = ualloc(c, (void **)q, nw * sizeof(float))) return rc;
+        float **fb[] = {&u->b[l], &u->mb[l], &u->vb[l], &u->gb[l]};
+        for (float **q : fb)
+            if (int rc = ualloc(c, (void **)q, nb * sizeof(float))) return rc;
+        if (int rc = ualloc(c, (void **)&u->wb[l], nw * sizeof(bf16))) return rc;
+        if (int rc = ualloc(c, (void **)&u->wt[l], nw * sizeof(bf16))) return rc;
+    }
+    if (int rc = ualloc(c, (void **)&u->scale, 4 * sizeof(float))) return rc;
+    return ualloc(c, (void **)&u->mx_part, 3 * 1024 * sizeof(float));
+}
+
+// the context's network, created on first use; a failed creation leaves none
 static int unet_get(tl_ctx *c, Unet **out)
 {
     if (!c->unet) {
-        Unet *u = new Unet();
-        c->unet = u;
-        for (int l = 0; l < NL; ++l) {
-            const size_t nw = wcount(l), nb = LAY[l].coutp;
-            float **fw[] = {&u->w[l], &u->mw[l], &u->vw[l], &u->gw[l]};
-            for (float **q : fw)
-                if (int rc = ualloc(c, (void **)q, nw * sizeof(float))) return rc;
-            float **fb[] = {&u->b[l], &u->mb[l], &u->vb[l], &u->gb[l]};
-            for (float **q : fb)
-                if (int rc = ualloc(c, (void **)q, nb * sizeof(float))) return rc;
-            if (int rc = ualloc(c, (void **)&u->wb[l], nw * sizeof(bf16))) return rc;
-            if (int rc = ualloc(c, (void **)&u->wt[l], nw * sizeof(bf16))) return rc;
+        c->unet = new Unet();
+        if (int rc = unet_alloc(c, c->unet)) {
+            unet_destroy(c);
+            return rc;
         }
-        if (int rc = ualloc(c, (void **)&u->scale, 4 * sizeof(float))) return rc;
-        if (int rc = ualloc(c, (void **)&u->mx_part, 3 * 1024 * sizeof(float))) return rc;
     }
     *out = c->unet;
     return TL_OK;
@@ -692,7 +702,10 @@ static int acts_ensure(tl_ctx *c, Unet *u, int B, int H, int W, bool grads)
              {&a.e3b, H4, W4, 64}, {&a.d2a, H2, W2, 32}, {&a.d2b, H2, W2, 32}, {&a.d1a, H, W, 16},
              {&a.d1b, H, W, 16},   {&a.out, H, W, 16}};
     for (auto &q : f)
-        if (int rc = talloc(c, *q.t, B, q.h, q.w, q.ch)) return rc;
+        if (int rc = talloc(c, *q.t, B, q.h, q.w, q.ch)) {
+            acts_free(a);
+            return rc;
+        }
     if (grads) {
         struct {
             Tensor *t;
@@ -702,7 +715,10 @@ static int acts_ensure(tl_ctx *c, Unet *u, int B, int H, int W, bool grads)
                  {&a.g_e3a, H4, W4, 64}, {&a.g_p2, H4, W4, 32},  {&a.g_e2a, H2, W2, 32}, {&a.g_p1, H2, W2, 16},
                  {&a.g_e1a, H, W, 16},   {&a.g_e1b, H, W, 16},   {&a.g_e2b, H2, W2, 32}, {&a.dz, H, W, 64}};
         for (auto &q : g)
-            if (int rc = talloc(c, *q.t, B, q.h, q.w, q.ch)) return rc;
+            if (int rc = talloc(c, *q.t, B, q.h, q.w, q.ch)) {
+                acts_free(a);
+                return rc;
+            }
     }
     return TL_OK;
 }
@@ -784,6 +800,8 @@ static int back_layer(tl_ctx *c, Unet *u, int l, const TV &g, const TV &gp, int 
     const size_t need = (size_t)nchunk * 9 * npair;
     if (need > u->dw_part_len) {
         if (u->dw_part) (void)hipFree(u->dw_part);
+        u->dw_part = nullptr;
+        u->dw_part_len = 0;
         if (int rc = ualloc(c, (void **)&u->dw_part, need * sizeof(float))) return rc;
         u->dw_part_len = need;
     }
@@ -1040,10 +1058,19 @@ extern "C" int tl_unet_record(tl_ctx *c, int capacity, int crops_per_step)
         u->ds_cap = capacity;
         const size_t px = (size_t)u->ds_crop * u->ds_crop;
         const size_t xb = (size_t)u->ds_cap * (u->ds_crop + 2) * (u->ds_crop + 2) * 16 * sizeof(bf16);
-        if (int rc = ualloc(c, (void **)&u->ds_x, xb)) return rc;
-        if (int rc = ualloc(c, (void **)&u->ds_y, (size_t)u->ds_cap * px * sizeof(float))) return rc;
-        if (int rc = ualloc(c, (void **)&u->ds_m, (size_t)u->ds_cap * px * sizeof(float))) return rc;
-        if (int rc = ualloc(c, (void **)&u->ds_k, (size_t)u->ds_cap * 2 * px * sizeof(float))) return rc;
+        int rc = ualloc(c, (void **)&u->ds_x, xb);
+        if (rc == TL_OK) rc = ualloc(c, (void **)&u->ds_y, (size_t)u->ds_cap * px * sizeof(float));
+        if (rc == TL_OK) rc = ualloc(c, (void **)&u->ds_m, (size_t)u->ds_cap * px * sizeof(float));
+        if (rc == TL_OK) rc = ualloc(c, (void **)&u->ds_k, (size_t)u->ds_cap * 2 * px * sizeof(float));
+        if (rc != TL_OK) {  // no partial dataset: the next call allocates it again
+            void **ds[] = {(void **)&u->ds_x, (void **)&u->ds_y, (void **)&u->ds_m, (void **)&u->ds_k};
+            for (void **q : ds) {
+                if (*q) (void)hipFree(*q);
+                *q = nullptr;
+            }
+            u->ds_cap = 0;
+            return rc;
+        }
         u->ds_edges.assign(u->ds_cap, 0);
     }
     c->recording = capacity > 0;
@@ -1070,9 +1097,13 @@ extern "C" int tl_unet_train(tl_ctx *c, const tl_train_params *tp, float *loss_o
     const size_t px = (size_t)crop * crop;
     if (int rc = acts_ensure(c, u, B, crop, crop, true)) return rc;
     if (u->bt_cap < B) {
-        void *old[] = {u->bt_y, u->bt_m, u->bt_k, u->bt_e, u->bt_r, u->bt_edges};
-        for (void *q : old)
-            if (q) (void)hipFree(q);
+        void **old[] = {(void **)&u->bt_y, (void **)&u->bt_m, (void **)&u->bt_k,
+                        (void **)&u->bt_e, (void **)&u->bt_r, (void **)&u->bt_edges};
+        for (void **q : old) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
+        u->bt_cap = 0;
         if (int rc = ualloc(c, (void **)&u->bt_y, B * px * sizeof(float))) return rc;
         if (int rc = ualloc(c, (void **)&u->bt_m, B * px * sizeof(float))) return rc;
         if (int rc = ualloc(c, (void **)&u->bt_k, 2 * B * px * sizeof(float))) return rc;
@@ -1087,6 +1118,8 @@ extern "C" int tl_unet_train(tl_ctx *c, const tl_train_params *tp, float *loss_o
     const unsigned nlb = blocks(npx);
     if ((size_t)nlb > u->loss_part_len) {
         if (u->loss_part) (void)hipFree(u->loss_part);
+        u->loss_part = nullptr;
+        u->loss_part_len = 0;
         if (int rc = ualloc(c, (void **)&u->loss_part, nlb * sizeof(float))) return rc;
         u->loss_part_len = nlb;
     }
@@ -1202,11 +1235,23 @@ extern "C" int tl_unet_eval(tl_ctx *c, const float *x, const float *target, int 
                 if (out) out[((size_t)b * H + y) * W + xx] = f;
             }
     if (!want_grad) return TL_OK;
-    float *dy, *dm, *lp;
+    // scratch for the loss, freed on every return path
+    struct Scratch {
+        float *dy = nullptr, *dm = nullptr, *lp = nullptr;
+        ~Scratch()
+        {
+            for (float *q : {dy, dm, lp})
+                if (q) (void)hipFree(q);
+        }
+    } sc;
     const unsigned nlb = blocks((long)npx);
-    if (hipMalloc((void **)&dy, npx * sizeof(float)) != hipSuccess) return TL_E_NOMEM;
-    if (hipMalloc((void **)&dm, npx * sizeof(float)) != hipSuccess) return TL_E_NOMEM;
-    if (hipMalloc((void **)&lp, nlb * sizeof(float)) != hipSuccess) return TL_E_NOMEM;
+    if (hipMalloc((void **)&sc.dy, npx * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&sc.dm, npx * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&sc.lp, nlb * sizeof(float)) != hipSuccess) {
+        c->err = "unet: device allocation failed";
+        return TL_E_NOMEM;
+    }
+    float *dy = sc.dy, *dm = sc.dm, *lp = sc.lp;
     std::vector<float> ones(npx, 1.f), lph(nlb);
     UCHK(c, hipMemcpyAsync(dy, target, npx * sizeof(float), hipMemcpyHostToDevice, c->stream));
     UCHK(c, hipMemcpyAsync(dm, ones.data(), npx * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -1231,8 +1276,5 @@ extern "C" int tl_unet_eval(tl_ctx *c, const float *x, const float *target, int 
                 for (int ci = 0; ci < L.cin; ++ci) grads[o++] = w[((size_t)t * L.coutp + co) * L.cinp + ci];
         for (int co = 0; co < L.cout; ++co) grads[o++] = b[co];
     }
-    (void)hipFree(dy);
-    (void)hipFree(dm);
-    (void)hipFree(lp);
     return TL_OK;
 }

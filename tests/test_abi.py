@@ -67,6 +67,18 @@ def test_parser_build_keys_and_warnings():
         T.Settings.from_text("x_cells=4\n")  # no state 1
 
 
+def test_parser_mixed_precision_keys():
+    base = decks.random_rect_deck(1, n=64, steps=1)
+    s = T.Settings.from_text(base)
+    assert s.precision == T.TL_FP64 and s.ir_delta == 0.1  # defaults
+    s = T.Settings.from_text(base + "precision = fp32_ir\nir_delta=0.3\nrel_tol=1e-9\nstop_rule=relative\n")
+    assert s.precision == T.TL_FP32_IR and s.ir_delta == 0.3 and s.rel_tol == 1e-9
+    assert s.stop_rule == T.TL_STOP_RELATIVE
+    n0 = T.Settings.from_text(base).n_warnings
+    for bad in ("precision=fp16\n", "ir_delta=abc\n", "stop_rule=sometimes\n", "warm_start=maybe\n"):
+        assert T.Settings.from_text(base + bad).n_warnings == n0 + 1, bad  # warned and ignored
+
+
 def test_eigen_bounds_match_scipy_and_oracle():
     rng = np.random.default_rng(2)
     L = T.load()

@@ -102,7 +102,8 @@ typedef struct {
     double rel_tol;       /* for TL_STOP_RELATIVE */
     int warm_start;       /* TL_WARM_* */
     int check_result;     /* Settings.checkresult: residual! in tl_solve_finished */
-    double ir_delta;      /* TL_FP32_IR: refresh the FP64 residual when the FP32 one falls by this factor */
+    double ir_delta;      /* TL_FP32_IR: refresh the FP64 residual when the FP32 one falls by this factor
+                             (0 < ir_delta < 1; any other value selects the default 0.1) */
     int launch;           /* TL_LAUNCH_* */
     int reserved[7];
 } tl_solve_params;

@@ -137,3 +137,16 @@ def test_kernel_names_follow_coefficient_format():
     assert _lib.kernel_name(_lib.TL_K_CG_UR32, _lib.TL_COEF_BF16) == "k_cgg<4, true, 4, true, 0>"
     assert _lib.kernel_name(_lib.TL_K_CG_W32, _lib.TL_COEF_BF16) == "k_cgg<4, false, 2, true, 0>"
     assert _lib.kernel_name(_lib.TL_K_CG_UR64, _lib.TL_COEF_BF16) == _lib.KERNEL_NAMES[_lib.TL_K_CG_UR64]
+
+
+def test_benchmark_problem_is_fixed():
+    # SplitMix64 known-answer vector (seed 0) and the benchmark deck itself:
+    # a change of the generator or of the deck text changes the measured problem
+    import hashlib
+    g = decks.SplitMix64(0)
+    assert [g.next() for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    text = decks.random_rect_deck(1001, n=8192, steps=1000, precision="fp32_ir", warm_start="learned")
+    assert hashlib.sha256(text.encode()).hexdigest() == \
+        "3a117da74cbbfd7a139452e2efe5f47c023044df222040def0fa82fae7366f86"
+    s = T.Settings.from_text(text)
+    assert (s.xcells, s.ycells) == (8192, 8192) and s.precision == T.TL_FP32_IR

@@ -32,7 +32,7 @@ import time
 
 import numpy as np
 
-from . import Settings, initialiseapp, step, fieldsummary, TL_WARM_LEARNED, TL_WARM_NONE
+from . import Settings, initialiseapp, step, fieldsummary
 from . import decks, unet
 
 

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Iterable, Optional
+from typing import Optional
 
 import numpy as np
 

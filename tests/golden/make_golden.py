@@ -34,7 +34,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tealeaf.jl_amd")]
 
-import numpy as np  # noqa: E402
 
 import oracle as O  # noqa: E402
 from tealeaf_hip import decks  # noqa: E402

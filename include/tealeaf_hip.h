@@ -30,6 +30,8 @@ extern "C" {
 #define TL_E_PARSE 6        /* malformed deck */
 #define TL_E_STATE 7        /* call out of order (e.g. a learned warm start without a network) */
 #define TL_E_IO 8           /* file could not be read (checkingvalue's open, settings.jl:181) */
+#define TL_E_CHECK 9        /* the solve claimed convergence but the FP64 residual of the returned u,
+                               recomputed by separate kernels, does not meet the rule (tl_check_solution) */
 
 /* solvers, settings.jl:103-111 */
 #define TL_CG 0
@@ -221,6 +223,23 @@ int tl_eigen_bounds(const double *alpha, const double *beta, int n, double *eigm
 void tl_cheby_coef(double eigmin, double eigmax, int n, double *alpha, double *beta, double *theta);
 /* Cheby.calciter, Cheby.jl:109-118 */
 int tl_cheby_calciter(double eigmin, double eigmax, double error, double bb);
+/* The check tl_solve applies to every solve before it returns (no counterpart
+ * in the reference, which trusts its recursive residual, CG.jl:24).  Inputs:
+ * the solve's parameters and statistics (st->iterations, st->converged), bb =
+ * sum u0^2 as the solve itself reduced it (negative: the solver does not form
+ * it, Jacobi), and two FP64 sums recomputed from the returned u by separate
+ * kernels, u0sq = sum u0^2 and resid2 = sum (u0 - A u)^2 over the interior.
+ * Returns TL_OK when the claim holds, else TL_E_CHECK:
+ *   - bb >= 0 and u0sq disagree by more than 1e-9 relative (one of the two
+ *     reductions did not see its inputs);
+ *   - a CG solve (either precision) under TL_STOP_RELATIVE claims
+ *     convergence but sqrt(resid2) > rel_tol sqrt(u0sq) (1 + 1e-6; the slack
+ *     covers the summation order of the two residual kernels);
+ *   - a CG-family solve (CG, or the CG phase of Chebyshev / PPCG) claims an
+ *     exact start (converged after 0 iterations) while the recomputed
+ *     residual misses the rule's threshold (sqrt(resid2) < eps for the
+ *     reference rule, as above for the relative one). */
+int tl_check_solution(const tl_solve_params *p, const tl_solve_stats *st, double bb, double u0sq, double resid2);
 
 /* ------------------------------------------------- device-side diagnostics */
 /* w = A*field over the interior (smvp, kernels.jl:178-184), result left in TL_F_W */

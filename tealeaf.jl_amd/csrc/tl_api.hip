@@ -538,14 +538,47 @@ extern "C" int tl_cheby_calciter(double eigmin, double eigmax, double error, dou
 
 // --------------------------------------------------------------- solvers
 
-static int finish_stats(tl_ctx *c, tl_solve_stats *st)
+extern "C" int tl_check_solution(const tl_solve_params *p, const tl_solve_stats *st, double bb, double u0sq,
+                                 double resid2)
 {
-    // true relative residual of the returned u (FP64)
+    if (!p || !st) return TL_E_INVALID;
+    if (!(u0sq >= 0.0) || !(resid2 >= 0.0)) return TL_E_CHECK;  // negative or NaN sums
+    // the solve's own sum u0^2 and the recomputed one come from the same field
+    if (bb >= 0.0 && std::fabs(bb - u0sq) > 1e-9 * std::max(bb, u0sq)) return TL_E_CHECK;
+    const bool cg_family = p->solver == TL_CG || p->solver == TL_CHEBY || p->solver == TL_PPCG;
+    const bool relative = p->stop_rule == TL_STOP_RELATIVE && (p->solver == TL_CG || p->precision == TL_FP32_IR);
+    const double rel_ok = p->rel_tol * std::sqrt(u0sq) * (1.0 + 1e-6);
+    if (st->converged && relative && std::sqrt(resid2) > rel_ok) return TL_E_CHECK;
+    if (st->converged && cg_family && st->iterations == 0) {
+        // an exact start: the residual the solve found to be zero must be so
+        const double r = std::sqrt(resid2);
+        if (!(relative ? r <= rel_ok : r < p->eps)) return TL_E_CHECK;
+    }
+    return TL_OK;
+}
+
+// FP64 relative residual of the returned u, recomputed by separate kernels,
+// and the check of the solve's convergence claim against it
+static int finish_stats(tl_ctx *c, const tl_solve_params *p, tl_solve_stats *st)
+{
+    Ctrl k0;
+    RETIF(read_ctrl(c, &k0));  // the solve's own scalars (bb = sum u0^2 of CG-init)
+    const double bb = p->solver == TL_JACOBI && p->precision != TL_FP32_IR ? -1.0 : k0.bb;
     HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U0], c->f[TL_F_U], 2));
     HIPCHK(c, launch_sumsq_slot(c, c->f[TL_F_U0], 0));
     Ctrl k;
     RETIF(read_ctrl(c, &k));
     st->rel_resid = k.sums[0] > 0 ? std::sqrt(k.sums[2] / k.sums[0]) : std::sqrt(k.sums[2]);
+    if (tl_check_solution(p, st, bb, k.sums[0], k.sums[2]) != TL_OK) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "convergence check failed: %d iterations, converged %d, recomputed residual %.6e "
+                      "(relative %.6e), sum u0^2 %.17g by the solve and %.17g recomputed",
+                      st->iterations, st->converged, std::sqrt(k.sums[2]), st->rel_resid, bb, k.sums[0]);
+        c->err = msg;
+        st->converged = 0;
+        return TL_E_CHECK;
+    }
     return TL_OK;
 }
 
@@ -1023,7 +1056,7 @@ extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double r
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const auto t1 = std::chrono::steady_clock::now();
     st->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    RETIF(finish_stats(c, st));
+    RETIF(finish_stats(c, p, st));
     RETIF(prof_harvest(c));
     // training data for the learned warm start: (inputs of this step, solution)
     if (c->recording) RETIF(unet_record_step(c, rx, ry, p->coefficient));

@@ -96,6 +96,17 @@ __device__ __forceinline__ double wave_sum(double v)
 
 __device__ __forceinline__ unsigned block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
 
+// Partial sums cross kernel boundaries between blocks that run on different
+// XCDs, each with its own L2.  They are stored with agent-scope atomic
+// stores, so a consumer that reads them with agent-scope atomic loads (the
+// single-block finalizers, COH below) sees them whatever the dispatch
+// packets' cache fences do; the stencil kernels read them with plain loads
+// and rely on the kernel-boundary acquire, as stream order guarantees.
+__device__ __forceinline__ void store_partial(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // one partial per block: part[a * stride + block] = sum over the block of v[a]
 template <int N, int NT = TB>
 __device__ __forceinline__ void write_partials(const double (&v)[N], double *__restrict__ part, size_t stride)
@@ -115,13 +126,14 @@ __device__ __forceinline__ void write_partials(const double (&v)[N], double *__r
             double s = sh[a][0];
 #pragma unroll
             for (int q = 1; q < NT / 64; ++q) s += sh[a][q];
-            part[(size_t)a * stride + b] = s;
+            store_partial(&part[(size_t)a * stride + b], s);
         }
     }
 }
 
-// every thread of the block gets the fixed-order sum of nb partials
-template <int N, int NT = TB>
+// every thread of the block gets the fixed-order sum of nb partials;
+// COH: agent-scope atomic loads (coherent across the XCDs' L2s)
+template <int N, int NT = TB, bool COH = false>
 __device__ __forceinline__ void sum_partials(const double *__restrict__ part, unsigned nb, size_t stride,
                                              double (&tot)[N])
 {
@@ -135,7 +147,11 @@ __device__ __forceinline__ void sum_partials(const double *__restrict__ part, un
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const unsigned q = q0 + (unsigned)k * NT;
-                v[k] = q < nb ? part[(size_t)a * stride + q] : 0.0;
+                const double *pq = part + (size_t)a * stride + q;
+                if constexpr (COH)
+                    v[k] = q < nb ? __hip_atomic_load(pq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+                else
+                    v[k] = q < nb ? *pq : 0.0;
             }
 #pragma unroll
             for (int k = 0; k < 8; ++k) s += v[k];
@@ -381,14 +397,14 @@ __global__ __launch_bounds__(TB) void k_fin(Ctrl *__restrict__ c, const double *
     if ((kind == FK_CHEBY || kind == FK_JACOBI) && t > c->stop_iter) return;
     double tot[2] = {0.0, 0.0};
     if (n >= 2) {
-        sum_partials<2>(part, nb, stride, tot);
+        sum_partials<2, TB, true>(part, nb, stride, tot);
     } else {
         double t1[1];
-        sum_partials<1>(part, nb, stride, t1);
+        sum_partials<1, TB, true>(part, nb, stride, t1);
         tot[0] = t1[0];
     }
     double t34[2] = {0.0, 0.0};
-    if (n == 4) sum_partials<2>(part + 2 * (size_t)stride, nb, stride, t34);
+    if (n == 4) sum_partials<2, TB, true>(part + 2 * (size_t)stride, nb, stride, t34);
     if (threadIdx.x != 0) return;
     const double v[4] = {tot[0], tot[1], t34[0], t34[1]};
     for (int a = 0; a < n && slot + a < 4; ++a) c->sums[slot + a] = v[a];
@@ -401,6 +417,9 @@ __global__ __launch_bounds__(TB) void k_fin(Ctrl *__restrict__ c, const double *
         c->tol_abs = c->rel_tol * sqrt(v[1]);
         c->refresh_abs = c->ir_delta * sqrt(v[0]);
         if (v[0] == 0.0) {  // exact solution already: nothing to iterate
+            // (tl_solve recomputes the residual of the returned u with other
+            // kernels and rejects the claim if it is not zero: a reduction
+            // that missed its partials cannot pass as an exact start)
             c->why = WHY_CONVERGED;
             c->stop_iter = 0;
         }

@@ -13,7 +13,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(ROOT, "lib", "libtealeaf_hip.so")
 
-TL_OK, TL_E_INVALID, TL_E_HIP, TL_E_NOMEM, TL_E_BREAKDOWN, TL_E_EIGEN, TL_E_PARSE, TL_E_STATE, TL_E_IO = range(9)
+(TL_OK, TL_E_INVALID, TL_E_HIP, TL_E_NOMEM, TL_E_BREAKDOWN, TL_E_EIGEN, TL_E_PARSE, TL_E_STATE, TL_E_IO,
+ TL_E_CHECK) = range(10)
 TL_CG, TL_CHEBY, TL_PPCG, TL_JACOBI = 0, 1, 2, 3
 TL_FP64, TL_FP32_IR = 0, 1
 TL_STOP_REFERENCE, TL_STOP_RELATIVE = 0, 1
@@ -33,7 +34,7 @@ EXPORTS = [
     "tl_cheby_coef", "tl_cheby_calciter", "tl_apply_operator", "tl_get_cg_history", "tl_unet_param_count",
     "tl_unet_set_weights", "tl_unet_get_weights", "tl_unet_record", "tl_unet_dataset_size", "tl_unet_train",
     "tl_unet_predict", "tl_profile_begin", "tl_profile_read", "tl_profile_end",
-    "tl_kernel_repeat", "tl_unet_eval", "tl_checking_value", "tl_qa_check",
+    "tl_kernel_repeat", "tl_unet_eval", "tl_checking_value", "tl_qa_check", "tl_check_solution",
 ]
 TL_START_PREVIOUS, TL_START_WARM = 0, 1
 TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32 = range(6)
@@ -161,6 +162,8 @@ def load():
     L.tl_kernel_repeat.argtypes = [vp, C.c_int, C.c_int]
     L.tl_checking_value.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, dp, ip]
     L.tl_qa_check.argtypes = [C.c_double, C.c_double, dp]
+    L.tl_check_solution.argtypes = [C.POINTER(SolveParams), C.POINTER(SolveStats), C.c_double, C.c_double,
+                                    C.c_double]
     _lib = L
     return L
 

@@ -107,6 +107,9 @@ def test_cheby_coef_and_calciter_match_oracle():
     assert th.value == oth
     for args in ((1.2, 880.0, 3.3e-2, 4.1e3), (0.9, 55.0, 1e-3, 1e2), (1.0, 2.0e6, 5.0, 1e9)):
         assert L.tl_cheby_calciter(*args) == O.calciter(*args)
+    # known answers (condition number 9: gamma = 1/2; eps bb / (4 err) = 2^-20 -> 10)
+    assert L.tl_cheby_calciter(1.0, 9.0, 1.0, 2.0 ** 34) == 10
+    assert L.tl_cheby_calciter(1.0, 9.0, 2.0 ** -6, 2.0 ** 34) == 7
 
 
 def test_invalid_arguments_return_codes():
@@ -150,3 +153,44 @@ def test_benchmark_problem_is_fixed():
         "3a117da74cbbfd7a139452e2efe5f47c023044df222040def0fa82fae7366f86"
     s = T.Settings.from_text(text)
     assert (s.xcells, s.ycells) == (8192, 8192) and s.precision == T.TL_FP32_IR
+
+
+def _check(solver=_lib.TL_CG, precision=_lib.TL_FP64, stop_rule=_lib.TL_STOP_RELATIVE, rel_tol=1e-10, eps=1e-15,
+           iterations=100, converged=1, bb=4.0, u0sq=4.0, resid2=0.0):
+    p = _lib.SolveParams()
+    T.load().tl_default_params(C.byref(p))
+    p.solver, p.precision, p.stop_rule, p.rel_tol, p.eps = solver, precision, stop_rule, rel_tol, eps
+    st = _lib.SolveStats()
+    st.iterations, st.converged = iterations, converged
+    return T.load().tl_check_solution(C.byref(p), C.byref(st), bb, u0sq, resid2)
+
+
+def test_convergence_check_rejects_stale_reductions():
+    """tl_solve's last step: a convergence claim is checked against the FP64
+    residual of the returned u recomputed by separate kernels.  A reduction
+    that read zeros instead of its partials -- the solve's sum u0^2, the
+    residual at CG-init taken for an exact start, or an FP64 refresh taken for
+    convergence -- must come back as TL_E_CHECK, never as a converged solve."""
+    ok, bad = _lib.TL_OK, _lib.TL_E_CHECK
+    # relative rule, ||u0|| = 2: the threshold is 2e-10
+    assert _check(resid2=(1.9e-10) ** 2) == ok
+    assert _check(resid2=(2.0e-10) ** 2) == ok                         # on the threshold
+    assert _check(resid2=(2.0e-10 * (1 + 1e-8)) ** 2) == ok            # summation-order slack
+    assert _check(resid2=(2.1e-10) ** 2) == bad                        # claimed, not met
+    assert _check(precision=_lib.TL_FP32_IR, resid2=(3e-10) ** 2) == bad
+    assert _check(resid2=(3e-10) ** 2, converged=0) == ok               # max_iters: no claim
+    # the solve's sum u0^2 read as zero (stale partials) while the field is not zero
+    assert _check(bb=0.0, resid2=0.0) == bad
+    assert _check(bb=4.0 * (1 + 1e-12)) == ok
+    assert _check(bb=-1.0, solver=_lib.TL_JACOBI) == ok                # Jacobi forms no bb
+    # exact start (0 iterations): the recomputed residual must meet the rule
+    for solver in (_lib.TL_CG, _lib.TL_CHEBY, _lib.TL_PPCG):
+        assert _check(solver=solver, stop_rule=_lib.TL_STOP_REFERENCE, iterations=0, resid2=0.0) == ok
+        assert _check(solver=solver, stop_rule=_lib.TL_STOP_REFERENCE, iterations=0, resid2=1e-2) == bad
+    assert _check(iterations=0, resid2=1e-2) == bad
+    assert _check(iterations=0, resid2=(1e-10) ** 2) == ok
+    # the reference rule trusts the recursive residual after iteration 0 (CG.jl:24)
+    assert _check(stop_rule=_lib.TL_STOP_REFERENCE, iterations=5, resid2=1e-2) == ok
+    # NaN sums never pass
+    assert _check(resid2=float("nan")) == bad
+    assert _check(u0sq=float("nan"), bb=-1.0) == bad

@@ -308,12 +308,36 @@ def test_cheby_coef_closed_form():
     assert th == (emin + emax) / 2
 
 
-def test_calciter_formula():
-    # est = round(log10(eps*bb/(4 err)) / (2 log10(gamma))), gamma = (sqrt(k)-1)/(sqrt(k)+1)
-    emin, emax, err, bb = 1.2, 880.0, 3.3e-2, 4.1e3
-    g = (np.sqrt(emax / emin) - 1) / (np.sqrt(emax / emin) + 1)
-    expect = int(np.rint(np.log10(np.finfo(float).eps * bb / (4 * err)) / (2 * np.log10(g))))
-    assert O.calciter(emin, emax, err, bb) == expect
+def test_calciter_known_answers():
+    """Cheby.calciter (Cheby.jl:109-118) on hand-computed cases.  With
+    condition number 9, gamma = (3 - 1) / (3 + 1) = 1/2; eps(Float64) = 2^-52,
+    so bb = 2^34, err = 1 gives eps bb / (4 err) = 2^-20 and the estimate is
+    log(2^-20) / (2 log(1/2)) = 10 in any log base.  A dropped factor 2 gives
+    20, an inverted ratio -10, gamma taken as 1/gamma -10."""
+    assert O.calciter(1.0, 9.0, 1.0, 2.0 ** 34) == 10
+    assert O.calciter(0.5, 4.5, 1.0, 2.0 ** 34) == 10      # only the ratio eigmax/eigmin matters
+    assert O.calciter(1.0, 9.0, 2.0 ** -6, 2.0 ** 34) == 7  # err 2^-6: 2^-14 -> 7
+    # condition number 4: gamma = 1/3; eps bb / (4 err) = 3^-8 -> 4
+    bb = 3.0 ** -8 * 4 / np.finfo(float).eps
+    assert O.calciter(1.0, 4.0, 1.0, bb) == 4
+
+
+def test_calciter_rounding_bound():
+    """The estimate k is the nearest integer to x = ln(ita) / (2 ln gamma), so
+    |2 k ln(gamma) - ln(ita)| = 2 |ln gamma| |k - x| <= |ln gamma| for every
+    input (ita = eps bb / (4 err)); checked on random inputs spanning the
+    solver's range of condition numbers, errors and norms."""
+    rng = np.random.default_rng(12)
+    eps = np.finfo(float).eps
+    for _ in range(2000):
+        emin = 10.0 ** rng.uniform(-3, 1)
+        kappa = 10.0 ** rng.uniform(0.05, 7)
+        err = 10.0 ** rng.uniform(-12, 3)
+        bb = 10.0 ** rng.uniform(-3, 12)
+        k = O.calciter(emin, emin * kappa, err, bb)
+        g = (np.sqrt(kappa) - 1) / (np.sqrt(kappa) + 1)
+        ita = eps * bb / (4 * err)
+        assert abs(2 * k * np.log(g) - np.log(ita)) <= abs(np.log(g)) * (1 + 1e-9), (emin, kappa, err, bb, k)
 
 
 # ------------------------------------------------- literal-mode closed form

@@ -155,41 +155,77 @@ def roofline_from(prof, n, coef_format=None):
     }
 
 
-def cpu_baseline(n, seed, gpu_iters, budget_s):
-    """The CPU oracle's FP64 CG on the host: time a bounded number of CG
-    iterations of the first step at n^2, then extrapolate s/timestep with the
-    GPU's FP64 iteration count."""
+def cpu_model():
+    """CPU model name of this host (first 'model name' of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n, seed, gpu_iters, budget_s, c1_n=512):
+    """The CPU oracle's FP64 CG (the reference's algorithm restated in C) on
+    this host.  The reference is single-threaded, so `value` is the 1-thread
+    figure: a bounded number of CG iterations of the benchmark's first step at
+    n^2 is timed and s/timestep extrapolated with the GPU's FP64 iteration
+    count.  The same with OpenMP on every core this process may use is
+    reported beside it, and the 512^2 reference deck (10 CG steps) is timed in
+    full, on all cores and (when it fits the budget) on 1 thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    cores = len(os.sched_getaffinity(0))
     text = decks.random_rect_deck(seed, n=n, steps=1)
-    d = O.parse_deck(text)
-    d.stop_rule = O.STOP_RELATIVE
-    d.rel_tol = 1e-10
 
-    d.maxiters = 2000
-    o = O.Oracle(d, threads=threads)
-    o.initialise()
+    def per_iteration(threads, budget):
+        d = O.parse_deck(text)
+        d.stop_rule, d.rel_tol, d.maxiters = O.STOP_RELATIVE, 1e-10, 2000
+        o = O.Oracle(d, threads=threads)
+        o.initialise()
 
-    def timed(iters):
-        o.set_maxiters(iters)
+        def timed(iters):
+            o.set_maxiters(iters)
+            t0 = time.perf_counter()
+            o.step()
+            return time.perf_counter() - t0
+
+        t2, t4 = timed(2), timed(4)
+        per = max((t4 - t2) / 2, 1e-9)
+        fixed = max(t2 - 2 * per, 0.0)
+        iters = int(max(4, min(2000, (budget - fixed) / per)))
+        tn = timed(iters)
+        per = (tn - fixed) / iters
+        return {"s_per_timestep": float(f"{fixed + per * gpu_iters:.5g}"), "threads": threads,
+                "iterations_timed": iters, "timed_s": round(tn, 2), "ms_per_iteration": float(f"{per * 1e3:.5g}"),
+                "step_overhead_s": float(f"{fixed:.4g}")}
+
+    one = per_iteration(1, budget_s * 0.5)
+    allc = per_iteration(cores, budget_s * 0.3)
+
+    def c1(threads):
         t0 = time.perf_counter()
-        st = o.step()
-        return time.perf_counter() - t0, st
+        _, (summ, st) = O.run_deck(decks.reference_deck(c1_n, 10), threads=threads)
+        dt = time.perf_counter() - t0
+        return {"threads": threads, "s_total": round(dt, 3), "s_per_timestep": round(dt / 10, 4),
+                "iterations": [x["iterations"] for x in st], "temp_step10": float(summ[-1][3])}
 
-    t2, _ = timed(2)
-    t4, _ = timed(4)
-    per_iter = max((t4 - t2) / 2, 1e-9)
-    fixed = max(t2 - 2 * per_iter, 0.0)
-    iters = int(max(4, min(2000, (budget_s - fixed) / per_iter)))
-    tn, st = timed(iters)
-    per_iter = (tn - fixed) / iters
+    c1_runs = [c1(cores)]
+    est_one = c1_runs[0]["s_total"] * one["ms_per_iteration"] / max(allc["ms_per_iteration"], 1e-9)
+    if est_one <= budget_s * 1.5:
+        c1_runs.append(c1(1))
     return {
-        "value": round(fixed + per_iter * gpu_iters, 3), "unit": "s/timestep", "cores": threads, "kind": "port",
-        "sample": (f"CPU oracle FP64 CG at {n}^2 (seed {seed}): {iters} iterations timed ({tn:.1f} s, "
-                   f"{per_iter * 1e3:.1f} ms/iteration, step overhead {fixed:.2f} s), extrapolated to the GPU "
-                   f"FP64 CG's {gpu_iters:.0f} iterations/step"),
+        "value": one["s_per_timestep"], "unit": "s/timestep", "cores": 1, "kind": "port",
+        "cpu_model": cpu_model(), "host_cores_available": cores,
+        "sample": (f"CPU oracle (the reference's FP64 CG restated in C, -O3) at {n}^2 (seed {seed}) on 1 thread: "
+                   f"{one['iterations_timed']} CG iterations timed ({one['timed_s']} s, {one['ms_per_iteration']} "
+                   f"ms/iteration), extrapolated to the GPU FP64 CG's {gpu_iters:.0f} iterations/step"),
+        "all_cores": dict(allc, note=f"same sample with OpenMP on {cores} threads, extrapolated likewise"),
+        f"c1_{c1_n}_10_steps": c1_runs,
     }
 
 
@@ -258,6 +294,66 @@ def kernel_repeat(n, seed, kid, reps):
     print(json.dumps({"kernel": _lib.KERNEL_NAMES[kid], "reps": reps, "n": n}))
 
 
+# algorithmic bytes per interior cell of the kernels kernel_table measures
+# (FP64 solver kernels: arrays read once, written once; k_residual in the form
+# tl_kernel_repeat runs it, writing the FP64 r)
+TABLE_BYTES = dict(KERNEL_BYTES)
+TABLE_BYTES.update({_lib.TL_K_CHEBY64: 56, _lib.TL_K_PPCG64: 64, _lib.TL_K_JACOBI64: 40, _lib.TL_K_RESID64: 40})
+
+
+def kernel_table(n, seed, reps):
+    """Per-kernel timing on the n^2 benchmark problem: `reps` launches (CG:
+    iterations) of each hot kernel bracketed by HIP events on the library's
+    stream (tl_kernel_repeat + tl_profile_*), one JSON line per kernel with the
+    mean duration and the algorithmic GB/s; the 16-byte copy kernel is timed
+    with the host clock around `reps` launches as the bandwidth reference."""
+    L = _lib.load()
+    groups = [(_lib.TL_K_CG_W64, (_lib.TL_K_CG_W64, _lib.TL_K_CG_UR64)),
+              (_lib.TL_K_CG_W32, (_lib.TL_K_CG_W32, _lib.TL_K_CG_UR32)),
+              (_lib.TL_K_CHEBY64, (_lib.TL_K_CHEBY64,)), (_lib.TL_K_PPCG64, (_lib.TL_K_PPCG64,)),
+              (_lib.TL_K_JACOBI64, (_lib.TL_K_JACOBI64,)), (_lib.TL_K_RESID64, (_lib.TL_K_RESID64,))]
+    s = make_settings(n, seed, "fp64", "none")
+    chunk = T.initialiseapp(s)
+    ctx = chunk._ctx
+    for run_id, kids in groups:
+        mask = 0
+        for k in kids:
+            mask |= 1 << k
+        _lib.check(L.tl_profile_begin(ctx, mask, 1), ctx, "profile")
+        _lib.check(L.tl_kernel_repeat(ctx, run_id, reps), ctx, "kernel_repeat")
+        for k in kids:
+            cnt = C.c_int()
+            _lib.check(L.tl_profile_read(ctx, k, None, 0, C.byref(cnt)), ctx, "profile")
+            buf = np.zeros(cnt.value)
+            _lib.check(L.tl_profile_read(ctx, k, buf.ctypes.data_as(C.POINTER(C.c_double)), cnt.value,
+                                         C.byref(cnt)), ctx, "profile")
+            # launch i is iteration i + 1; the first quarter counts as warm-up
+            idx = np.arange(len(buf))
+            keep = idx >= len(buf) // 4
+            if k == _lib.TL_K_CG_UR32:
+                forms = {"odd": buf[keep & (idx % 2 == 0)], "even": buf[keep & (idx % 2 == 1)]}
+            else:
+                forms = {"": buf[keep]}
+            for form, v in forms.items():
+                b = TABLE_BYTES[k] if form == "" else (14 if form == "odd" else 26)
+                ms = float(np.mean(v))
+                print(json.dumps({"kernel": _lib.KERNEL_NAMES[k] + (f" ({form} t)" if form else ""), "n": n,
+                                  "launches": int(len(v)), "avg_ms": round(ms, 5), "median_ms": round(
+                                      float(np.median(v)), 5), "bytes_per_cell": b,
+                                  "GBps": round(b * n * n / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        L.tl_profile_end(ctx)
+    for kid, b in ((-2, 16),):
+        L.tl_kernel_repeat(ctx, kid, 5)
+        t0 = time.perf_counter()
+        _lib.check(L.tl_kernel_repeat(ctx, kid, reps), ctx, "kernel_repeat")
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        pitch = (n + chunk.hd + 32 + 63) // 64 * 64  # the library's row pitch (elements)
+        nbytes = 2 * 8 * (pitch * chunk.Y + 64)       # one field allocation read and written
+        print(json.dumps({"kernel": _lib.KERNEL_NAMES[kid], "n": n, "launches": reps, "avg_ms": round(ms, 5),
+                          "bytes": nbytes, "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+    chunk.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,7 +363,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--kernel-repeat", type=int, nargs=2, metavar=("KERNEL_ID", "REPS"))
     ap.add_argument("--compare-steps", type=int, default=0,
                     help="FP32-refined vs FP64 CG side by side for this many timesteps (per-step summaries)")
@@ -279,12 +375,17 @@ def main():
                     help="FP32-refined CG: refresh the FP64 residual when the FP32 one falls by this factor")
     ap.add_argument("--solvers", action="store_true",
                     help="CG, Chebyshev and PPCG on the 3-state reference deck at --n (default 4096 here)")
+    ap.add_argument("--kernel-table", type=int, default=0, metavar="REPS",
+                    help="per-kernel HIP-event timing of every hot kernel, REPS launches each, at --n")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the solve would shard by 2-D domain decomposition "
               "(halo exchange of p and u, two scalar all-reduces per CG iteration) but no multi-GPU path "
               "is built, so --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if a.kernel_table:
+        kernel_table(a.n, a.seed, a.kernel_table)
+        return
     if a.kernel_repeat:
         kernel_repeat(a.n, a.seed, a.kernel_repeat[0], a.kernel_repeat[1])
         return

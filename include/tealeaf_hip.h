@@ -257,6 +257,11 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 #define TL_K_CG_W32 3  /* FP32 PW of the inner CG (w = A p only reduced, not stored) */
 #define TL_K_CG_UR32 4 /* FP32 UR of the inner CG (recomputes A p) */
 #define TL_K_CG_P32 5  /* unused: folded into TL_K_CG_W32 */
+#define TL_K_CHEBY64 6 /* Chebyshev iteration (Cheby.mainstep!) */
+#define TL_K_PPCG64 7  /* PPCG inner Chebyshev step */
+#define TL_K_JACOBI64 8 /* Jacobi sweep */
+#define TL_K_RESID64 9 /* residual! r = u0 - A u (also the FP64 refresh of TL_FP32_IR) */
+#define TL_K_COUNT 10
 /* Bracket every period-th launch of each kernel in kernel_mask (bit =
  * 1 << TL_K_*; period >= 1) with HIP events on the context's stream (an event
  * pair costs a few microseconds of GPU time, so a period of about 16 keeps the
@@ -273,7 +278,9 @@ int tl_profile_end(tl_ctx *ctx);
  * that never fires), TL_K_CG_P64 `reps` launches of that kernel; -1 runs a
  * copy of one field's (nx+2h)*(ny+2h) cells (8 B read + 8 B written per cell,
  * one cell per lane), -2 a 16-byte-per-lane copy of one field's whole
- * allocation */
+ * allocation; TL_K_CHEBY64, TL_K_PPCG64, TL_K_JACOBI64 and TL_K_RESID64 run
+ * `reps` launches of that kernel on the CG-initialised fields with fixed
+ * coefficients (values not meaningful: for timing and counters only) */
 int tl_kernel_repeat(tl_ctx *ctx, int kernel_id, int reps);
 
 /* --------------------------------------------------- learned warm start */

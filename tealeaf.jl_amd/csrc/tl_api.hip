@@ -930,7 +930,7 @@ static int prof_harvest(tl_ctx *c)
     for (size_t i = 0; i + 1 < c->prof_used; i += 2) {
         float ms = 0.0f;
         HIPCHK(c, hipEventElapsedTime(&ms, c->prof_ev[i], c->prof_ev[i + 1]));
-        c->prof_ms[c->prof_tag[i / 2] & 7].push_back(ms);
+        c->prof_ms[c->prof_tag[i / 2]].push_back(ms);
     }
     c->prof_used = 0;
     return TL_OK;
@@ -938,7 +938,7 @@ static int prof_harvest(tl_ctx *c)
 
 extern "C" int tl_profile_begin(tl_ctx *c, unsigned kernel_mask, int period)
 {
-    if (!c || (kernel_mask >> (TL_K_CG_P32 + 1)) || period < 1) return TL_E_INVALID;
+    if (!c || (kernel_mask >> TL_K_COUNT) || period < 1) return TL_E_INVALID;
     RETIF(prof_harvest(c));
     c->prof_mask = kernel_mask;
     c->prof_period = (unsigned)period;
@@ -949,7 +949,7 @@ extern "C" int tl_profile_begin(tl_ctx *c, unsigned kernel_mask, int period)
 
 extern "C" int tl_profile_read(tl_ctx *c, int kernel_id, double *durations_ms, int max, int *count)
 {
-    if (!c || !count || kernel_id < 0 || kernel_id > TL_K_CG_P32) return TL_E_INVALID;
+    if (!c || !count || kernel_id < 0 || kernel_id >= TL_K_COUNT) return TL_E_INVALID;
     RETIF(prof_harvest(c));
     const std::vector<double> &v = c->prof_ms[kernel_id];
     *count = (int)v.size();
@@ -969,7 +969,7 @@ extern "C" int tl_profile_end(tl_ctx *c)
 
 extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
 {
-    if (!c || reps < 1 || kernel_id < -2 || kernel_id > TL_K_CG_P32) return TL_E_INVALID;
+    if (!c || reps < 1 || kernel_id < -2 || kernel_id >= TL_K_COUNT) return TL_E_INVALID;
     HIPCHK(c, hipSetDevice(c->device));
     if (kernel_id < 0) {
         for (int i = 0; i < reps; ++i)
@@ -983,6 +983,23 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
     HIPCHK(c, launch_init_coef(c, 1.0, 1.0, 1, INIT_CG));
     HIPCHK(c, launch_init_ctrl(c, MODE_CG_REF, 1 << 30, 0.0, 0.0, 0.1, 0, 0, 0.0));
     HIPCHK(c, launch_cg_init_resid(c, nullptr));
+    if (kernel_id >= TL_K_CHEBY64) {
+        // one FP64 solver kernel, `reps` launches on the CG-initialised fields
+        // (u, u0, p, r, kx, ky) with fixed coefficients; never stopped
+        HIPCHK(c, launch_set_mode(c, MODE_JACOBI, 1 << 30));
+        double *a = c->f[TL_F_U], *b = c->u_alt, *s0 = c->f[TL_F_SD], *s1 = c->sd_alt;
+        for (int i = 0; i < reps; ++i) {
+            const int t = 1 + i;
+            switch (kernel_id) {
+            case TL_K_CHEBY64: HIPCHK(c, launch_cheby_iter(c, 0.5, 1e-3, a, b, 0, t)); std::swap(a, b); break;
+            case TL_K_PPCG64: HIPCHK(c, launch_ppcg_inner(c, 0.5, 1e-3, s0, s1, 0, t)); std::swap(s0, s1); break;
+            case TL_K_JACOBI64: HIPCHK(c, launch_jacobi(c, a, b, t, 1)); std::swap(a, b); break;
+            default: HIPCHK(c, launch_residual(c)); break;
+            }
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return TL_OK;
+    }
     if (kernel_id >= TL_K_CG_W32) {
         RETIF(ensure_fp32(c));
         HIPCHK(c, launch_ir_prepare(c, 1.0, 1.0, 1));

@@ -82,9 +82,14 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_EDGE_PF
 #define TL_EDGE_PF 1
 #endif
-// XCD-aware block-to-tile mapping of the vectorised kernels
+// block-to-tile mapping of the vectorised kernels (blocks go round-robin to
+// the 8 XCDs, block b to XCD b % 8): 0 row-major; 1 each XCD a contiguous
+// slab of bands; 2 groups of 8 bands, band 8q + x on XCD x with its tiles in
+// consecutive slots, so horizontally adjacent tiles share one L2.  Measured
+// at 8192^2 in A/B runs on one box: 2 makes the FP32 PW 2%, the FP32 UR 4%
+// (odd t) and the FP64 PW 1.4% faster than 0; 1 is 2-4% slower than 0.
 #ifndef TL_XCD_MAP
-#define TL_XCD_MAP 0  // measured at 8192^2: 1.5% less traffic but 5-8% slower kernels
+#define TL_XCD_MAP 2
 #endif
 // optional occupancy target (waves per SIMD) of the FP32 inner-CG kernels
 #ifdef TL_WPE32
@@ -210,11 +215,11 @@ struct tl_ctx {
     // per-launch event timing of selected kernels (tl_profile_begin)
     unsigned prof_mask = 0;
     unsigned prof_period = 1;   // bracket every prof_period-th launch of a kernel
-    unsigned prof_seq[8] = {};  // launches per kernel since tl_profile_begin
+    unsigned prof_seq[TL_K_COUNT] = {};  // launches per kernel since tl_profile_begin
     std::vector<hipEvent_t> prof_ev;          // pairs, reused
     std::vector<int> prof_tag;                // kernel id of each recorded pair
     size_t prof_used = 0;                     // events recorded since the last harvest
-    std::vector<double> prof_ms[8];
+    std::vector<double> prof_ms[TL_K_COUNT];
     tl::Unet *unet = nullptr;
     bool recording = false;
     // hipGraph batches of CG iterations (TL_LAUNCH_GRAPH), keyed by solve kind,

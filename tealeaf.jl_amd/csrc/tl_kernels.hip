@@ -827,22 +827,31 @@ struct VCol {
     int lane;
 };
 
-// Tile of this block.  The vector kernels launch a 1-D grid of gx*gy blocks,
-// row-major over the tiles.  With TL_XCD_MAP the hardware's round-robin
-// (block b to XCD b % 8) is undone, tile = (b % 8) * (nb / 8) + b / 8, so
-// each XCD works on a contiguous slab of row bands and the rows and columns
-// a block re-reads from its neighbours' tiles stay in one L2; measured at
-// 8192^2 this saves 1.5% of the fetched bytes but slows the kernels by 5-8%,
-// so it is off by default.
+// Tile of this block.  The vector kernels launch a 1-D grid of gx*gy blocks
+// that the hardware deals round-robin to the 8 XCDs (block b to XCD b % 8),
+// each XCD with its own L2.  Row-major tiles would put every horizontally
+// adjacent tile on another XCD, so the neighbour columns a block reads at its
+// edges come from another L2.  TL_XCD_MAP 2 (the default) gives the 8 bands
+// of a group to the 8 XCDs and a band's tiles to consecutive slots of one
+// XCD; the blocks in flight still cover a contiguous range of rows.
+// TL_XCD_MAP 1 gives each XCD a contiguous slab of bands (slower).
 template <int V>
 __device__ __forceinline__ void vtile(const Geo &g, int &bx, int &by)
 {
     const int gxv = V == 4 ? g.gx4 : g.gx2;
     const unsigned b = blockIdx.x;
     unsigned tile = b;
-#if TL_XCD_MAP
+#if TL_XCD_MAP == 1
     const unsigned nb = gridDim.x;
     if (nb % 8 == 0) tile = (b % 8) * (nb / 8) + b / 8;
+#elif TL_XCD_MAP == 2
+    // groups of 8 bands: band 8q + x of group q on XCD x = b % 8, its tiles
+    // in consecutive slots, so horizontally adjacent tiles share an L2
+    const unsigned nb = gridDim.x, gyv = nb / (unsigned)gxv;
+    if (gyv % 8 == 0) {
+        const unsigned x = b % 8, s = b / 8;
+        tile = ((s / (unsigned)gxv) * 8 + x) * (unsigned)gxv + s % (unsigned)gxv;
+    }
 #endif
     bx = (int)(tile % (unsigned)gxv);
     by = (int)(tile / (unsigned)gxv);
@@ -1967,7 +1976,7 @@ static inline int targ(int t, int tref) { return tref > 0 ? -(t - tref) : t; }
 static bool prof_start(tl_ctx *c, int kind)
 {
     if (c->capturing || !(c->prof_mask & (1u << kind))) return false;
-    if (c->prof_seq[kind & 7]++ % c->prof_period != 0) return false;
+    if (c->prof_seq[kind]++ % c->prof_period != 0) return false;
     if (c->prof_used + 2 > c->prof_ev.size()) {
         hipEvent_t e0, e1;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return false;
@@ -2172,8 +2181,10 @@ hipError_t launch_cg64_true_residual(tl_ctx *c, int T)
 
 hipError_t launch_residual(tl_ctx *c)
 {
+    const bool prof = prof_start(c, TL_K_RESID64);
     k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],
                                                   c->f[TL_F_KY], c->f[TL_F_R], nullptr, nullptr, c->ctrl, nullptr);
+    if (prof) prof_stop(c);
     return hipGetLastError();
 }
 
@@ -2187,9 +2198,11 @@ hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t)
 
 hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next)
 {
+    const bool prof = prof_start(c, TL_K_RESID64);
     k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],
                                                   c->f[TL_F_KY], nullptr, c->r32, reset_p ? c->p32 : nullptr,
                                                   c->ctrl, part_r(c, t_next - 1));
+    if (prof) prof_stop(c);
     nbr(c, t_next - 1) = nbh(c);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? fin(c, part_r(c, t_next - 1), nbh(c), 1, 0, FK_IR, t_next - 1) : e;
@@ -2253,8 +2266,10 @@ hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double
 
 hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t)
 {
+    const bool prof = prof_start(c, TL_K_CHEBY64);
     k_cheby_iter<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_P],
                                                     c->f[TL_F_KX], c->f[TL_F_KY], a, b, norm, c->ctrl, part_m(c));
+    if (prof) prof_stop(c);
     hipError_t e = hipGetLastError();
     return (e == hipSuccess && norm) ? fin(c, part_m(c), nbh(c), 1, 0, FK_CHEBY, t) : e;
 }
@@ -2269,17 +2284,21 @@ hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t)
 
 hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t)
 {
+    const bool prof = prof_start(c, TL_K_PPCG64);
     k_ppcg_inner<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, sd_in, sd_out, c->f[TL_F_R], c->f[TL_F_U],
                                                     c->f[TL_F_KX], c->f[TL_F_KY], a, b, last, c->ctrl,
                                                     part_r(c, t));
+    if (prof) prof_stop(c);
     if (last) nbr(c, t) = nbh(c);
     return hipGetLastError();
 }
 
 hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer)
 {
+    const bool prof = prof_start(c, TL_K_JACOBI64);
     k_jacobi<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_KX], c->f[TL_F_KY],
                                                 c->ctrl, part_m(c));
+    if (prof) prof_stop(c);
     hipError_t e = hipGetLastError();
     return (e == hipSuccess && !defer) ? fin(c, part_m(c), nbh(c), 1, 0, FK_JACOBI, t) : e;
 }

@@ -37,10 +37,13 @@ EXPORTS = [
     "tl_kernel_repeat", "tl_unet_eval", "tl_checking_value", "tl_qa_check", "tl_check_solution",
 ]
 TL_START_PREVIOUS, TL_START_WARM = 0, 1
-TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32 = range(6)
+(TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32, TL_K_CHEBY64, TL_K_PPCG64,
+ TL_K_JACOBI64, TL_K_RESID64) = range(10)
+TL_K_COUNT = 10
 KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, false>", 2: "k_cg_p<double>",
                 3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
                 5: "(fused into k_cgg<4, false, 2, true, 1>)",
+                6: "k_cheby_iter", 7: "k_ppcg_inner", 8: "k_jacobi", 9: "k_residual",
                 -1: "k_copy_full", -2: "k_copy_vec"}
 
 

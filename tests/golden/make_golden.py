@@ -20,6 +20,10 @@ Fixtures (JSON; floats written as shortest round-trip decimals):
                        bounds
   rect_32_u.json       a 32^2 random-rectangle deck (seed 9): the interior
                        of u after one CG step
+  rect_256_fp64.json   a 256^2 random-rectangle deck (seed 3), 3 steps of
+                       FP64 CG to a relative residual of 1e-12: the
+                       reference answer of the mixed-precision (FP32-refined)
+                       solver, which must agree within 1e-9
   tea.problems         written to tealeaf.jl_amd/decks/: checking values
                        (final temp) for the reference decks above, in the
                        reference's "x y endstep value" format
@@ -73,11 +77,23 @@ def rect_u():
     return {"deck": text, "summary": s.tolist(), "stats": _stats(st), "u_interior": o.interior("u").tolist()}
 
 
+def rect_fp64():
+    text = decks.random_rect_deck(3, n=256, steps=3, precision="fp32_ir")
+    d = O.parse_deck(text)
+    d.stop_rule, d.rel_tol, d.maxiters = O.STOP_RELATIVE, 1e-12, 100000
+    o = O.Oracle(d, threads=8)
+    o.initialise()
+    s, st = o.run(3)
+    return {"deck": text, "rel_tol": 1e-12, "summary": s.tolist(), "stats": _stats(st)}
+
+
 def main():
     c = c1()
     sv = solvers()
     r = rect_u()
-    for name, obj in (("c1_512_cg.json", c), ("solvers_64.json", sv), ("rect_32_u.json", r)):
+    rf = rect_fp64()
+    for name, obj in (("c1_512_cg.json", c), ("solvers_64.json", sv), ("rect_32_u.json", r),
+                      ("rect_256_fp64.json", rf)):
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(obj, f, indent=1)
             f.write("\n")
@@ -85,7 +101,8 @@ def main():
              f"64 64 3 {sv['cg']['summary'][-1][3]!r}"]
     with open(os.path.join(ROOT, "tealeaf.jl_amd", "decks", "tea.problems"), "w") as f:
         f.write("\n".join(lines) + "\n")
-    print("wrote", ", ".join(["c1_512_cg.json", "solvers_64.json", "rect_32_u.json", "tea.problems"]))
+    print("wrote", ", ".join(["c1_512_cg.json", "solvers_64.json", "rect_32_u.json", "rect_256_fp64.json",
+                              "tea.problems"]))
 
 
 if __name__ == "__main__":

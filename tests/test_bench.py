@@ -51,3 +51,17 @@ def test_more_than_one_gpu_is_refused():
     assert p.returncode == 2
     assert "--gpus must be 1" in p.stderr
     assert p.stdout == ""
+
+
+def test_cpu_baseline_record():
+    """The CPU leg's record on a tiny grid: the 1-thread figure is the value
+    (the reference is single-threaded), the all-core figure, the CPU model and
+    the reference deck timed in full ride along."""
+    r = bench.cpu_baseline(48, 1001, 500, 1.0, c1_n=24)
+    assert r["cores"] == 1 and r["kind"] == "port" and r["unit"] == "s/timestep" and r["value"] > 0
+    assert r["cpu_model"] and r["host_cores_available"] >= 1
+    assert r["all_cores"]["threads"] == r["host_cores_available"]
+    c1 = r["c1_24_10_steps"]
+    assert c1[0]["threads"] == r["host_cores_available"] and len(c1[0]["iterations"]) == 10
+    if len(c1) > 1:
+        assert c1[1]["threads"] == 1 and c1[1]["temp_step10"] == c1[0]["temp_step10"]

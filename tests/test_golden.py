@@ -83,3 +83,21 @@ def test_checking_value_lookup_and_qa_rule(tmp_path):
     assert L.tl_qa_check(temp * (1 + 1.1e-5), temp, C.byref(d)) == 0  # |100 a/cv - 100| >= 0.001 fails
     s = T.Settings.from_text(g["deck"])
     assert T.checkingvalue(s, PROBLEMS) == temp
+
+
+def test_oracle_reproduces_fp64_rectangle_fixture():
+    """The FP64 answer the mixed-precision solver is held to: the oracle
+    reproduces it bit for bit, each step's residual meets 1e-12 and the
+    conserved quantities stay fixed (vol, mass exactly; ie = sum vol*u to
+    the residual)."""
+    g = _load("rect_256_fp64.json")
+    d = O.parse_deck(g["deck"])
+    d.stop_rule, d.rel_tol, d.maxiters = O.STOP_RELATIVE, g["rel_tol"], 100000
+    o = O.Oracle(d, threads=8)
+    o.initialise()
+    s, st = o.run(3)
+    assert np.array_equal(s, np.array(g["summary"]))
+    assert all(x["rel_resid"] <= 1e-12 for x in st)
+    assert np.all(s[:, 0] == s[0, 0]) and np.all(s[:, 1] == s[0, 1])
+    np.testing.assert_allclose(s[:, 2], s[0, 2], rtol=1e-9)
+

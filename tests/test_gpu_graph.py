@@ -7,6 +7,9 @@ stream run, after every timestep: each solve replays the cached graph many
 times, with the stream's own work (FP64 refreshes, finalise, halo updates,
 field summary) and host synchronisations between the replays.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -80,3 +83,48 @@ def test_launch_mode_validated():
     c = T.initialiseapp(s)
     with pytest.raises(T.TeaLeafError):
         T.step(c, s)
+
+
+def _golden(name):
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)) as f:
+        return json.load(f)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+def test_graph_mode_c1_deck_matches_golden():
+    """The 512^2 reference deck, 10 CG steps with hipGraph batches, against
+    the oracle's fixture (not against stream launches): field summaries
+    within 1e-10 every step, iterations within max(3, 2%) of the oracle's."""
+    print("case: graph mode, golden 512^2 reference deck, 10 CG steps", flush=True)
+    g = _golden("c1_512_cg.json")
+    s = T.Settings.from_text(g["deck"]).set(launch=T.TL_LAUNCH_GRAPH)
+    c = T.initialiseapp(s)
+    summ, st = T.diffuse(c, s, 10)
+    for t in range(10):
+        want = g["intended"]["summary"][t]
+        assert _rel(summ[t], want) < 1e-10, (t, summ[t], want)
+        wi = g["intended"]["stats"][t]["iterations"]
+        assert abs(st[t]["iterations"] - wi) <= max(3, wi // 50), (t, st[t]["iterations"], wi)
+        assert st[t]["iterations"] > 64  # the solve ran replays, not only the first stream batch
+
+
+@pytest.mark.parametrize("precision", ["fp32_ir", "fp64"])
+def test_graph_mode_random_rectangles_match_fp64_fixture(precision):
+    """A 256^2 random-rectangle deck, 3 steps with hipGraph batches in each
+    precision, against the oracle's FP64 CG solved to a relative residual
+    of 1e-12 (rect_256_fp64.json): summaries within 1e-9 every step and the
+    FP64 residual of every returned solution within 1e-10."""
+    print(f"case: graph mode {precision}, 256^2 random rectangles vs the FP64 fixture", flush=True)
+    g = _golden("rect_256_fp64.json")
+    text = g["deck"].replace("precision=fp32_ir", f"precision={precision}")
+    s = T.Settings.from_text(text).set(launch=T.TL_LAUNCH_GRAPH)
+    c = T.initialiseapp(s)
+    summ, st = T.diffuse(c, s, 3)
+    for t in range(3):
+        assert st[t]["converged"] == 1 and st[t]["rel_resid"] <= 1e-10, (t, st[t])
+        assert st[t]["iterations"] > 64
+        assert _rel(summ[t], g["summary"][t]) < 1e-9, (t, summ[t], g["summary"][t])

@@ -49,11 +49,17 @@ PROFILE_PERIOD = 15  # odd: both parities of the alternating FP32 UR kernel are 
 #          and at even iterations also x += alpha(t-1) p(t-1) + alpha(t) p(t):
 #          reads x, p(t-1), writes x                                     (+12 B)
 #          i.e. 20 B per launch on average
+#   FP32 inner CG as one kernel per iteration (the default when nx % 4 == 0):
+#          reads r(t-1), p(t-1) (4 B each), gc (2 B), writes r(t), p(t)       (18 B)
+#          and at even iterations reads and writes x                      (+8 B)
+#          i.e. 22 B per launch (= per iteration) on average
 # P64 is only the final p update of an FP64 solve.
 KERNEL_BYTES = {
     _lib.TL_K_CG_W64: 48, _lib.TL_K_CG_UR64: 48, _lib.TL_K_CG_P64: 24,
-    _lib.TL_K_CG_W32: 14, _lib.TL_K_CG_UR32: 20, _lib.TL_K_CG_P32: 0,
+    _lib.TL_K_CG_W32: 14, _lib.TL_K_CG_UR32: 20, _lib.TL_K_CG_P32: 0, _lib.TL_K_CGF32: 22,
 }
+# kernels whose odd and even iterations move different bytes: (odd, even) B/cell
+ALTERNATING = {_lib.TL_K_CG_UR32: (14, 26), _lib.TL_K_CGF32: (18, 26)}
 WEIGHTS = os.path.join(ROOT, "tealeaf.jl_amd", "weights", "unet_warmstart.npy")
 # HBM bytes per launch of the CG kernels at 8192^2 from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE counter runs, scaled by a copy kernel of known size
@@ -77,9 +83,15 @@ def make_settings(n, seed, precision, warm):
     return T.Settings.from_text(text)
 
 
-def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, launch="stream", ir_delta=None):
+FP32_KERNELS = {"auto": _lib.TL_FP32_KERNELS_AUTO, "two": _lib.TL_FP32_KERNELS_TWO,
+                "fused": _lib.TL_FP32_KERNELS_FUSED}
+
+
+def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, launch="stream", ir_delta=None,
+               fp32_kernels="auto"):
     s = make_settings(n, seed, precision, warm)
-    s.set(launch={"stream": T.TL_LAUNCH_STREAM, "graph": T.TL_LAUNCH_GRAPH}[launch])
+    s.set(launch={"stream": T.TL_LAUNCH_STREAM, "graph": T.TL_LAUNCH_GRAPH}[launch],
+          fp32_kernels=FP32_KERNELS[fp32_kernels])
     if ir_delta is not None:
         s.set(ir_delta=ir_delta)
     chunk = T.initialiseapp(s)
@@ -113,7 +125,7 @@ def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, la
     t_all1 = time.perf_counter()
     prof = {}
     if profile_mask:
-        for kid in range(6):
+        for kid in range(_lib.TL_K_COUNT):
             if profile_mask & (1 << kid):
                 cnt = C.c_int()
                 _lib.check(L.tl_profile_read(chunk._ctx, kid, None, 0, C.byref(cnt)), chunk._ctx, "profile")
@@ -256,14 +268,14 @@ def solvers_run(n, steps):
         print(json.dumps({"solver": solver, "max_rel_summary_diff_vs_cg": d}), flush=True)
 
 
-def compare_run(n, seed, steps):
+def compare_run(n, seed, steps, fp32_kernels="auto"):
     """The benchmark problem for `steps` timesteps with the FP32-refined CG
     and with FP64 CG side by side: one JSON line per step with both field
     summaries, their largest relative difference, iterations, true residuals
     and times."""
     runs = {}
     for prec in ("fp32_ir", "fp64"):
-        s = make_settings(n, seed, prec, "none")
+        s = make_settings(n, seed, prec, "none").set(fp32_kernels=FP32_KERNELS[fp32_kernels])
         runs[prec] = (s, T.initialiseapp(s))
     worst = 0.0
     for k in range(steps):
@@ -310,6 +322,7 @@ def kernel_table(n, seed, reps):
     L = _lib.load()
     groups = [(_lib.TL_K_CG_W64, (_lib.TL_K_CG_W64, _lib.TL_K_CG_UR64)),
               (_lib.TL_K_CG_W32, (_lib.TL_K_CG_W32, _lib.TL_K_CG_UR32)),
+              (_lib.TL_K_CGF32, (_lib.TL_K_CGF32,)),
               (_lib.TL_K_CHEBY64, (_lib.TL_K_CHEBY64,)), (_lib.TL_K_PPCG64, (_lib.TL_K_PPCG64,)),
               (_lib.TL_K_JACOBI64, (_lib.TL_K_JACOBI64,)), (_lib.TL_K_RESID64, (_lib.TL_K_RESID64,))]
     s = make_settings(n, seed, "fp64", "none")
@@ -330,12 +343,12 @@ def kernel_table(n, seed, reps):
             # launch i is iteration i + 1; the first quarter counts as warm-up
             idx = np.arange(len(buf))
             keep = idx >= len(buf) // 4
-            if k == _lib.TL_K_CG_UR32:
+            if k in ALTERNATING:
                 forms = {"odd": buf[keep & (idx % 2 == 0)], "even": buf[keep & (idx % 2 == 1)]}
             else:
                 forms = {"": buf[keep]}
             for form, v in forms.items():
-                b = TABLE_BYTES[k] if form == "" else (14 if form == "odd" else 26)
+                b = TABLE_BYTES[k] if form == "" else ALTERNATING[k][0 if form == "odd" else 1]
                 ms = float(np.mean(v))
                 print(json.dumps({"kernel": _lib.KERNEL_NAMES[k] + (f" ({form} t)" if form else ""), "n": n,
                                   "launches": int(len(v)), "avg_ms": round(ms, 5), "median_ms": round(
@@ -369,6 +382,8 @@ def main():
                     help="FP32-refined vs FP64 CG side by side for this many timesteps (per-step summaries)")
     ap.add_argument("--only", choices=("flagship", "fp64"), default=None,
                     help="run one method only (long runs); per-step field summaries are in the JSON")
+    ap.add_argument("--fp32-kernels", choices=tuple(FP32_KERNELS), default="auto",
+                    help="FP32-refined CG: one fused kernel per iteration or the two-kernel form")
     ap.add_argument("--launch", choices=("stream", "graph"), default="stream",
                     help="CG iterations as stream launches or hipGraph replays (graph: no per-kernel timing)")
     ap.add_argument("--ir-delta", type=float, default=None,
@@ -393,7 +408,7 @@ def main():
         solvers_run(a.n if a.n != 8192 else 4096, a.steps)
         return
     if a.compare_steps:
-        compare_run(a.n, a.seed, a.compare_steps)
+        compare_run(a.n, a.seed, a.compare_steps, a.fp32_kernels)
         return
     learned = os.path.exists(WEIGHTS)
     warm = "learned" if learned else "none"
@@ -405,9 +420,9 @@ def main():
                           "rel_resid": [st["rel_resid"] for st in base["stats"]],
                           "summaries": [list(map(float, x)) for x in base["summaries"]]}))
         return
-    mask = (1 << _lib.TL_K_CG_W32) | (1 << _lib.TL_K_CG_UR32) | (1 << _lib.TL_K_CG_P32)
+    mask = (1 << _lib.TL_K_CG_W32) | (1 << _lib.TL_K_CG_UR32) | (1 << _lib.TL_K_CGF32)
     flag = run_method("fp32-ir" + ("+learned" if learned else ""), a.n, a.seed, "fp32_ir", warm, a.steps,
-                      a.warmup, mask if a.launch == "stream" else 0, a.launch, a.ir_delta)
+                      a.warmup, mask if a.launch == "stream" else 0, a.launch, a.ir_delta, a.fp32_kernels)
     base = None
     if not a.no_baseline and a.only is None:
         base = run_method("fp64-cg", a.n, a.seed, "fp64", "none", a.steps, a.warmup, launch=a.launch)
@@ -420,6 +435,7 @@ def main():
         "config": {
             "workload": f"TeaLeaf {a.n}x{a.n} random rectangles (32, seed {a.seed}), dt 0.004, rel. resid 1e-10",
             "grid": a.n, "seed": a.seed, "warm_start": warm, "parallelism": "single GPU", "launch": a.launch,
+            "fp32_kernels": a.fp32_kernels,
         },
         "flagship": {
             "s_per_step": [round(t, 4) for t in flag["times"]],

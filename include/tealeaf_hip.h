@@ -107,8 +107,15 @@ typedef struct {
     double ir_delta;      /* TL_FP32_IR: refresh the FP64 residual when the FP32 one falls by this factor
                              (0 < ir_delta < 1; any other value selects the default 0.1) */
     int launch;           /* TL_LAUNCH_* */
-    int reserved[7];
+    int fp32_kernels;     /* TL_FP32_IR: TL_FP32_KERNELS_* */
+    int reserved[6];
 } tl_solve_params;
+
+/* tl_solve_params.fp32_kernels: how a TL_FP32_IR iteration is split into kernels.
+ * Both reach the same FP64 tolerance; their FP32 roundings differ. */
+#define TL_FP32_KERNELS_AUTO 0   /* fused when nx is a multiple of 4, else two kernels */
+#define TL_FP32_KERNELS_TWO 1    /* two kernels and two reductions per iteration (34 B/cell) */
+#define TL_FP32_KERNELS_FUSED 2  /* one kernel and one reduction per iteration (22 B/cell); needs nx % 4 == 0 */
 
 /* replaces the reference's @info lines (CG.jl:27, Cheby.jl:60, PPCG.jl:54, Jacobi.jl:29) */
 typedef struct {
@@ -261,7 +268,8 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 #define TL_K_PPCG64 7  /* PPCG inner Chebyshev step */
 #define TL_K_JACOBI64 8 /* Jacobi sweep */
 #define TL_K_RESID64 9 /* residual! r = u0 - A u (also the FP64 refresh of TL_FP32_IR) */
-#define TL_K_COUNT 10
+#define TL_K_CGF32 10  /* FP32 inner CG iteration as one kernel (TL_FP32_KERNELS_FUSED) */
+#define TL_K_COUNT 11
 /* Bracket every period-th launch of each kernel in kernel_mask (bit =
  * 1 << TL_K_*; period >= 1) with HIP events on the context's stream (an event
  * pair costs a few microseconds of GPU time, so a period of about 16 keeps the

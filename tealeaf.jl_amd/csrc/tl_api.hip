@@ -155,7 +155,7 @@ extern "C" void tl_destroy(tl_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     unet_destroy(c);
     void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->gcell, c->gc_range, c->p32, c->r32, c->x32,
-                    c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
+                    c->r32b, c->part_f, c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
     for (int i = 0; i < TL_NFIELDS; ++i)
         if (c->f[i]) (void)hipFree(c->f[i]);
     for (void *b : bufs)
@@ -272,7 +272,7 @@ extern "C" int tl_initialise(tl_ctx *c, const tl_state *states, int nstates)
 // ------------------------------------------------------- hipGraph batches
 
 #define TL_GRAPH_CACHE 4
-enum GraphKind { GRAPH_CG64 = 0, GRAPH_CG32 = 1 };
+enum GraphKind { GRAPH_CG64 = 0, GRAPH_CG32 = 1, GRAPH_CGF = 2 };
 
 // what a captured batch of `kind` holds besides the Ctrl-resident scalars:
 // batch length, buffers, FP32 face scales and the partial-sum block counts
@@ -295,6 +295,12 @@ static std::vector<uintptr_t> graph_key(const tl_ctx *c, int kind, int k)
             v.push_back((uintptr_t)q);
         v.push_back(hx);
         v.push_back(hy);
+        if (kind == GRAPH_CGF) {
+            v.push_back((uintptr_t)c->r32b);
+            v.push_back((uintptr_t)c->part_f);
+            v.push_back(c->nb_f[0]);
+            v.push_back(c->nb_f[1]);
+        }
     }
     return v;
 }
@@ -330,6 +336,10 @@ static int graph_get(tl_ctx *c, int kind, int k, hipGraphExec_t *exec)
     hipError_t e = launch_graph_base(c, k);
     for (int i = 0; i < k && e == hipSuccess; ++i) {
         const int t = tref + i;
+        if (kind == GRAPH_CGF) {
+            e = launch_cgf(c, t, tref);
+            continue;
+        }
         e = kind == GRAPH_CG64 ? launch_cg_pw64(c, t, tref) : launch_cg_w32(c, t, tref);
         if (e == hipSuccess) e = kind == GRAPH_CG64 ? launch_cg_urv64(c, t, tref) : launch_cg_ur32(c, t, tref);
     }
@@ -437,6 +447,16 @@ static int ensure_fp32(tl_ctx *c)
     if (!c->x32) RETIF(alloc_f(c, &c->x32, n));
     return TL_OK;
 }
+
+// the fused FP32 kernel's second residual buffer and its per-block sums
+static int ensure_fused(tl_ctx *c)
+{
+    if (!c->r32b) RETIF(alloc_f(c, &c->r32b, c->g.elems));
+    if (!c->part_f) RETIF(alloc_d(c, &c->part_f, (size_t)8 * c->g.nbmax));
+    return TL_OK;
+}
+
+static bool fused_fits(const tl_ctx *c) { return c->g.nx % 4 == 0; }
 
 static int status_of(tl_ctx *c, const Ctrl &k)
 {
@@ -888,13 +908,19 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     HIPCHK(c, launch_ir_prepare(c, rx, ry, p->coefficient));
     const double delta = p->ir_delta > 0 && p->ir_delta < 1 ? p->ir_delta : 0.1;
     HIPCHK(c, launch_init_ctrl(c, MODE_IR, p->max_iters, p->eps, p->rel_tol, delta, 0, 0, p->epslim));
+    const bool fused = p->fp32_kernels == TL_FP32_KERNELS_FUSED ||
+                       (p->fp32_kernels == TL_FP32_KERNELS_AUTO && fused_fits(c));
+    if (fused) RETIF(ensure_fused(c));
     const double *guess = nullptr;
     RETIF(choose_guess(c, p, rx, ry, &guess, st));
     // u0 = b = energy*density; u = guess or b; bb, tol_abs
     HIPCHK(c, launch_cg_init_resid(c, guess));
     HIPCHK(c, hipMemsetAsync(c->x32, 0, c->g.elems * sizeof(float), c->stream));
     int t = 0;  // last iteration enqueued
+    // r = u0 - A u (FP64, stored in FP32) and p = r; fused: the four sums of
+    // iteration 1 (r and p in the buffers of iteration 0)
     HIPCHK(c, launch_ir_residual(c, 1, t + 1));
+    if (fused) HIPCHK(c, launch_cgf_dots(c, 0));
     Ctrl k;
     RETIF(read_ctrl(c, &k));
     st->init_rel_resid = k.bb > 0 ? std::sqrt(k.sums[0] / k.bb) : 0.0;
@@ -902,16 +928,29 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     while (k.stop_iter == INT_MAX || k.why == WHY_REFRESH) {
         if (k.stop_iter != INT_MAX && k.why == WHY_REFRESH) {
             t = k.stop_iter;
-            HIPCHK(c, launch_ir_accumulate(c, t));
-            HIPCHK(c, launch_ir_residual(c, 0, t + 1));
+            if (fused) {
+                // u += x (with the pending update of an odd t), the FP64
+                // residual into r(t)'s buffer, p(t) kept, and the four sums
+                HIPCHK(c, launch_ir_accumulate(c, t, cgf_p(c, t)));
+                HIPCHK(c, launch_ir_residual(c, 0, t + 1, cgf_r(c, t)));
+                HIPCHK(c, launch_cgf_dots(c, t));
+            } else {
+                HIPCHK(c, launch_ir_accumulate(c, t));
+                HIPCHK(c, launch_ir_residual(c, 0, t + 1));
+            }
             ++refreshes;
             RETIF(read_ctrl(c, &k));
             continue;
         }
-        RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return cg32_iteration(c, ++t); }, &k,
-                             p->launch == TL_LAUNCH_GRAPH ? GRAPH_CG32 : -1, &t));
+        if (fused)
+            RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return launch_cgf(c, ++t); }, &k,
+                                 p->launch == TL_LAUNCH_GRAPH ? GRAPH_CGF : -1, &t));
+        else
+            RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return cg32_iteration(c, ++t); }, &k,
+                                 p->launch == TL_LAUNCH_GRAPH ? GRAPH_CG32 : -1, &t));
     }
-    if (k.why == WHY_MAXITERS) HIPCHK(c, launch_ir_accumulate(c, k.stop_iter));  // keep the partial correction
+    if (k.why == WHY_MAXITERS)  // keep the partial correction
+        HIPCHK(c, launch_ir_accumulate(c, k.stop_iter, fused ? cgf_p(c, k.stop_iter) : nullptr));
     st->inner_iters = k.iters;
     st->iterations = k.iters;
     st->cg_iters = k.iters;
@@ -983,6 +1022,18 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
     HIPCHK(c, launch_init_coef(c, 1.0, 1.0, 1, INIT_CG));
     HIPCHK(c, launch_init_ctrl(c, MODE_CG_REF, 1 << 30, 0.0, 0.0, 0.1, 0, 0, 0.0));
     HIPCHK(c, launch_cg_init_resid(c, nullptr));
+    if (kernel_id == TL_K_CGF32) {
+        // `reps` iterations of the fused FP32 CG (after three warm-up ones)
+        if (!fused_fits(c)) return TL_E_INVALID;
+        RETIF(ensure_fp32(c));
+        RETIF(ensure_fused(c));
+        HIPCHK(c, launch_ir_prepare(c, 1.0, 1.0, 1));
+        HIPCHK(c, launch_ir_residual(c, 1, 1));
+        HIPCHK(c, launch_cgf_dots(c, 0));
+        for (int t = 1; t <= 3 + reps; ++t) HIPCHK(c, launch_cgf(c, t));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return TL_OK;
+    }
     if (kernel_id >= TL_K_CHEBY64) {
         // one FP64 solver kernel, `reps` launches on the CG-initialised fields
         // (u, u0, p, r, kx, ky) with fixed coefficients; never stopped
@@ -1046,6 +1097,11 @@ extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double r
     }
     if (p->precision == TL_FP32_IR && (p->solver != TL_CG || p->stop_rule != TL_STOP_RELATIVE)) {
         c->err = "TL_FP32_IR needs the CG solver and the relative stop rule";
+        return TL_E_INVALID;
+    }
+    if (p->fp32_kernels < TL_FP32_KERNELS_AUTO || p->fp32_kernels > TL_FP32_KERNELS_FUSED ||
+        (p->precision == TL_FP32_IR && p->fp32_kernels == TL_FP32_KERNELS_FUSED && !fused_fits(c))) {
+        c->err = "tl_solve: fp32_kernels must be TL_FP32_KERNELS_AUTO, _TWO or _FUSED (fused needs nx % 4 == 0)";
         return TL_E_INVALID;
     }
     if (p->launch != TL_LAUNCH_STREAM && p->launch != TL_LAUNCH_GRAPH) {

@@ -11,7 +11,7 @@
 // Keys this build adds (the reference parser warns about and ignores them):
 //   precision = fp64 | fp32_ir      stop_rule = reference | relative
 //   rel_tol = <float>               warm_start = none | learned
-//   ir_delta = <float>
+//   ir_delta = <float>              fp32_kernels = auto | two | fused
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -203,6 +203,16 @@ extern "C" int tl_deck_parse(const char *text, tl_deck *d)
         } else if (key == "irdelta") {
             ok = parse_float(val, &fv);
             if (ok) d->params.ir_delta = fv;
+        } else if (key == "fp32kernels") {
+            const std::string v = strip(val);
+            if (v == "auto")
+                d->params.fp32_kernels = TL_FP32_KERNELS_AUTO;
+            else if (v == "two")
+                d->params.fp32_kernels = TL_FP32_KERNELS_TWO;
+            else if (v == "fused")
+                d->params.fp32_kernels = TL_FP32_KERNELS_FUSED;
+            else
+                ok = false;
         } else if (key == "warmstart") {
             const std::string v = strip(val);
             if (v == "none")

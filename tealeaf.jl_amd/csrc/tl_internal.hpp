@@ -73,6 +73,25 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_VEC_BLOCKS64
 #define TL_VEC_BLOCKS64 8192
 #endif
+// the fused FP32 inner-CG kernel (k_cgf): rows per load batch, software
+// pipelining, block count on large grids and the least rows per band (each
+// band re-reads 2 rows below and 2 above it).  Measured at 8192^2 (A/B runs
+// on one box): 1-row batches pipelined (143 VGPRs, 3 waves per SIMD) in 768
+// blocks -- one block per 3 waves of every SIMD, a single round -- take
+// 0.314 ms per iteration; 2-row batches (188 VGPRs) 0.317 ms with 1024
+// blocks, 0.352 ms with 768; 4-row batches 0.36 ms
+#ifndef TL_RV_F32
+#define TL_RV_F32 1
+#endif
+#ifndef TL_PIPE_F32
+#define TL_PIPE_F32 1
+#endif
+#ifndef TL_VEC_BLOCKS_F32
+#define TL_VEC_BLOCKS_F32 768
+#endif
+#ifndef TL_MIN_ROWS_F32
+#define TL_MIN_ROWS_F32 32
+#endif
 // neighbour exchange of the FP32 PW kernel by DPP wave shifts (else LDS shuffles)
 #ifndef TL_DPP
 #define TL_DPP 1
@@ -133,6 +152,7 @@ struct Geo {
     int gx4, gy4, ty4;         // FP32 vector kernels: blocks of 4*TB columns x ty4 rows
     int gyp4, typ4;            // FP32 PW kernel: gx4*gyp4 blocks of 4*TB columns x typ4 rows
     int gx2, gy2, ty2;         // FP64 vector kernels: blocks of 2*TB columns x ty2 rows
+    int gyf, tyf;              // fused FP32 kernel: gx4*gyf blocks of 4*TB columns x tyf rows
     int nbmax;                 // stride of the partial-sum arrays (max block count)
     size_t elems;              // elements per allocation
 };
@@ -203,6 +223,11 @@ struct tl_ctx {
     double *cg_u = nullptr;    // the vector the fused FP64 UR adds alpha p to: u, or (relative stop
                                // rule) a correction added to u at every true-residual check
     float *p32b = nullptr;     // same for FP32
+    // fused FP32 inner CG: second residual buffer (r ping-pongs, being read
+    // at the neighbours), two sets of the four per-block sums, block counts
+    float *r32b = nullptr;
+    double *part_f = nullptr;
+    unsigned nb_f[2] = {0, 0};
     tl::Ctrl *ctrl = nullptr;        // device
     tl::Ctrl *ctrl_h = nullptr;      // pinned host mirror
     double *hist_a = nullptr, *hist_b = nullptr;  // device CG alpha/beta history
@@ -269,7 +294,11 @@ hipError_t launch_graph_seed(tl_ctx *c, int t0);
 hipError_t launch_residual(tl_ctx *c);
 hipError_t launch_cg64_true_residual(tl_ctx *c, int T);
 hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t);
-hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next);
+hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next, float *r_dst = nullptr);
+hipError_t launch_cgf(tl_ctx *c, int t, int tref = 0);
+hipError_t launch_cgf_dots(tl_ctx *c, int T);
+float *cgf_r(tl_ctx *c, int t);
+float *cgf_p(tl_ctx *c, int t);
 hipError_t launch_apply(tl_ctx *c, const double *src, double *dst);
 hipError_t launch_finalise(tl_ctx *c);
 hipError_t launch_field_summary(tl_ctx *c);
@@ -281,7 +310,7 @@ hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t);
 hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t);
 hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer);
 hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient);
-hipError_t launch_ir_accumulate(tl_ctx *c, int T);
+hipError_t launch_ir_accumulate(tl_ctx *c, int T, const float *pT = nullptr);
 hipError_t launch_acc64(tl_ctx *c, double *u, double *d);
 
 Geo make_geo(int nx, int ny, int hd);

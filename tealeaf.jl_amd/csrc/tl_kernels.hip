@@ -81,7 +81,18 @@ Geo make_geo(int nx, int ny, int hd)
     int gxp;
     vec(4, TL_VEC_BLOCKS32_PW, TL_RV_PW32, gxp, g.gyp4, g.typ4);  // FP32 PW: same columns, its own bands
     vec(2, TL_VEC_BLOCKS64, TL_RV, g.gx2, g.gy2, g.ty2);
+    {
+        // fused FP32 kernel: about TL_VEC_BLOCKS_F32 blocks, bands of at
+        // least TL_MIN_ROWS_F32 rows (a multiple of its row batch)
+        const long r2 = (long)ny * g.gx4;
+        int t2 = (int)((r2 + TL_VEC_BLOCKS_F32 - 1) / TL_VEC_BLOCKS_F32);
+        t2 = std::max(t2, std::min(TL_MIN_ROWS_F32, ny));
+        t2 = (t2 + TL_RV_F32 - 1) / TL_RV_F32 * TL_RV_F32;
+        g.tyf = std::min(t2, 1024);
+        g.gyf = (ny + g.tyf - 1) / g.tyf;
+    }
     g.nbmax = std::max(std::max(g.gx * g.gy, g.gx4 * g.gyp4), std::max(g.gx4 * g.gy4, g.gx2 * g.gy2));
+    g.nbmax = std::max(g.nbmax, g.gx4 * g.gyf);
     return g;
 }
 
@@ -1473,6 +1484,363 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgg(Geo g, int ty, int t, c
     write_partials<1, TBV>(acc, part_out, g.nbmax);
 }
 
+// ----------------------- FP32 inner CG, one kernel per iteration (fused)
+//
+// Single-reduction CG: the two global reductions of an iteration (p.Ap before
+// alpha, r.r before beta) are replaced by one, so that an iteration is one
+// sweep over the grid.  Kernel t receives from kernel t-1 (or from k_cgf_dots
+// after a start or an FP64 refresh) four sums for r = r(t-1) and p' = p(t-1):
+//   gamma = (r, r), delta = (r, A r), eps = (r, A p'), pi = (p', A p')
+// and derives, identically in every block,
+//   beta = gamma / gamma(t-2)   (0 at t = 1)
+//   (p, A p) = delta + 2 beta eps + beta^2 pi    for p = r + beta p'
+//   alpha = gamma / (p, A p).
+// (p, A p) is the expansion of (r + beta p', A (r + beta p')) and holds
+// without assuming conjugacy, so it stays exact when an FP64 refresh replaces
+// r and keeps p' (Chronopoulos and Gear's form, delta - beta gamma /
+// alpha(t-1), assumes (r, A p') = -gamma / alpha(t-1), which the replacement
+// breaks).  The kernel then applies the iteration: p = r + beta p',
+// q = A p, r(t) = r - alpha q, x += alpha p (two iterations at a time, at even
+// t, as in k_cgg), and accumulates the four sums of iteration t+1: (r(t),
+// r(t)), (r(t), A r(t)), (r(t), q), (p, q).  A r(t) needs r(t) at the
+// neighbours, hence q at the neighbours and p at distance 2: a thread walks
+// its band keeping rows L-3 .. L in registers (p at L-2..L, r(t) at L-3..L-1,
+// gc at L-3..L), takes the columns next to its 4 from the adjacent lanes by
+// DPP wave shifts, and lanes 0 / 63 load the two columns outside the wave and
+// compute q and r(t) at the first one themselves.
+// Bytes per cell: reads r, p (4 each), gc (2), writes r(t), p (4 each): 18;
+// at even t also reads and writes x: 26 (22 on average), against 34 for the
+// two-kernel iteration (k_cgg PW + UR).  Needs nx % 4 == 0 (whole 4-column
+// vectors per lane); other grids use k_cgg.
+
+// one loaded row of the fused kernel: own 4 columns of r(t-1), p(t-1), x and
+// the raw 16-bit coefficients; lanes 0 / 63 also the two columns outside
+// the wave (lane 0: x0-2, x0-1; lane 63: x0+4, x0+5)
+template <int RB>
+struct FBatch {
+    float r[RB][4], p[RB][4], x[RB][4];
+    unsigned g[RB][2];
+    float er[RB][2], ep[RB][2];
+    unsigned eg[RB];
+};
+
+__device__ __forceinline__ unsigned dpp_up1u(unsigned v) { return (unsigned)dpp_up1((int)v); }
+__device__ __forceinline__ unsigned dpp_down1u(unsigned v) { return (unsigned)dpp_down1((int)v); }
+__device__ __forceinline__ float dpp_up1f(float v) { return __int_as_float(dpp_up1(__float_as_int(v))); }
+__device__ __forceinline__ float dpp_down1f(float v) { return __int_as_float(dpp_down1(__float_as_int(v))); }
+
+// 16-bit coefficient k (0 or 1) of a raw 32-bit word
+template <int FMT>
+__device__ __forceinline__ float gc_word(unsigned w, int k)
+{
+    return gc_from_raw<FMT>(k ? (w >> 16) : (w & 0xffffu));
+}
+
+// The FP32 operator of the fused kernel, flux form with a chain of fused
+// multiply-adds: c + (kxr (c - r) + (kxl (c - l) + (kyu (c - u) + kyd (c - d))))
+__device__ __forceinline__ float smvp5f(float c, float l, float r, float d, float u, float kxl, float kxr,
+                                        float kyd, float kyu)
+{
+    float t = kyd * (c - d);
+    t = __builtin_fmaf(kyu, c - u, t);
+    t = __builtin_fmaf(kxl, c - l, t);
+    t = __builtin_fmaf(kxr, c - r, t);
+    return c + t;
+}
+
+template <int RB, bool PIPE, int FMT>
+__global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgf(Geo g, int ty, int t, const float *__restrict__ rin,
+                                                         float *__restrict__ rout, const float *__restrict__ pin,
+                                                         float *__restrict__ pout, float *__restrict__ x,
+                                                         const tl_gc_t *__restrict__ gc, float hx, float hy,
+                                                         Ctrl *__restrict__ ctrl, const double *__restrict__ sin,
+                                                         unsigned nbin, double *__restrict__ sout)
+{
+    t = iter_of(t, ctrl);
+    if (t > ctrl->stop_iter) return;
+    double s4[4];
+    sum_partials<4, TBV>(sin, nbin, g.nbmax, s4);
+    const double gam = s4[0];
+    const int why = decide_why(ctrl, t - 1, gam);
+    const double bd = t >= 2 ? gam / ctrl->rrs[(t - 2) & 1] : 0.0;
+    const double pap = s4[1] + bd * (2.0 * s4[2] + bd * s4[3]);
+    const double ad = gam / pap;
+    const bool ok = pap > 0.0 && isfinite(ad);
+    if (block_id() == 0 && threadIdx.x == 0) {
+        ctrl->rrs[(t - 1) & 1] = gam;
+        ctrl->rro = gam;
+        ctrl->err = gam;
+        ctrl->beta = bd;
+        ctrl->iters = t - 1;
+        record_stop(ctrl, t - 1, why);
+        if (why == WHY_NONE) {
+            ctrl->pw = pap;
+            ctrl->alpha = ad;
+            ctrl->alphas[t & 1] = ad;
+            if (!ok) {  // non-positive curvature: stop before this iteration's update
+                ctrl->status = TL_E_BREAKDOWN;
+                ctrl->why = WHY_BREAKDOWN;
+                ctrl->stop_iter = t - 1;
+            }
+        }
+    }
+    if (why != WHY_NONE || !ok) return;  // every block derives the same decision
+    const float beta = (float)bd, alpha = (float)ad;
+    const bool xupd = (t & 1) == 0;
+    const float alpha_prev = xupd ? (float)ctrl->alphas[(t - 1) & 1] : 0.0f;
+
+    const VCol<4> cl = vcolumn<4>(g, ty);
+    const int x0 = cl.x0, lane = cl.lane;
+    // lanes at the wave's edges compute the column next to the wave (hcol:
+    // x0-1 for lane 0, x0+4 for lane 63) from the two outside columns they
+    // load (the other one: fcol); the grid's own edges clamp (zero flux)
+    const bool eleft = lane == 0 && x0 > 0 && cl.active;
+    const bool eright = lane == 63 && x0 + 4 < g.nx && cl.active;
+    const bool eneed = eleft || eright;
+    const long eoff = lane == 0 ? -2 : 4;     // first of the two outside columns
+    const int hk = lane == 0 ? 1 : 0;         // which of the two is hcol
+    const bool at_left = x0 == 0, at_right = x0 + 4 >= g.nx;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};     // gamma, delta, eps, pi of iteration t+1
+
+    if (cl.y0 < g.ny) {
+        const int y0 = cl.y0, y1 = cl.y1;
+        // The window, rows L-3 .. L (own 4 columns; *h: hcol, *f: fcol):
+        //   p(t) at L-2, L-1, L; r(t-1) at L-1, L; r(t) at L-3, L-2, L-1;
+        //   gc at L; face coefficients kx of rows L-2, L-1 (5 faces: left of
+        //   column 0 .. right of column 3) and ky between rows L-3|L-2,
+        //   L-2|L-1, L-1|L.  Faces are formed once per row and serve both
+        //   A p(t) (row L-1) and, a row later, A r(t).
+        float P0[4], P1[4], P2[4], P0h = 0.f, P1h = 0.f, P1f = 0.f, P2h = 0.f, P2f = 0.f;
+        float R1[4], R2[4], R1h = 0.f, R2h = 0.f;
+        float N0[4], N1[4], N2[4], N1h = 0.f, N2h = 0.f;
+        float Gc[4], Gch = 0.f, Gcf = 0.f;           // gc of row L (own, hcol, fcol)
+        unsigned Gw[2] = {0u, 0u};                   // its raw words (for the neighbour lanes)
+        float KX1[5], KX2[5], KX1o = 0.f;            // kx of rows L-1, L-2; KX1o: fcol|hcol face of row L-1
+        float KY0[4], KY1[4], KY2[4], KY1h = 0.f, KY2h = 0.f;  // ky L-3|L-2, L-2|L-1, L-1|L (hcol: 1, 2)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            P0[j] = P1[j] = P2[j] = R1[j] = R2[j] = N0[j] = N1[j] = N2[j] = 0.f;
+            Gc[j] = KY0[j] = KY1[j] = KY2[j] = 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) KX1[j] = KX2[j] = 0.f;
+
+        auto load_batch = [&](FBatch<RB> &B, int yb) {
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int yy = min(max(yb + q, 0), g.ny - 1);
+                const long i = at(g, cl.xl, yy);
+                vload<float, 4>(B.r[q], rin + i);
+                vload<float, 4>(B.p[q], pin + i);
+                {
+                    typedef unsigned uv2 __attribute__((ext_vector_type(2)));
+                    const uv2 w = *reinterpret_cast<const uv2 *>(gc + i);
+                    B.g[q][0] = w[0];
+                    B.g[q][1] = w[1];
+                }
+                if (xupd) vload<float, 4>(B.x[q], x + i);
+                B.er[q][0] = B.er[q][1] = B.ep[q][0] = B.ep[q][1] = 0.f;
+                B.eg[q] = 0u;
+                if (eneed) {
+                    const long ie = i + eoff;
+                    B.er[q][0] = rin[ie];
+                    B.er[q][1] = rin[ie + 1];
+                    B.ep[q][0] = pin[ie];
+                    B.ep[q][1] = pin[ie + 1];
+                    B.eg[q] = (unsigned)gc_raw(gc[ie]) | ((unsigned)gc_raw(gc[ie + 1]) << 16);
+                }
+            }
+        };
+
+        // the value left / right of the own columns from the adjacent lane;
+        // the edge lanes take their hcol value, the grid edges the cell itself
+        auto nbr_l = [&](float src3, float c0, float hval) -> float {
+            float v = dpp_up1f(src3);
+            if (eleft) v = hval;
+            if (at_left) v = c0;
+            return v;
+        };
+        auto nbr_r = [&](float src0, float c3, float hval) -> float {
+            float v = dpp_down1f(src0);
+            if (eright) v = hval;
+            if (at_right) v = c3;
+            return v;
+        };
+
+        auto compute_batch = [&](const FBatch<RB> &B, int yb) {
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int L = yb + q;
+                if (L > y1 + 1) break;
+                // shift the window by one row
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    P0[j] = P1[j];
+                    P1[j] = P2[j];
+                    R1[j] = R2[j];
+                    N0[j] = N1[j];
+                    N1[j] = N2[j];
+                    KY0[j] = KY1[j];
+                    KY1[j] = KY2[j];
+                }
+#pragma unroll
+                for (int j = 0; j < 5; ++j) KX2[j] = KX1[j];
+                P0h = P1h;
+                P1h = P2h;
+                P1f = P2f;
+                R1h = R2h;
+                N1h = N2h;
+                KY1h = KY2h;
+                // faces of row L-1 from its coefficients (still in Gc) and row L's
+                float gL[4];
+                unsigned wL[2] = {B.g[q][0], B.g[q][1]};
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    gL[2 * k] = gc_word<FMT>(wL[k], 0);
+                    gL[2 * k + 1] = gc_word<FMT>(wL[k], 1);
+                }
+                const float gLh = gc_word<FMT>(B.eg[q], hk), gLf = gc_word<FMT>(B.eg[q], 1 - hk);
+                {
+                    // kx of row L-1: neighbours' coefficients by DPP of the raw words
+                    float gl = gc_word<FMT>(dpp_up1u(Gw[1]), 1), gr = gc_word<FMT>(dpp_down1u(Gw[0]), 0);
+                    if (eleft) gl = Gch;
+                    if (eright) gr = Gch;
+                    KX1[0] = hx * (gl + Gc[0]);
+#pragma unroll
+                    for (int j = 1; j < 4; ++j) KX1[j] = hx * (Gc[j - 1] + Gc[j]);
+                    KX1[4] = hx * (Gc[3] + gr);
+                    KX1o = lane == 0 ? hx * (Gcf + Gch) : hx * (Gch + Gcf);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) KY2[j] = hy * (Gc[j] + gL[j]);
+                    KY2h = hy * (Gch + gLh);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) Gc[j] = gL[j];
+                Gch = gLh;
+                Gcf = gLf;
+                Gw[0] = wL[0];
+                Gw[1] = wL[1];
+                // row L: r(t-1) and p(t) = r + beta p(t-1)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    R2[j] = B.r[q][j];
+                    P2[j] = __builtin_fmaf(beta, B.p[q][j], B.r[q][j]);
+                }
+                {
+                    const float rh = B.er[q][hk], rf = B.er[q][1 - hk];
+                    P2h = __builtin_fmaf(beta, B.ep[q][hk], rh);
+                    P2f = __builtin_fmaf(beta, B.ep[q][1 - hk], rf);
+                    R2h = rh;
+                }
+                if (L >= y0 && L < y1 && cl.active) {
+                    const long i = at(g, x0, L);
+                    vstore<float, 4>(pout + i, P2);
+                    if (xupd) {
+                        float xn[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            xn[j] = __builtin_fmaf(alpha, P2[j], __builtin_fmaf(alpha_prev, B.p[q][j], B.x[q][j]));
+                        vstore<float, 4>(x + i, xn);
+                    }
+                }
+                // q = A p(t) and r(t) = r(t-1) - alpha q on row L-1
+                if (L >= y0) {
+                    const int y = L - 1;
+                    const float pl = nbr_l(P1[3], P1[0], P1h), pr = nbr_r(P1[0], P1[3], P1h);
+                    const bool ybot = y <= 0, ytop = y >= g.ny - 1;
+                    float qv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float c = P1[j];
+                        const float l = j > 0 ? P1[j - 1] : pl;
+                        const float r = j < 3 ? P1[j + 1] : pr;
+                        const float d = ybot ? c : P0[j];
+                        const float u = ytop ? c : P2[j];
+                        qv[j] = smvp5f(c, l, r, d, u, KX1[j], KX1[j + 1], KY1[j], KY2[j]);
+                        N2[j] = __builtin_fmaf(-alpha, qv[j], R1[j]);
+                    }
+                    {
+                        // hcol of the edge lanes (fcol lies inside the grid when needed)
+                        const float c = P1h;
+                        const float own = lane == 0 ? P1[0] : P1[3];
+                        const float l = lane == 0 ? P1f : own, r = lane == 0 ? own : P1f;
+                        const float kxl = lane == 0 ? KX1o : KX1[4], kxr = lane == 0 ? KX1[0] : KX1o;
+                        const float d = ybot ? c : P0h, u = ytop ? c : P2h;
+                        const float qh = smvp5f(c, l, r, d, u, kxl, kxr, KY1h, KY2h);
+                        N2h = __builtin_fmaf(-alpha, qh, R1h);
+                    }
+                    if (y >= y0 && y < y1 && cl.active) {
+                        vstore<float, 4>(rout + at(g, x0, y), N2);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const double qd = (double)qv[j];
+                            acc[2] += (double)N2[j] * qd;
+                            acc[3] += (double)P1[j] * qd;
+                        }
+                    }
+                }
+                // A r(t) on row L-2, and the sums of r(t) there
+                if (L >= y0 + 2) {
+                    const int y = L - 2;
+                    const float nl = nbr_l(N1[3], N1[0], N1h), nr = nbr_r(N1[0], N1[3], N1h);
+                    const bool ybot = y <= 0, ytop = y >= g.ny - 1;
+                    if (y < y1 && cl.active) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const float c = N1[j];
+                            const float l = j > 0 ? N1[j - 1] : nl;
+                            const float r = j < 3 ? N1[j + 1] : nr;
+                            const float d = ybot ? c : N0[j];
+                            const float u = ytop ? c : N2[j];
+                            const float ar = smvp5f(c, l, r, d, u, KX2[j], KX2[j + 1], KY0[j], KY1[j]);
+                            const double cd = (double)c;
+                            acc[0] += cd * cd;
+                            acc[1] += cd * (double)ar;
+                        }
+                    }
+                }
+            }
+        };
+        row_batches<PIPE, RB, FBatch<RB>>(y0 - 2, y1 + 2, load_batch, compute_batch);
+    }
+    write_partials<4, TBV>(acc, sout, g.nbmax);
+}
+
+// The four sums k_cgf needs at its first iteration and after an FP64 refresh,
+// for the residual r and the kept direction p: (r, r), (r, A r), (r, A p),
+// (p, A p), with the same FP32 operator (flux form, clamped neighbours).  One
+// thread per cell of a column band; runs once per start or refresh.
+template <int FMT>
+__global__ __launch_bounds__(TB) void k_cgf_dots(Geo g, const float *__restrict__ r, const float *__restrict__ p,
+                                                 const tl_gc_t *__restrict__ gc, float hx, float hy,
+                                                 double *__restrict__ part)
+{
+    const Col c = column(g);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c.active) {
+        const int x = c.x;
+        const int xl = max(x - 1, 0), xr = min(x + 1, g.nx - 1);
+        auto G = [&](int xx, int yy) -> float { return gc_from_raw<FMT>(gc_raw(gc[at(g, xx, yy)])); };
+        for (int y = c.y0; y < c.y1; ++y) {
+            const int yd = max(y - 1, 0), yu = min(y + 1, g.ny - 1);
+            const float gm = G(x, y), gL = G(xl, y), gR = G(xr, y), gD = G(x, yd), gU = G(x, yu);
+            const float kxl = hx * (gL + gm), kxr = hx * (gm + gR), kyd = hy * (gD + gm), kyu = hy * (gm + gU);
+            const long i = at(g, x, y);
+            const float rc = r[i], pc = p[i];
+            const float ar = smvp5f(rc, r[at(g, xl, y)], r[at(g, xr, y)], r[at(g, x, yd)], r[at(g, x, yu)], kxl,
+                                    kxr, kyd, kyu);
+            const float ap = smvp5f(pc, p[at(g, xl, y)], p[at(g, xr, y)], p[at(g, x, yd)], p[at(g, x, yu)], kxl,
+                                    kxr, kyd, kyu);
+            const double rd = (double)rc;
+            acc[0] += rd * rd;
+            acc[1] += rd * (double)ar;
+            acc[2] += rd * (double)ap;
+            acc[3] += (double)pc * (double)ap;
+        }
+    }
+    write_partials<4>(acc, part, g.nbmax);
+}
+
 // ---------------------------------------------- residuals and summaries
 
 // residual!, kernels.jl:227-232: r = u0 - A u.  Optional outputs: r (FP64),
@@ -2196,12 +2564,12 @@ hipError_t launch_jacobi_residual(tl_ctx *c, const double *u, int t)
     return e == hipSuccess ? fin(c, part_m(c), nbh(c), 2, 0, FK_JACOBI, t) : e;
 }
 
-hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next)
+hipError_t launch_ir_residual(tl_ctx *c, int reset_p, int t_next, float *r_dst)
 {
     const bool prof = prof_start(c, TL_K_RESID64);
     k_residual<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, 0, c->f[TL_F_U0], c->f[TL_F_U], c->f[TL_F_KX],
-                                                  c->f[TL_F_KY], nullptr, c->r32, reset_p ? c->p32 : nullptr,
-                                                  c->ctrl, part_r(c, t_next - 1));
+                                                  c->f[TL_F_KY], nullptr, r_dst ? r_dst : c->r32,
+                                                  reset_p ? c->p32 : nullptr, c->ctrl, part_r(c, t_next - 1));
     if (prof) prof_stop(c);
     nbr(c, t_next - 1) = nbh(c);
     hipError_t e = hipGetLastError();
@@ -2361,10 +2729,40 @@ hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient)
     return hipGetLastError();
 }
 
-hipError_t launch_ir_accumulate(tl_ctx *c, int T)
+hipError_t launch_ir_accumulate(tl_ctx *c, int T, const float *pT)
 {
-    k_ir_accumulate<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U], c->x32, T >= 1 ? p32(c, T) : nullptr,
-                                                       c->ctrl, T);
+    k_ir_accumulate<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, c->f[TL_F_U], c->x32,
+                                                       T >= 1 ? (pT ? pT : p32(c, T)) : nullptr, c->ctrl, T);
+    return hipGetLastError();
+}
+
+// fused FP32 inner CG: r(t) in buffer t & 1, p(t) in buffer t & 1, the four
+// per-block sums for iteration t+1 in set t & 1
+float *cgf_r(tl_ctx *c, int t) { return (t & 1) ? c->r32b : c->r32; }
+float *cgf_p(tl_ctx *c, int t) { return (t & 1) ? c->p32b : c->p32; }
+static inline double *part_f(tl_ctx *c, int t) { return c->part_f + (size_t)4 * (t & 1) * stride(c); }
+static inline unsigned nbf(const tl_ctx *c) { return (unsigned)(c->g.gx4 * c->g.gyf); }
+
+hipError_t launch_cgf(tl_ctx *c, int t, int tref)
+{
+    const bool prof = prof_start(c, TL_K_CGF32);
+    auto kern = c->gc_fmt == GC_F16 ? k_cgf<TL_RV_F32, TL_PIPE_F32 != 0, GC_F16>
+                                    : k_cgf<TL_RV_F32, TL_PIPE_F32 != 0, GC_BF16>;
+    kern<<<nbf(c), TBV, 0, c->stream>>>(c->g, c->g.tyf, targ(t, tref), cgf_r(c, t - 1), cgf_r(c, t),
+                                        cgf_p(c, t - 1), cgf_p(c, t), c->x32, c->gcell, c->hx32, c->hy32, c->ctrl,
+                                        part_f(c, t - 1), c->nb_f[(t - 1) & 1], part_f(c, t));
+    c->nb_f[t & 1] = nbf(c);
+    if (prof) prof_stop(c);
+    return hipGetLastError();
+}
+
+// the four sums of r(T) and p(T) for iteration T+1 (after a start or a refresh)
+hipError_t launch_cgf_dots(tl_ctx *c, int T)
+{
+    auto kern = c->gc_fmt == GC_F16 ? k_cgf_dots<GC_F16> : k_cgf_dots<GC_BF16>;
+    kern<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, cgf_r(c, T), cgf_p(c, T), c->gcell, c->hx32, c->hy32,
+                                            part_f(c, T));
+    c->nb_f[T & 1] = nbh(c);
     return hipGetLastError();
 }
 

@@ -38,12 +38,14 @@ EXPORTS = [
 ]
 TL_START_PREVIOUS, TL_START_WARM = 0, 1
 (TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32, TL_K_CHEBY64, TL_K_PPCG64,
- TL_K_JACOBI64, TL_K_RESID64) = range(10)
-TL_K_COUNT = 10
+ TL_K_JACOBI64, TL_K_RESID64, TL_K_CGF32) = range(11)
+TL_K_COUNT = 11
+TL_FP32_KERNELS_AUTO, TL_FP32_KERNELS_TWO, TL_FP32_KERNELS_FUSED = 0, 1, 2
 KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, false>", 2: "k_cg_p<double>",
                 3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
                 5: "(fused into k_cgg<4, false, 2, true, 1>)",
                 6: "k_cheby_iter", 7: "k_ppcg_inner", 8: "k_jacobi", 9: "k_residual",
+                10: "k_cgf<1, true, 1>",
                 -1: "k_copy_full", -2: "k_copy_vec"}
 
 
@@ -53,7 +55,7 @@ def kernel_name(kid, coef_format=None):
     0 = bfloat16), which the solver picks per solve (`SolveStats.coef_format`);
     without a format the half instantiation is named."""
     name = KERNEL_NAMES[kid]
-    if coef_format == TL_COEF_BF16 and kid in (TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32):
+    if coef_format == TL_COEF_BF16 and kid in (TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32, TL_K_CGF32):
         name = name.replace(", 1>", ", 0>")
     return name
 
@@ -63,7 +65,8 @@ class SolveParams(C.Structure):
         ("solver", C.c_int), ("coefficient", C.c_int), ("max_iters", C.c_int), ("presteps", C.c_int),
         ("ppcg_inner_steps", C.c_int), ("error_switch", C.c_int), ("eps", C.c_double), ("epslim", C.c_double),
         ("precision", C.c_int), ("stop_rule", C.c_int), ("rel_tol", C.c_double), ("warm_start", C.c_int),
-        ("check_result", C.c_int), ("ir_delta", C.c_double), ("launch", C.c_int), ("reserved", C.c_int * 7),
+        ("check_result", C.c_int), ("ir_delta", C.c_double), ("launch", C.c_int), ("fp32_kernels", C.c_int),
+        ("reserved", C.c_int * 6),
     ]
 
 

@@ -74,8 +74,13 @@ def test_parser_mixed_precision_keys():
     s = T.Settings.from_text(base + "precision = fp32_ir\nir_delta=0.3\nrel_tol=1e-9\nstop_rule=relative\n")
     assert s.precision == T.TL_FP32_IR and s.ir_delta == 0.3 and s.rel_tol == 1e-9
     assert s.stop_rule == T.TL_STOP_RELATIVE
+    assert s._deck.params.fp32_kernels == _lib.TL_FP32_KERNELS_AUTO
+    for v, code in (("two", _lib.TL_FP32_KERNELS_TWO), ("fused", _lib.TL_FP32_KERNELS_FUSED),
+                    ("auto", _lib.TL_FP32_KERNELS_AUTO)):
+        assert T.Settings.from_text(base + f"fp32_kernels={v}\n")._deck.params.fp32_kernels == code
     n0 = T.Settings.from_text(base).n_warnings
-    for bad in ("precision=fp16\n", "ir_delta=abc\n", "stop_rule=sometimes\n", "warm_start=maybe\n"):
+    for bad in ("precision=fp16\n", "ir_delta=abc\n", "stop_rule=sometimes\n", "warm_start=maybe\n",
+                "fp32_kernels=three\n"):
         assert T.Settings.from_text(base + bad).n_warnings == n0 + 1, bad  # warned and ignored
 
 

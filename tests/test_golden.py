@@ -101,3 +101,26 @@ def test_oracle_reproduces_fp64_rectangle_fixture():
     assert np.all(s[:, 0] == s[0, 0]) and np.all(s[:, 1] == s[0, 1])
     np.testing.assert_allclose(s[:, 2], s[0, 2], rtol=1e-9)
 
+
+def test_c2_fixture_invariants():
+    """The 4096^2 fixture (too costly to recompute here): vol and mass fixed,
+    ie conserved (1^T A = 1^T under reflection), the three solvers agree
+    within their tolerances at every step they share, and the first step's CG
+    pre-steps of Chebyshev and PPCG are the same solve."""
+    g = _load("c2_4096.json")
+    ref = None
+    for solver in ("cg", "cheby", "ppcg"):
+        if solver not in g:
+            continue
+        s = np.array(g[solver]["summary"])
+        assert np.all(s[:, 0] == s[0, 0]) and np.all(s[:, 1] == s[0, 1])
+        np.testing.assert_allclose(s[:, 2], s[0, 2], rtol=1e-9)
+        assert np.all(np.diff(s[:, 3]) != 0)
+        if ref is None:
+            ref = s
+        else:
+            k = min(len(ref), len(s))
+            np.testing.assert_allclose(s[:k], ref[:k], rtol=1e-8)
+    assert "cheby" in g and "ppcg" in g
+    # the first step's CG phase is the same computation for both
+    assert g["cheby"]["stats"][0]["cg_iters"] == g["ppcg"]["stats"][0]["cg_iters"]

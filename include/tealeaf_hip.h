@@ -202,6 +202,21 @@ int tl_solve(tl_ctx *ctx, const tl_solve_params *p, double rx, double ry, tl_sol
 int tl_solve_finished(tl_ctx *ctx, int check_result);
 /* fieldsummary, kernels.jl:119-133: out = {vol, mass, ie, temp} */
 int tl_field_summary(tl_ctx *ctx, double out[4]);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc): tl_field_summary of caller-owned DEVICE fields in the host
+ * layout ((nx + 2*halo_depth) * (ny + 2*halo_depth) doubles, x fastest),
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream), writing
+ * {vol, mass, ie, temp} to out_dev (4 doubles of device memory).  No context
+ * and no allocation: the per-block partial sums live in a static device array
+ * of the library (256 KiB per GPU, held until the process ends), so the grid
+ * may have at most 8192 summation blocks (256-column strips x row bands of up
+ * to 1024 rows: any grid up to 32768^2); larger grids return TL_E_INVALID.
+ * The same kernels and summation order as tl_field_summary, so the results
+ * are bit-identical.  All calls on one GPU share that array: calls that
+ * overlap in time on one GPU (other streams, other threads) race. */
+int tl_field_summary_noalloc(const double *density, const double *energy, const double *u, int nx, int ny,
+                             int halo_depth, double dx, double dy, double *out_dev, void *stream);
 /* one timestep of diffuse!, TeaLeaf.jl:67-81 (dt = Settings.dtinit) */
 int tl_step(tl_ctx *ctx, const tl_solve_params *p, double dt, tl_solve_stats *out);
 

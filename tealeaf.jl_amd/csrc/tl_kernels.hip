@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <mutex>
 
 #include "tl_internal.hpp"
 
@@ -2767,3 +2768,80 @@ hipError_t launch_cgf_dots(tl_ctx *c, int T)
 }
 
 }  // namespace tl
+
+// ------------------------------------------- field summary without allocation
+//
+// tl_field_summary_noalloc: the field summary of caller-owned device fields
+// in the host layout (X*Y doubles, x fastest), for callers that cannot
+// allocate.  Its per-block partial sums live in `scratch`, a static device
+// array of the library (one copy per GPU, held until the process ends).
+// Host code never passes the array's name to a kernel: it is a host-side
+// stand-in there.  The array's device address is taken once per device with
+// hipGetSymbolAddress and kept in noalloc_scratch_ptr.
+
+namespace {
+
+constexpr int NOALLOC_MAX_BLOCKS = 8192;  // partial sums of 4 quantities for up to this many blocks
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+static __device__ __align__(256) double scratch[4 * NOALLOC_MAX_BLOCKS];
+
+// sum the 4 partial arrays (nb long, stride NOALLOC_MAX_BLOCKS) in the order
+// k_fin uses for the context's field summary, into out[0..3]
+__global__ __launch_bounds__(tl::TB) void k_sum4_out(const double *__restrict__ part, unsigned nb,
+                                                     double *__restrict__ out)
+{
+    double a[2], b[2];
+    tl::sum_partials<2, tl::TB, true>(part, nb, NOALLOC_MAX_BLOCKS, a);
+    tl::sum_partials<2, tl::TB, true>(part + 2 * (size_t)NOALLOC_MAX_BLOCKS, nb, NOALLOC_MAX_BLOCKS, b);
+    if (threadIdx.x == 0) {
+        out[0] = a[0];
+        out[1] = a[1];
+        out[2] = b[0];
+        out[3] = b[1];
+    }
+}
+
+std::mutex noalloc_mu;
+double *noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// device address of this device's `scratch` (the current device must be dev)
+hipError_t noalloc_scratch_ptr(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        noalloc_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = noalloc_ptr[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int tl_field_summary_noalloc(const double *density, const double *energy, const double *u, int nx,
+                                        int ny, int halo_depth, double dx, double dy, double *out_dev,
+                                        void *stream)
+{
+    if (!density || !energy || !u || !out_dev || nx < 1 || ny < 1 || halo_depth < 1 || halo_depth > 16)
+        return TL_E_INVALID;
+    // the layout of the caller's arrays: pitch X, interior (0, 0) at (hd, hd)
+    tl::Geo g = tl::make_geo(nx, ny, halo_depth);
+    g.pitch = g.X;
+    g.off = (long)halo_depth * g.X + halo_depth;
+    const long nb = (long)g.gx * g.gy;
+    if (nb > NOALLOC_MAX_BLOCKS) return TL_E_INVALID;  // beyond the static partial-sum array
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return TL_E_HIP;
+    double *part = nullptr;
+    if (noalloc_scratch_ptr(dev, &part) != hipSuccess) return TL_E_HIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // one partial per block and quantity at part[q * NOALLOC_MAX_BLOCKS + block]
+    g.nbmax = NOALLOC_MAX_BLOCKS;
+    tl::k_field_summary<<<dim3(g.gx, g.gy), tl::TB, 0, s>>>(g, density, energy, u, dx * dy, part);
+    if (hipGetLastError() != hipSuccess) return TL_E_HIP;
+    k_sum4_out<<<1, tl::TB, 0, s>>>(part, (unsigned)nb, out_dev);
+    return hipGetLastError() == hipSuccess ? TL_OK : TL_E_HIP;
+}

@@ -35,6 +35,7 @@ EXPORTS = [
     "tl_unet_set_weights", "tl_unet_get_weights", "tl_unet_record", "tl_unet_dataset_size", "tl_unet_train",
     "tl_unet_predict", "tl_profile_begin", "tl_profile_read", "tl_profile_end",
     "tl_kernel_repeat", "tl_unet_eval", "tl_checking_value", "tl_qa_check", "tl_check_solution",
+    "tl_field_summary_noalloc",
 ]
 TL_START_PREVIOUS, TL_START_WARM = 0, 1
 (TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32, TL_K_CHEBY64, TL_K_PPCG64,
@@ -168,6 +169,8 @@ def load():
     L.tl_kernel_repeat.argtypes = [vp, C.c_int, C.c_int]
     L.tl_checking_value.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, dp, ip]
     L.tl_qa_check.argtypes = [C.c_double, C.c_double, dp]
+    L.tl_field_summary_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                           C.c_double, C.c_double, C.c_void_p, C.c_void_p]
     L.tl_check_solution.argtypes = [C.POINTER(SolveParams), C.POINTER(SolveStats), C.c_double, C.c_double,
                                     C.c_double]
     _lib = L

@@ -73,18 +73,15 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_VEC_BLOCKS64
 #define TL_VEC_BLOCKS64 8192
 #endif
-// the fused FP32 inner-CG kernel (k_cgf): rows per load batch, software
-// pipelining, block count on large grids and the least rows per band (each
-// band re-reads 2 rows below and 2 above it).  Measured at 8192^2 (A/B runs
+// the fused FP32 inner-CG kernel (k_cgf, software-pipelined): rows per load
+// batch, block count on large grids and the least rows per band (each band
+// re-reads 2 rows below and 2 above it).  Measured at 8192^2 (A/B runs
 // on one box): 1-row batches pipelined (143 VGPRs, 3 waves per SIMD) in 768
 // blocks -- one block per 3 waves of every SIMD, a single round -- take
 // 0.314 ms per iteration; 2-row batches (188 VGPRs) 0.317 ms with 1024
 // blocks, 0.352 ms with 768; 4-row batches 0.36 ms
 #ifndef TL_RV_F32
 #define TL_RV_F32 1
-#endif
-#ifndef TL_PIPE_F32
-#define TL_PIPE_F32 1
 #endif
 #ifndef TL_VEC_BLOCKS_F32
 #define TL_VEC_BLOCKS_F32 768
@@ -115,6 +112,12 @@ constexpr int TBV = TL_TBV;
 #define TL_CGG_ATTR __attribute__((amdgpu_waves_per_eu(TL_WPE32)))
 #else
 #define TL_CGG_ATTR
+#endif
+// the same for the fused FP32 kernel
+#ifdef TL_WPE_F32
+#define TL_CGF_ATTR __attribute__((amdgpu_waves_per_eu(TL_WPE_F32)))
+#else
+#define TL_CGF_ATTR
 #endif
 // storage of the FP32 inner CG's cell coefficient gc: 16 bits (1) or FP32
 // (0).  16-bit storage is IEEE half with a power-of-two scale when the

@@ -1549,8 +1549,8 @@ __device__ __forceinline__ float smvp5f(float c, float l, float r, float d, floa
     return c + t;
 }
 
-template <int RB, bool PIPE, int FMT>
-__global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgf(Geo g, int ty, int t, const float *__restrict__ rin,
+template <int RB, int FMT>
+__global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, const float *__restrict__ rin,
                                                          float *__restrict__ rout, const float *__restrict__ pin,
                                                          float *__restrict__ pout, float *__restrict__ x,
                                                          const tl_gc_t *__restrict__ gc, float hx, float hy,
@@ -1559,6 +1559,59 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgf(Geo g, int ty, int t, c
 {
     t = iter_of(t, ctrl);
     if (t > ctrl->stop_iter) return;
+    const bool xupd = (t & 1) == 0;
+    const VCol<4> cl = vcolumn<4>(g, ty);
+    const int x0 = cl.x0, lane = cl.lane;
+    // lanes at the wave's edges compute the column next to the wave (hcol:
+    // x0-1 for lane 0, x0+4 for lane 63) from the two outside columns they
+    // load (the other one: fcol); the grid's own edges clamp (zero flux)
+    const bool eleft = lane == 0 && x0 > 0 && cl.active;
+    const bool eright = lane == 63 && x0 + 4 < g.nx && cl.active;
+    const bool eneed = eleft || eright;
+    const long eoff = lane == 0 ? -2 : 4;     // first of the two outside columns
+    const int hk = lane == 0 ? 1 : 0;         // which of the two is hcol
+    const bool at_left = x0 == 0, at_right = x0 + 4 >= g.nx;
+
+    auto load_batch = [&](FBatch<RB> &B, int yb) {
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int yy = min(max(yb + q, 0), g.ny - 1);
+            const long i = at(g, cl.xl, yy);
+            vload<float, 4>(B.r[q], rin + i);
+            vload<float, 4>(B.p[q], pin + i);
+            {
+                typedef unsigned uv2 __attribute__((ext_vector_type(2)));
+                const uv2 w = *reinterpret_cast<const uv2 *>(gc + i);
+                B.g[q][0] = w[0];
+                B.g[q][1] = w[1];
+            }
+            if (xupd) vload<float, 4>(B.x[q], x + i);
+            B.er[q][0] = B.er[q][1] = B.ep[q][0] = B.ep[q][1] = 0.f;
+            B.eg[q] = 0u;
+            if (eneed) {
+                // the two outside columns start at an even element (x0 - 2 or
+                // x0 + 4): one 8-byte load each of r and p, one 32-bit word of gc
+                typedef float fv2 __attribute__((ext_vector_type(2)));
+                const long ie = i + eoff;
+                const fv2 er = *reinterpret_cast<const fv2 *>(rin + ie);
+                const fv2 ep = *reinterpret_cast<const fv2 *>(pin + ie);
+                B.er[q][0] = er[0];
+                B.er[q][1] = er[1];
+                B.ep[q][0] = ep[0];
+                B.ep[q][1] = ep[1];
+#if TL_GC16
+                B.eg[q] = *reinterpret_cast<const unsigned *>(gc + ie);
+#else
+                B.eg[q] = (unsigned)gc_raw(gc[ie]) | ((unsigned)gc_raw(gc[ie + 1]) << 16);
+#endif
+            }
+        }
+    };
+
+    // the band's first rows are requested before the block sums the previous
+    // iteration's partials, so the reduction overlaps their latency
+    FBatch<RB> B0, B1;
+    load_batch(B0, cl.y0 - 2);
     double s4[4];
     sum_partials<4, TBV>(sin, nbin, g.nbmax, s4);
     const double gam = s4[0];
@@ -1587,24 +1640,11 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgf(Geo g, int ty, int t, c
     }
     if (why != WHY_NONE || !ok) return;  // every block derives the same decision
     const float beta = (float)bd, alpha = (float)ad;
-    const bool xupd = (t & 1) == 0;
     const float alpha_prev = xupd ? (float)ctrl->alphas[(t - 1) & 1] : 0.0f;
-
-    const VCol<4> cl = vcolumn<4>(g, ty);
-    const int x0 = cl.x0, lane = cl.lane;
-    // lanes at the wave's edges compute the column next to the wave (hcol:
-    // x0-1 for lane 0, x0+4 for lane 63) from the two outside columns they
-    // load (the other one: fcol); the grid's own edges clamp (zero flux)
-    const bool eleft = lane == 0 && x0 > 0 && cl.active;
-    const bool eright = lane == 63 && x0 + 4 < g.nx && cl.active;
-    const bool eneed = eleft || eright;
-    const long eoff = lane == 0 ? -2 : 4;     // first of the two outside columns
-    const int hk = lane == 0 ? 1 : 0;         // which of the two is hcol
-    const bool at_left = x0 == 0, at_right = x0 + 4 >= g.nx;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};     // gamma, delta, eps, pi of iteration t+1
 
-    if (cl.y0 < g.ny) {
-        const int y0 = cl.y0, y1 = cl.y1;
+    {
+        const int y0 = cl.y0, y1 = cl.y1;  // y0 < ny: the grid has ceil(ny / ty) bands
         // The window, rows L-3 .. L (own 4 columns; *h: hcol, *f: fcol):
         //   p(t) at L-2, L-1, L; r(t-1) at L-1, L; r(t) at L-3, L-2, L-1;
         //   gc at L; face coefficients kx of rows L-2, L-1 (5 faces: left of
@@ -1625,33 +1665,6 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgf(Geo g, int ty, int t, c
         }
 #pragma unroll
         for (int j = 0; j < 5; ++j) KX1[j] = KX2[j] = 0.f;
-
-        auto load_batch = [&](FBatch<RB> &B, int yb) {
-#pragma unroll
-            for (int q = 0; q < RB; ++q) {
-                const int yy = min(max(yb + q, 0), g.ny - 1);
-                const long i = at(g, cl.xl, yy);
-                vload<float, 4>(B.r[q], rin + i);
-                vload<float, 4>(B.p[q], pin + i);
-                {
-                    typedef unsigned uv2 __attribute__((ext_vector_type(2)));
-                    const uv2 w = *reinterpret_cast<const uv2 *>(gc + i);
-                    B.g[q][0] = w[0];
-                    B.g[q][1] = w[1];
-                }
-                if (xupd) vload<float, 4>(B.x[q], x + i);
-                B.er[q][0] = B.er[q][1] = B.ep[q][0] = B.ep[q][1] = 0.f;
-                B.eg[q] = 0u;
-                if (eneed) {
-                    const long ie = i + eoff;
-                    B.er[q][0] = rin[ie];
-                    B.er[q][1] = rin[ie + 1];
-                    B.ep[q][0] = pin[ie];
-                    B.ep[q][1] = pin[ie + 1];
-                    B.eg[q] = (unsigned)gc_raw(gc[ie]) | ((unsigned)gc_raw(gc[ie + 1]) << 16);
-                }
-            }
-        };
 
         // the value left / right of the own columns from the adjacent lane;
         // the edge lanes take their hcol value, the grid edges the cell itself
@@ -1802,7 +1815,16 @@ __global__ __launch_bounds__(TBV) TL_CGG_ATTR void k_cgf(Geo g, int ty, int t, c
                 }
             }
         };
-        row_batches<PIPE, RB, FBatch<RB>>(y0 - 2, y1 + 2, load_batch, compute_batch);
+        // rows y0-2 .. y1+1 in batches of RB, the next batch's loads issued
+        // before the current batch is computed (the first one above)
+        for (int yb = y0 - 2; yb < y1 + 2; yb += 2 * RB) {
+            const bool more = yb + RB < y1 + 2;
+            if (more) load_batch(B1, yb + RB);
+            compute_batch(B0, yb);
+            if (!more) break;
+            if (yb + 2 * RB < y1 + 2) load_batch(B0, yb + 2 * RB);
+            compute_batch(B1, yb + RB);
+        }
     }
     write_partials<4, TBV>(acc, sout, g.nbmax);
 }
@@ -2747,8 +2769,7 @@ static inline unsigned nbf(const tl_ctx *c) { return (unsigned)(c->g.gx4 * c->g.
 hipError_t launch_cgf(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CGF32);
-    auto kern = c->gc_fmt == GC_F16 ? k_cgf<TL_RV_F32, TL_PIPE_F32 != 0, GC_F16>
-                                    : k_cgf<TL_RV_F32, TL_PIPE_F32 != 0, GC_BF16>;
+    auto kern = c->gc_fmt == GC_F16 ? k_cgf<TL_RV_F32, GC_F16> : k_cgf<TL_RV_F32, GC_BF16>;
     kern<<<nbf(c), TBV, 0, c->stream>>>(c->g, c->g.tyf, targ(t, tref), cgf_r(c, t - 1), cgf_r(c, t),
                                         cgf_p(c, t - 1), cgf_p(c, t), c->x32, c->gcell, c->hx32, c->hy32, c->ctrl,
                                         part_f(c, t - 1), c->nb_f[(t - 1) & 1], part_f(c, t));

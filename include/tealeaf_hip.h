@@ -281,7 +281,7 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 #define TL_K_CG_P32 5  /* unused: folded into TL_K_CG_W32 */
 #define TL_K_CHEBY64 6 /* Chebyshev iteration (Cheby.mainstep!) */
 #define TL_K_PPCG64 7  /* PPCG inner Chebyshev step */
-#define TL_K_JACOBI64 8 /* Jacobi sweep */
+#define TL_K_JACOBI64 8 /* Jacobi sweep (the sweeps that do not also form the residual) */
 #define TL_K_RESID64 9 /* residual! r = u0 - A u (also the FP64 refresh of TL_FP32_IR) */
 #define TL_K_CGF32 10  /* FP32 inner CG iteration as one kernel (TL_FP32_KERNELS_FUSED) */
 #define TL_K_COUNT 11

@@ -1044,7 +1044,7 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
             switch (kernel_id) {
             case TL_K_CHEBY64: HIPCHK(c, launch_cheby_iter(c, 0.5, 1e-3, a, b, 0, t)); std::swap(a, b); break;
             case TL_K_PPCG64: HIPCHK(c, launch_ppcg_inner(c, 0.5, 1e-3, s0, s1, 0, t)); std::swap(s0, s1); break;
-            case TL_K_JACOBI64: HIPCHK(c, launch_jacobi(c, a, b, t, 1)); std::swap(a, b); break;
+            case TL_K_JACOBI64: HIPCHK(c, launch_jacobi(c, a, b, t, 0)); std::swap(a, b); break;
             default: HIPCHK(c, launch_residual(c)); break;
             }
         }

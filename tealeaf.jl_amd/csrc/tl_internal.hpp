@@ -73,6 +73,10 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_VEC_BLOCKS64
 #define TL_VEC_BLOCKS64 8192
 #endif
+// rows per load batch of the vectorised FP64 Chebyshev / PPCG / Jacobi kernels
+#ifndef TL_RV_VS64
+#define TL_RV_VS64 2
+#endif
 // the fused FP32 inner-CG kernel (k_cgf, software-pipelined): rows per load
 // batch, block count on large grids and the least rows per band (each band
 // re-reads 2 rows below and 2 above it).  Measured at 8192^2 (A/B runs

@@ -2082,43 +2082,6 @@ __global__ __launch_bounds__(TB) void k_cheby_init(Geo g, const double *__restri
     write_partials<1>(acc, part, g.nbmax);
 }
 
-// Cheby.mainstep!, Cheby.jl:84-106: w = A u; r = u0 - w; p = a p + b r;
-// u' = u + p; with `norm`, partials of sum r^2 (calc2norm)
-__global__ __launch_bounds__(TB) void k_cheby_iter(Geo g, int t, const double *__restrict__ uin,
-                                                   double *__restrict__ uout, const double *__restrict__ u0,
-                                                   double *__restrict__ p, const double *__restrict__ kx,
-                                                   const double *__restrict__ ky, double ca, double cb, int norm,
-                                                   const Ctrl *__restrict__ ctrl, double *__restrict__ part)
-{
-    if (t > ctrl->stop_iter) return;
-    const Col c = column(g);
-    double acc[1] = {0.0};
-    if (c.active) {
-        Win<double> win;
-        win.start(g, c, uin, ky);
-        for (int yb = c.y0; yb < c.y1; yb += R) {
-            Stn<double> st;
-            double b[R], pp[R];
-            load_stn(st, g, c, yb, uin, kx, ky);
-            load_rows(b, g, c, yb, u0);
-            load_rows(pp, g, c, yb, p);
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const double rv = b[q] - win.apply(st, q);
-                const double pv = ca * pp[q] + cb * rv;
-                if (yb + q < c.y1) {
-                    const long i = at(g, c.x, yb + q);
-                    p[i] = pv;
-                    uout[i] = win.cur + pv;
-                    acc[0] += rv * rv;
-                }
-                win.advance(st, q);
-            }
-        }
-    }
-    if (norm) write_partials<1>(acc, part, g.nbmax);
-}
-
 // ----------------------------------------------------------------- PPCG
 
 // CG.ur! without its norm (PPCG.jl:67, value unused) fused with
@@ -2154,43 +2117,149 @@ __global__ __launch_bounds__(TB) void k_ppcg_ur_sd(Geo g, int t, double *__restr
     }
 }
 
-// PPCG inner step (PPCG.jl:75-84 in the two-phase form): r -= A sd;
-// u += sd; sd' = a sd + b r.  With `last`, partials of rrn = sum r^2
-// (PPCG.jl:88), consumed by the following CG.p! kernel.
-__global__ __launch_bounds__(TB) void k_ppcg_inner(Geo g, int t, const double *__restrict__ sdin,
-                                                   double *__restrict__ sdout, double *__restrict__ r,
-                                                   double *__restrict__ u, const double *__restrict__ kx,
-                                                   const double *__restrict__ ky, double ca, double cb, int last,
-                                                   const Ctrl *__restrict__ ctrl, double *__restrict__ part)
+// ------------------------------- vectorised FP64 Chebyshev / PPCG / Jacobi
+//
+// The three stencil-update kernels of the other FP64 solvers on 16-byte
+// vectors (2 doubles per lane), in the row-batch scheme of k_cg_pwv: a lane
+// walks its column pair up a band keeping the rows below / at the current
+// row in registers, takes the columns left / right of its pair from the
+// adjacent lanes (lanes 0 and 63 load them with the row loads), and the next
+// batch's loads are issued before the current one is computed.  Per cell the
+// arithmetic is the reference's, operand for operand (smvp5's association;
+// Jacobi's as k_jacobi below), so results do not depend on the vector width.
+//   VS_CHEBY  (Cheby.mainstep!, Cheby.jl:84-106):
+//             r = u0 - A u; p = a p + b r; u' = u + p; [sum r^2 when calc2norm]
+//   VS_PPCG   (PPCG inner step, PPCG.jl:75-84 in the two-phase form: sd is
+//             ping-ponged): r -= A sd; u += sd; sd' = a sd + b r; [sum r^2 at
+//             the last inner step, PPCG.jl:88, consumed by the next CG.p!]
+//   VS_JACOBI (Jacobi.iterate!, Jacobi.jl:59-80):
+//             u' = (u0 + sum K u_nbr) / (1 + sum K); sum |u' - u|
+// `a_in` is the stencil input (u or sd), `a_out` its successor (u' or sd');
+// f1 / f2 are u0 / p (Chebyshev), r / u (PPCG), u0 / unused (Jacobi).
+enum VsOp : int { VS_CHEBY = 0, VS_PPCG = 1, VS_JACOBI = 2 };
+
+template <int OP, int RB, bool PIPE>
+__global__ __launch_bounds__(TBV) void k_vs64(Geo g, int ty, int t, const double *__restrict__ a_in,
+                                              double *__restrict__ a_out, const double *__restrict__ kx,
+                                              const double *__restrict__ ky, double *__restrict__ f1,
+                                              double *__restrict__ f2, double ca, double cb, int sums,
+                                              const Ctrl *__restrict__ ctrl, double *__restrict__ part)
 {
+    constexpr int V = 2;
+    typedef double T;
     if (t > ctrl->stop_iter) return;
-    const Col c = column(g);
+    const VCol<V> cl = vcolumn<V>(g, ty);
     double acc[1] = {0.0};
-    if (c.active) {
-        Win<double> win;
-        win.start(g, c, sdin, ky);
-        for (int yb = c.y0; yb < c.y1; yb += R) {
-            Stn<double> st;
-            double rr[R], uu[R];
-            load_stn(st, g, c, yb, sdin, kx, ky);
-            load_rows(rr, g, c, yb, r);
-            load_rows(uu, g, c, yb, u);
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const double s = win.cur;
-                const double rv = rr[q] - win.apply(st, q);
-                if (yb + q < c.y1) {
-                    const long i = at(g, c.x, yb + q);
-                    r[i] = rv;
-                    u[i] = uu[q] + s;
-                    sdout[i] = ca * s + cb * rv;
-                    acc[0] += rv * rv;
-                }
-                win.advance(st, q);
-            }
+    if (cl.y0 < g.ny) {
+        T dn[V], cur[V], kyd[V];
+        {
+            const long i0 = at(g, cl.xl, cl.y0), id = at(g, cl.xl, max(cl.y0 - 1, 0));
+            vload<T, V>(cur, a_in + i0);
+            vload<T, V>(dn, a_in + id);
+            vload<T, V>(kyd, ky + i0);
         }
+        const bool eneed = (cl.lane == 0 && cl.xl > 0) || (cl.lane == 63 && cl.xl + V < g.nx);
+        const long eoff = cl.lane == 0 ? -1 : V;
+        T ecur = (T)0;
+        if (eneed) ecur = a_in[at(g, cl.xl, cl.y0) + eoff];
+        struct Batch {
+            T up[RB][V], kxv[RB][V], kyu[RB][V], e1[RB][V], e2[RB][V];
+            T eup[RB], ekx[RB];
+        };
+        auto load_batch = [&](Batch &B, int yb) {
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int y = yb + q;
+                const long ic = atc(g, cl.xl, y), iu = atc(g, cl.xl, y + 1);
+                vload<T, V>(B.up[q], a_in + iu);
+                vload<T, V>(B.kxv[q], kx + ic);
+                vload<T, V>(B.kyu[q], ky + at(g, cl.xl, min(y + 1, g.ny)));
+                vload<T, V>(B.e1[q], f1 + ic);
+                if (OP != VS_JACOBI) vload<T, V>(B.e2[q], f2 + ic);
+                B.eup[q] = B.ekx[q] = (T)0;
+                if (eneed) B.eup[q] = a_in[iu + eoff];
+                if (cl.lane == 63) B.ekx[q] = kx[ic + V];
+            }
+        };
+        auto compute_batch = [&](const Batch &B, int yb) {
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int y = yb + q;
+                const long ic = atc(g, cl.xl, y);
+                T u2[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) u2[j] = y + 1 >= g.ny ? cur[j] : B.up[q][j];
+                T lv, rv;
+                auto col_val = [&](int) -> T { return ecur; };
+                vneigh<T, V>(g, cl, cur, col_val, lv, rv);
+                T kxn = lane_down1<false>(B.kxv[q][0]);
+                if (cl.lane == 63) kxn = B.ekx[q];
+                T o1[V], o2[V], o3[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const T kxl = B.kxv[q][j], kxr = j < V - 1 ? B.kxv[q][j + 1] : kxn;
+                    const T kyu = B.kyu[q][j];
+                    if constexpr (OP == VS_JACOBI) {
+                        const T l = j > 0 ? cur[j - 1] : lv;
+                        T r = j < V - 1 ? cur[j + 1] : rv;
+                        if (cl.xl + j + 1 >= g.nx) r = cur[j];
+                        double num = B.e1[q][j] + kxr * r;
+                        num = num + kxl * l;
+                        num = num + kyu * u2[j];
+                        num = num + kyd[j] * dn[j];
+                        double den = 1.0 + kxl;
+                        den = den + kxr;
+                        den = den + kyd[j];
+                        den = den + kyu;
+                        o1[j] = num / den;
+                    } else {
+                        const T w = vsmvp<T, V>(g, cl.xl, j, cur, lv, rv, dn[j], u2[j], kxl, kxr, kyd[j], kyu);
+                        if constexpr (OP == VS_CHEBY) {
+                            const T rr = B.e1[q][j] - w;        // u0 - A u
+                            const T pv = ca * B.e2[q][j] + cb * rr;
+                            o1[j] = cur[j] + pv;                // u'
+                            o2[j] = pv;                         // p
+                            o3[j] = rr;
+                        } else {
+                            const T rr = B.e1[q][j] - w;        // r - A sd
+                            o1[j] = ca * cur[j] + cb * rr;      // sd'
+                            o2[j] = rr;                         // r
+                            o3[j] = B.e2[q][j] + cur[j];        // u + sd
+                        }
+                    }
+                }
+                if (y < cl.y1 && cl.active) {
+                    const int nv = g.nx - cl.x0;
+                    vstore_n<T, V>(a_out + ic, o1, nv);
+                    if constexpr (OP == VS_CHEBY) {
+                        vstore_n<T, V>(f2 + ic, o2, nv);
+#pragma unroll
+                        for (int j = 0; j < V; ++j)
+                            if (j < nv) acc[0] += o3[j] * o3[j];
+                    } else if constexpr (OP == VS_PPCG) {
+                        vstore_n<T, V>(f1 + ic, o2, nv);
+                        vstore_n<T, V>(f2 + ic, o3, nv);
+#pragma unroll
+                        for (int j = 0; j < V; ++j)
+                            if (j < nv) acc[0] += o2[j] * o2[j];
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < V; ++j)
+                            if (j < nv) acc[0] += fabs(o1[j] - cur[j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    dn[j] = cur[j];
+                    cur[j] = u2[j];
+                    kyd[j] = B.kyu[q][j];
+                }
+                ecur = B.eup[q];
+            }
+        };
+        row_batches<PIPE, RB, Batch>(cl.y0, cl.y1, load_batch, compute_batch);
     }
-    if (last) write_partials<1>(acc, part, g.nbmax);
+    if (sums) write_partials<1, TBV>(acc, part, g.nbmax);
 }
 
 // --------------------------------------------------------------- Jacobi
@@ -2658,11 +2727,12 @@ hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double
 hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t)
 {
     const bool prof = prof_start(c, TL_K_CHEBY64);
-    k_cheby_iter<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_P],
-                                                    c->f[TL_F_KX], c->f[TL_F_KY], a, b, norm, c->ctrl, part_m(c));
+    k_vs64<VS_CHEBY, TL_RV_VS64, true><<<nb2(c), TBV, 0, c->stream>>>(
+        c->g, c->g.ty2, t, u_in, u_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_U0], c->f[TL_F_P], a, b, norm,
+        c->ctrl, part_m(c));
     if (prof) prof_stop(c);
     hipError_t e = hipGetLastError();
-    return (e == hipSuccess && norm) ? fin(c, part_m(c), nbh(c), 1, 0, FK_CHEBY, t) : e;
+    return (e == hipSuccess && norm) ? fin(c, part_m(c), nb2(c), 1, 0, FK_CHEBY, t) : e;
 }
 
 hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t)
@@ -2676,22 +2746,30 @@ hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t)
 hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t)
 {
     const bool prof = prof_start(c, TL_K_PPCG64);
-    k_ppcg_inner<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, sd_in, sd_out, c->f[TL_F_R], c->f[TL_F_U],
-                                                    c->f[TL_F_KX], c->f[TL_F_KY], a, b, last, c->ctrl,
-                                                    part_r(c, t));
+    k_vs64<VS_PPCG, TL_RV_VS64, true><<<nb2(c), TBV, 0, c->stream>>>(
+        c->g, c->g.ty2, t, sd_in, sd_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_R], c->f[TL_F_U], a, b, last,
+        c->ctrl, part_r(c, t));
     if (prof) prof_stop(c);
-    if (last) nbr(c, t) = nbh(c);
+    if (last) nbr(c, t) = nb2(c);
     return hipGetLastError();
 }
 
 hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer)
 {
     const bool prof = prof_start(c, TL_K_JACOBI64);
-    k_jacobi<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_KX], c->f[TL_F_KY],
-                                                c->ctrl, part_m(c));
+    if (defer) {
+        // the sweep whose sum joins the residual's (every 50th): the scalar
+        // kernel, so that both partial arrays have k_residual's block count
+        k_jacobi<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_KX],
+                                                    c->f[TL_F_KY], c->ctrl, part_m(c));
+    } else {
+        k_vs64<VS_JACOBI, TL_RV_VS64, true><<<nb2(c), TBV, 0, c->stream>>>(
+            c->g, c->g.ty2, t, u_in, u_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_U0], nullptr, 0.0, 0.0, 1,
+            c->ctrl, part_m(c));
+    }
     if (prof) prof_stop(c);
     hipError_t e = hipGetLastError();
-    return (e == hipSuccess && !defer) ? fin(c, part_m(c), nbh(c), 1, 0, FK_JACOBI, t) : e;
+    return (e == hipSuccess && !defer) ? fin(c, part_m(c), nb2(c), 1, 0, FK_JACOBI, t) : e;
 }
 
 // u += d; d = 0 over the whole allocation (d is zero outside the interior)

@@ -45,7 +45,7 @@ TL_FP32_KERNELS_AUTO, TL_FP32_KERNELS_TWO, TL_FP32_KERNELS_FUSED = 0, 1, 2
 KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, false>", 2: "k_cg_p<double>",
                 3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
                 5: "(fused into k_cgg<4, false, 2, true, 1>)",
-                6: "k_cheby_iter", 7: "k_ppcg_inner", 8: "k_jacobi", 9: "k_residual",
+                6: "k_vs64<0, 2, true>", 7: "k_vs64<1, 2, true>", 8: "k_vs64<2, 2, true>", 9: "k_residual",
                 10: "k_cgf<1, 1>",
                 -1: "k_copy_full", -2: "k_copy_vec"}
 

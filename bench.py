@@ -451,6 +451,8 @@ def main():
             "iterations": [st["iterations"] for st in flag["stats"]],
             "refreshes": [st["refreshes"] for st in flag["stats"]],
             "ir_delta": flag["ir_delta"],
+            "warm_attempted": [st["warm_attempted"] for st in flag["stats"]],
+            "warm_ms": [round(st["warm_ms"], 2) for st in flag["stats"]],
             "fallback": [st["fallback"] for st in flag["stats"]],
             "start": [["previous", "learned"][st["start"]] for st in flag["stats"]],
             "init_rel_resid": [st["init_rel_resid"] for st in flag["stats"]],

@@ -108,8 +108,17 @@ typedef struct {
                              (0 < ir_delta < 1; any other value selects the default 0.1) */
     int launch;           /* TL_LAUNCH_* */
     int fp32_kernels;     /* TL_FP32_IR: TL_FP32_KERNELS_* */
-    int reserved[6];
+    int warm_policy;      /* TL_WARM_LEARNED: TL_WARM_POLICY_* */
+    int reserved[5];
 } tl_solve_params;
+
+/* tl_solve_params.warm_policy: when a TL_WARM_LEARNED solve runs the network.
+ * Either way an attempted prediction is kept only if its residual beats the
+ * previous-step start's. */
+#define TL_WARM_POLICY_BACKOFF 0 /* on the context's first learned solve and after every accepted
+                                    prediction; after a rejection the next attempt waits 2, 4, 8,
+                                    then 16 solves (a non-attempted solve starts from u) */
+#define TL_WARM_POLICY_EVERY 1   /* every solve */
 
 /* tl_solve_params.fp32_kernels: how a TL_FP32_IR iteration is split into kernels.
  * Both reach the same FP64 tolerance; their FP32 roundings differ. */
@@ -136,7 +145,10 @@ typedef struct {
     double solve_ms;      /* wall time of tl_solve */
     int start;            /* initial guess used: TL_START_* */
     int coef_format;      /* TL_FP32_IR: storage of the inner CG's cell coefficient, TL_COEF_*; 0 otherwise */
-    double reserved[3];
+    int warm_attempted;   /* a warm start was evaluated in this solve (network run or guess tested) */
+    int reserved_i;
+    double warm_ms;       /* wall time of that evaluation (prediction and residual test) */
+    double reserved[1];
 } tl_solve_stats;
 
 /* tl_solve_stats.coef_format: half with a power-of-two scale when the coefficient's range

@@ -614,6 +614,12 @@ static int choose_guess(tl_ctx *c, const tl_solve_params *p, double rx, double r
     *guess = nullptr;
     st->start = TL_START_PREVIOUS;
     if (p->warm_start == TL_WARM_NONE) return TL_OK;
+    if (p->warm_start == TL_WARM_LEARNED && p->warm_policy == TL_WARM_POLICY_BACKOFF) {
+        const long s = c->warm_solves++;
+        if (s < c->warm_next) return TL_OK;  // waiting after a rejection: start from u
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    st->warm_attempted = 1;
     if (p->warm_start == TL_WARM_LEARNED) {
         if (!c->guess) RETIF(alloc_d(c, &c->guess, c->g.elems));
         RETIF(unet_predict_guess(c, rx, ry, p->coefficient, c->guess));
@@ -629,12 +635,19 @@ static int choose_guess(tl_ctx *c, const tl_solve_params *p, double rx, double r
     HIPCHK(c, launch_resid_norm_slot(c, c->f[TL_F_U], c->f[TL_F_U], 1));
     Ctrl k;
     RETIF(read_ctrl(c, &k));
-    if (std::isfinite(k.sums[0]) && k.sums[0] < k.sums[1]) {
+    const bool better = std::isfinite(k.sums[0]) && k.sums[0] < k.sums[1];
+    if (better) {
         *guess = c->guess;
         st->start = TL_START_WARM;
     } else {
         st->fallback = 1;
     }
+    if (p->warm_start == TL_WARM_LEARNED && p->warm_policy == TL_WARM_POLICY_BACKOFF) {
+        // accepted: attempt again next solve; rejected: wait 2, 4, 8, then 16 solves
+        c->warm_gap = better ? 1 : std::min(2 * c->warm_gap, 16);
+        c->warm_next = c->warm_solves - 1 + c->warm_gap;
+    }
+    st->warm_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return TL_OK;
 }
 
@@ -1106,6 +1119,10 @@ extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double r
     }
     if (p->launch != TL_LAUNCH_STREAM && p->launch != TL_LAUNCH_GRAPH) {
         c->err = "tl_solve: launch must be TL_LAUNCH_STREAM or TL_LAUNCH_GRAPH";
+        return TL_E_INVALID;
+    }
+    if (p->warm_policy != TL_WARM_POLICY_BACKOFF && p->warm_policy != TL_WARM_POLICY_EVERY) {
+        c->err = "tl_solve: warm_policy must be TL_WARM_POLICY_BACKOFF or TL_WARM_POLICY_EVERY";
         return TL_E_INVALID;
     }
     if (p->warm_start != TL_WARM_NONE && p->solver != TL_CG) {

@@ -12,6 +12,7 @@
 //   precision = fp64 | fp32_ir      stop_rule = reference | relative
 //   rel_tol = <float>               warm_start = none | learned
 //   ir_delta = <float>              fp32_kernels = auto | two | fused
+//   warm_policy = backoff | every
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -211,6 +212,14 @@ extern "C" int tl_deck_parse(const char *text, tl_deck *d)
                 d->params.fp32_kernels = TL_FP32_KERNELS_TWO;
             else if (v == "fused")
                 d->params.fp32_kernels = TL_FP32_KERNELS_FUSED;
+            else
+                ok = false;
+        } else if (key == "warmpolicy") {
+            const std::string v = strip(val);
+            if (v == "backoff")
+                d->params.warm_policy = TL_WARM_POLICY_BACKOFF;
+            else if (v == "every")
+                d->params.warm_policy = TL_WARM_POLICY_EVERY;
             else
                 ok = false;
         } else if (key == "warmstart") {

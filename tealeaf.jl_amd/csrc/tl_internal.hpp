@@ -254,6 +254,10 @@ struct tl_ctx {
     std::vector<double> prof_ms[TL_K_COUNT];
     tl::Unet *unet = nullptr;
     bool recording = false;
+    // learned warm start under TL_WARM_POLICY_BACKOFF: learned solves so far,
+    // the one that attempts next, and the current gap after rejections
+    long warm_solves = 0, warm_next = 0;
+    int warm_gap = 1;
     // hipGraph batches of CG iterations (TL_LAUNCH_GRAPH), keyed by solve kind,
     // batch length and every launch argument that can change between solves
     struct GraphBatch {

@@ -76,84 +76,127 @@ struct TV {
 
 // ------------------------------------------------------------- kernels
 
-// Convolution 3x3, zero padding, as implicit GEMM on MFMA.  Block = 4 waves;
-// wave w computes pixels x0 + 16w .. +15 of row y for all output channels.
-// Lane l: m = l & 15 (pixel / output-channel index), kq = l >> 4 (4-channel
-// group of the 16-wide K chunk).  A (pixels x K): A[m][4kq+i] = in[pix m][ci];
-// B (K x N): B[4kq+i][m] = W[tap][co = m][ci]; D: D[4kq+i][m] -> pixel 4kq+i,
-// output channel m.
+// Convolution 3x3, zero padding, as implicit GEMM on MFMA with the output
+// channels on the M side: a wave computes 64 pixels (4 groups of 16) of row y
+// for all NT*16 output channels, a block 4 waves side by side.  Per 16-channel
+// chunk of the input and tap: A = W[tap][co][ci] (16 co x 16 ci; lane (i, kq)
+// holds ci 4kq..4kq+3 of co i), B = in[pixel][ci] (16 ci x 16 pixels; lane
+// (j, kq) holds ci 4kq..4kq+3 of pixel j), D[co][pixel]: lane (j, kq) holds
+// output channels 4kq..4kq+3 of pixel j, stored as one 8-byte vector.  Each
+// weight fragment serves the 4 pixel groups, and the 9 taps' input fragments
+// are requested before their products.
 template <int NT>
 __global__ __launch_bounds__(256) void k_conv(TV in, TV out, const bf16 *__restrict__ wgt,
                                               const float *__restrict__ bias, int cinp, int relu)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int m = lane & 15, kq = lane >> 4;
+    const int j = lane & 15, kq = lane >> 4;
     const int y = blockIdx.y, b = blockIdx.z;
-    const int x0 = blockIdx.x * 64 + wave * 16;
+    const int x0 = (blockIdx.x * 4 + wave) * 64;
     if (x0 >= in.W) return;
-    const int xa = min(x0 + m, in.W - 1);  // this lane's A-row pixel (clamped: rows past W are never stored)
-    f4 acc[NT];
+    int xa[4];  // this lane's pixel of each group (clamped: pixels past W are never stored)
 #pragma unroll
-    for (int n = 0; n < NT; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < 4; ++m) xa[m] = min(x0 + 16 * m + j, in.W - 1);
+    f4 acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     const int coutp = NT * 16;
-    for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const bf16 *src = in.p + in.idx(b, y + dy, xa + dx) + kq * 4;
-        const bf16 *wt = wgt + ((long)tap * coutp + m) * cinp + kq * 4;
-        for (int kc = 0; kc < cinp; kc += 16) {
-            const s4 a = *(const s4 *)(src + kc);
+    for (int kc = 0; kc < cinp; kc += 16) {
+        s4 bv[9][4];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) bv[tap][m] = *(const s4 *)(in.p + in.idx(b, y + dy, xa[m] + dx) + kc + kq * 4);
+        }
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
-                const s4 bv = *(const s4 *)(wt + (long)n * 16 * cinp + kc);
-                acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[n], 0, 0, 0);
+                const s4 av = *(const s4 *)(wgt + ((long)tap * coutp + n * 16 + j) * cinp + kc + kq * 4);
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[tap][m], acc[m][n], 0, 0, 0);
             }
         }
     }
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-        const int co = n * 16 + m;
-        const float bb = bias ? bias[co] : 0.f;
+        const int co = n * 16 + 4 * kq;
+        float bb[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = x0 + 4 * kq + i;
+        for (int i = 0; i < 4; ++i) bb[i] = bias ? bias[co + i] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int x = x0 + 16 * m + j;
             if (x < out.W) {
-                float v = acc[n][i] + bb;
-                if (relu) v = fmaxf(v, 0.f);
-                out.p[out.idx(b, y, x) + co] = f2bf(v);
+                uint32_t w2[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    float v0 = acc[m][n][2 * h] + bb[2 * h], v1 = acc[m][n][2 * h + 1] + bb[2 * h + 1];
+                    if (relu) {
+                        v0 = fmaxf(v0, 0.f);
+                        v1 = fmaxf(v1, 0.f);
+                    }
+                    w2[h] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+                }
+                typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+                *(u2 *)(out.p + out.idx(b, y, x) + co) = u2{w2[0], w2[1]};
             }
         }
     }
 }
 
-// 2x2 average pool: out (H/2) from in (H) view
+// 2x2 average pool: out (H/2) from in (H) view, 8 channels (16 bytes) per item
 __global__ void k_pool(TV in, TV out, int C)
 {
-    const long n = (long)out.B * out.H * out.W * C;
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const int C8 = C / 8;
+    const long n = (long)out.B * out.H * out.W * C8;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(t % C);
-    long r = t / C;
-    const int x = (int)(r % out.W);
-    r /= out.W;
-    const int y = (int)(r % out.H);
-    const int b = (int)(r / out.H);
-    float s = bf2f(in.p[in.idx(b, 2 * y, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y, 2 * x + 1) + c]) +
-              bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x) + c]) + bf2f(in.p[in.idx(b, 2 * y + 1, 2 * x + 1) + c]);
-    out.p[out.idx(b, y, x) + c] = f2bf(0.25f * s);
+        const int c = (int)(t % C8) * 8;
+        long r = t / C8;
+        const int x = (int)(r % out.W);
+        r /= out.W;
+        const int y = (int)(r % out.H);
+        const int b = (int)(r / out.H);
+        const u4 q0 = *(const u4 *)(in.p + in.idx(b, 2 * y, 2 * x) + c);
+        const u4 q1 = *(const u4 *)(in.p + in.idx(b, 2 * y, 2 * x + 1) + c);
+        const u4 q2 = *(const u4 *)(in.p + in.idx(b, 2 * y + 1, 2 * x) + c);
+        const u4 q3 = *(const u4 *)(in.p + in.idx(b, 2 * y + 1, 2 * x + 1) + c);
+        u4 o;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t w = 0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int sh = 16 * h;
+                const float s = bf2f((bf16)(q0[k] >> sh)) + bf2f((bf16)(q1[k] >> sh)) + bf2f((bf16)(q2[k] >> sh)) +
+                                bf2f((bf16)(q3[k] >> sh));
+                w |= (uint32_t)f2bf(0.25f * s) << sh;
+            }
+            o[k] = w;
+        }
+        *(u4 *)(out.p + out.idx(b, y, x) + c) = o;
     }
 }
 
-// nearest x2 upsample: out (2H) from in (H)
+// nearest x2 upsample: out (2H) from in (H), 8 channels (16 bytes) per item
 __global__ void k_up(TV in, TV out, int C)
 {
-    const long n = (long)out.B * out.H * out.W * C;
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const int C8 = C / 8;
+    const long n = (long)out.B * out.H * out.W * C8;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(t % C);
-    long r = t / C;
-    const int x = (int)(r % out.W);
-    r /= out.W;
-    const int y = (int)(r % out.H);
-    const int b = (int)(r / out.H);
-    out.p[out.idx(b, y, x) + c] = in.p[in.idx(b, y / 2, x / 2) + c];
+        const int c = (int)(t % C8) * 8;
+        long r = t / C8;
+        const int x = (int)(r % out.W);
+        r /= out.W;
+        const int y = (int)(r % out.H);
+        const int b = (int)(r / out.H);
+        *(u4 *)(out.p + out.idx(b, y, x) + c) = *(const u4 *)(in.p + in.idx(b, y / 2, x / 2) + c);
     }
 }
 
@@ -731,7 +774,7 @@ static unsigned gs_blocks(long n, int bs = 256) { return (unsigned)std::min<long
 static hipError_t conv(tl_ctx *c, const TV &in, const TV &out, const bf16 *w, const float *bias, int cinp, int coutp,
                        int relu)
 {
-    const dim3 grid((in.W + 63) / 64, in.H, in.B);
+    const dim3 grid((in.W + 255) / 256, in.H, in.B);
     switch (coutp) {
     case 16: k_conv<1><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
     case 32: k_conv<2><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
@@ -765,16 +808,16 @@ static int forward(tl_ctx *c, Unet *u)
     const long n1 = (long)a.B * a.H * a.W, n2 = n1 / 4;
     UCHK(c, fwd_layer(c, u, 0, a.x0.v(), a.e1a.v()));
     UCHK(c, fwd_layer(c, u, 1, a.e1a.v(), a.c1.v(32)));
-    k_pool<<<gs_blocks(n2 * 16), 256, 0, c->stream>>>(a.c1.v(32), a.p1.v(), 16);
+    k_pool<<<gs_blocks(n2 * 2), 256, 0, c->stream>>>(a.c1.v(32), a.p1.v(), 16);
     UCHK(c, fwd_layer(c, u, 2, a.p1.v(), a.e2a.v()));
     UCHK(c, fwd_layer(c, u, 3, a.e2a.v(), a.c2.v(64)));
-    k_pool<<<gs_blocks(n2 / 4 * 32), 256, 0, c->stream>>>(a.c2.v(64), a.p2.v(), 32);
+    k_pool<<<gs_blocks(n2 / 4 * 4), 256, 0, c->stream>>>(a.c2.v(64), a.p2.v(), 32);
     UCHK(c, fwd_layer(c, u, 4, a.p2.v(), a.e3a.v()));
     UCHK(c, fwd_layer(c, u, 5, a.e3a.v(), a.e3b.v()));
-    k_up<<<gs_blocks(n2 * 64), 256, 0, c->stream>>>(a.e3b.v(), a.c2.v(0), 64);
+    k_up<<<gs_blocks(n2 * 8), 256, 0, c->stream>>>(a.e3b.v(), a.c2.v(0), 64);
     UCHK(c, fwd_layer(c, u, 6, a.c2.v(), a.d2a.v()));
     UCHK(c, fwd_layer(c, u, 7, a.d2a.v(), a.d2b.v()));
-    k_up<<<gs_blocks(n1 * 32), 256, 0, c->stream>>>(a.d2b.v(), a.c1.v(0), 32);
+    k_up<<<gs_blocks(n1 * 4), 256, 0, c->stream>>>(a.d2b.v(), a.c1.v(0), 32);
     UCHK(c, fwd_layer(c, u, 8, a.c1.v(), a.d1a.v()));
     UCHK(c, fwd_layer(c, u, 9, a.d1a.v(), a.d1b.v()));
     UCHK(c, fwd_layer(c, u, 10, a.d1b.v(), a.out.v()));

@@ -42,6 +42,7 @@ TL_START_PREVIOUS, TL_START_WARM = 0, 1
  TL_K_JACOBI64, TL_K_RESID64, TL_K_CGF32) = range(11)
 TL_K_COUNT = 11
 TL_FP32_KERNELS_AUTO, TL_FP32_KERNELS_TWO, TL_FP32_KERNELS_FUSED = 0, 1, 2
+TL_WARM_POLICY_BACKOFF, TL_WARM_POLICY_EVERY = 0, 1
 KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, false>", 2: "k_cg_p<double>",
                 3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
                 5: "(fused into k_cgg<4, false, 2, true, 1>)",
@@ -67,7 +68,7 @@ class SolveParams(C.Structure):
         ("ppcg_inner_steps", C.c_int), ("error_switch", C.c_int), ("eps", C.c_double), ("epslim", C.c_double),
         ("precision", C.c_int), ("stop_rule", C.c_int), ("rel_tol", C.c_double), ("warm_start", C.c_int),
         ("check_result", C.c_int), ("ir_delta", C.c_double), ("launch", C.c_int), ("fp32_kernels", C.c_int),
-        ("reserved", C.c_int * 6),
+        ("warm_policy", C.c_int), ("reserved", C.c_int * 5),
     ]
 
 
@@ -77,7 +78,7 @@ class SolveStats(C.Structure):
         ("inner_iters", C.c_int), ("refreshes", C.c_int), ("converged", C.c_int), ("fallback", C.c_int),
         ("error", C.c_double), ("eigmin", C.c_double), ("eigmax", C.c_double), ("rel_resid", C.c_double),
         ("init_rel_resid", C.c_double), ("solve_ms", C.c_double), ("start", C.c_int), ("coef_format", C.c_int),
-        ("reserved", C.c_double * 3),
+        ("warm_attempted", C.c_int), ("reserved_i", C.c_int), ("warm_ms", C.c_double), ("reserved", C.c_double * 1),
     ]
 
     def as_dict(self):

@@ -78,9 +78,11 @@ def test_parser_mixed_precision_keys():
     for v, code in (("two", _lib.TL_FP32_KERNELS_TWO), ("fused", _lib.TL_FP32_KERNELS_FUSED),
                     ("auto", _lib.TL_FP32_KERNELS_AUTO)):
         assert T.Settings.from_text(base + f"fp32_kernels={v}\n")._deck.params.fp32_kernels == code
+    assert s._deck.params.warm_policy == _lib.TL_WARM_POLICY_BACKOFF
+    assert T.Settings.from_text(base + "warm_policy=every\n")._deck.params.warm_policy == _lib.TL_WARM_POLICY_EVERY
     n0 = T.Settings.from_text(base).n_warnings
     for bad in ("precision=fp16\n", "ir_delta=abc\n", "stop_rule=sometimes\n", "warm_start=maybe\n",
-                "fp32_kernels=three\n"):
+                "fp32_kernels=three\n", "warm_policy=sometimes\n"):
         assert T.Settings.from_text(base + bad).n_warnings == n0 + 1, bad  # warned and ignored
 
 

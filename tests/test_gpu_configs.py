@@ -28,8 +28,10 @@ def _rel(a, b):
     return np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.abs(np.asarray(b)))
 
 
-def _run(text, steps, weights=None):
+def _run(text, steps, weights=None, every=False):
     s = T.Settings.from_text(text)
+    if every:  # the network evaluated at every solve, not only after acceptances
+        s.set(warm_policy=_lib.TL_WARM_POLICY_EVERY)
     c = T.initialiseapp(s)
     if weights is not None:
         unet.load_weights(c, weights)
@@ -47,7 +49,7 @@ def test_c4_learned_start_4096_unseen_seed():
     print(f"case: C4 learned start, FP64 CG, 4096^2 seed 1000, {steps} steps", flush=True)
     base_text = decks.random_rect_deck(1000, n=4096, steps=steps)
     base = _run(base_text, steps)
-    learn = _run(base_text.replace("warm_start=none", "warm_start=learned"), steps, WEIGHTS)
+    learn = _run(base_text.replace("warm_start=none", "warm_start=learned"), steps, WEIGHTS, every=True)
     accepted = 0
     for t, ((sb, stb), (sl, stl)) in enumerate(zip(base, learn)):
         assert stb["converged"] == 1 and stb["rel_resid"] <= 1e-10, stb

@@ -96,6 +96,7 @@ def test_learned_start_falls_back_to_previous_guess():
     base = T.initialiseapp(s0)
     sb, stb = T.diffuse(base, s0, 2)
     s1 = T.Settings.from_text(text.replace("warm_start=none", "warm_start=learned"))
+    s1.set(warm_policy=_lib.TL_WARM_POLICY_EVERY)
     c = T.initialiseapp(s1)
     unet.set_weights(c, np.zeros(unet.param_count(), dtype=np.float32))
     sl, stl = T.diffuse(c, s1, 2)
@@ -125,6 +126,7 @@ def test_trained_network_gain_on_unseen_problem():
     base = T.initialiseapp(s0)
     sb, stb = T.diffuse(base, s0, 3)
     s1 = T.Settings.from_text(text.replace("warm_start=none", "warm_start=learned"))
+    s1.set(warm_policy=_lib.TL_WARM_POLICY_EVERY)
     c = T.initialiseapp(s1)
     unet.load_weights(c, path)
     sl, stl = T.diffuse(c, s1, 3)
@@ -137,3 +139,26 @@ def test_trained_network_gain_on_unseen_problem():
             assert stl[t]["fallback"] == 0 and stl[t]["init_rel_resid"] < stb[t]["init_rel_resid"]
         else:
             assert stl[t]["fallback"] == 1
+
+
+def test_learned_start_backoff_after_rejections():
+    """TL_WARM_POLICY_BACKOFF (the default): the network runs on the first
+    solve and after every accepted prediction; after a rejection the next
+    attempt waits 2, 4, 8, then 16 solves.  A zero network is always rejected,
+    so over 16 steps it runs at solves 1, 3, 7 and 15; the other solves start
+    from the previous-step guess without evaluating it, and every solve meets
+    the same tolerance as the run without a warm start."""
+    print("case: learned start back-off, 64^2 zero network, 16 steps", flush=True)
+    text = decks.random_rect_deck(32, n=64, steps=16)
+    base = T.initialiseapp(T.Settings.from_text(text))
+    sb, stb = T.diffuse(base, T.Settings.from_text(text), 16)
+    s1 = T.Settings.from_text(text.replace("warm_start=none", "warm_start=learned"))
+    c = T.initialiseapp(s1)
+    unet.set_weights(c, np.zeros(unet.param_count(), dtype=np.float32))
+    sl, stl = T.diffuse(c, s1, 16)
+    attempted = [i + 1 for i, x in enumerate(stl) if x["warm_attempted"]]
+    assert attempted == [1, 3, 7, 15], attempted
+    for t in range(16):
+        assert stl[t]["fallback"] == stl[t]["warm_attempted"] and stl[t]["start"] == _lib.TL_START_PREVIOUS
+        assert (stl[t]["warm_ms"] > 0) == bool(stl[t]["warm_attempted"])
+        assert stl[t]["iterations"] == stb[t]["iterations"] and np.array_equal(sl[t], sb[t])

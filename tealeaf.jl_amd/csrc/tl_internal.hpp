@@ -73,9 +73,14 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_VEC_BLOCKS64
 #define TL_VEC_BLOCKS64 8192
 #endif
-// rows per load batch of the vectorised FP64 Chebyshev / PPCG / Jacobi kernels
+// rows per load batch of the vectorised FP64 Chebyshev / PPCG / Jacobi
+// kernels, and whether the next batch's loads are issued before the current
+// one is computed
 #ifndef TL_RV_VS64
 #define TL_RV_VS64 2
+#endif
+#ifndef TL_PIPE_VS64
+#define TL_PIPE_VS64 1
 #endif
 // the fused FP32 inner-CG kernel (k_cgf, software-pipelined): rows per load
 // batch, block count on large grids and the least rows per band (each band
@@ -92,6 +97,15 @@ constexpr int TBV = TL_TBV;
 #endif
 #ifndef TL_MIN_ROWS_F32
 #define TL_MIN_ROWS_F32 32
+#endif
+// rows of the fused FP32 kernel staged through the LDS per wave (0: loads
+// into registers, one row ahead).  Measured at 8192^2 (A/B in one box):
+// depth 3 (116 VGPRs, 44 KB LDS per block, 3 waves per SIMD) 0.312 ms per
+// iteration against 0.311 ms for register loads; depth 2 (5 waves per SIMD)
+// and 4 (2 waves) 2-5% slower.  A band-walking copy with the same arrays and
+// geometry streams at 5.0 TB/s, so more rows in flight have little to win.
+#ifndef TL_CGF_DEPTH
+#define TL_CGF_DEPTH 0
 #endif
 // neighbour exchange of the FP32 PW kernel by DPP wave shifts (else LDS shuffles)
 #ifndef TL_DPP

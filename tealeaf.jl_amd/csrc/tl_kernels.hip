@@ -1549,7 +1549,97 @@ __device__ __forceinline__ float smvp5f(float c, float l, float r, float d, floa
     return c + t;
 }
 
-template <int RB, int FMT>
+// Row staging of the fused kernel through the LDS (DEPTH > 0).  Register
+// loads keep about one row per wave in flight: the window of rows the kernel
+// walks with takes most of its registers, so only 3 waves fit per SIMD and
+// HBM latency, not bandwidth, bounds the sweep.  global_load_lds_dwordx4
+// (gfx950) writes a row straight into the LDS without holding registers
+// while in flight, so a wave can have DEPTH-1 rows requested ahead of the one
+// it computes.  Per wave a ring of DEPTH slots of one row each; the DMA puts
+// lane l's 16 bytes at slot + 16 l (4 bytes at + 4 l for the 4-byte form).
+// The coefficient row (2 bytes per column) is fetched by lanes 0-31 with 16
+// bytes each.  The columns outside the wave come by 4-byte gathers of lanes
+// 0-4 (left: r, r, p, p at x0-2, x0-1, the gc word there) and 8-12 (right, at
+// x0+256, x0+257).
+struct alignas(16) FSlot {
+    float r[256], p[256], x[256];   // own 4 columns of each lane, 16 bytes per lane
+    unsigned g[128];                // coefficients, 8 bytes per lane
+    unsigned e[16];                 // outside columns (words 0-4 left, 8-12 right)
+};
+constexpr unsigned FSLOT_P = 1024, FSLOT_X = 2048, FSLOT_G = 3072, FSLOT_E = 3584;
+static_assert(sizeof(FSlot) == 3648, "slot layout");
+#define TL_LDS(ptr) ((__attribute__((address_space(3))) void *)(ptr))
+
+typedef float fv4_t __attribute__((ext_vector_type(4)));
+typedef unsigned uv2_t __attribute__((ext_vector_type(2)));
+
+// Read a staged row once at most N of this wave's vector-memory instructions
+// are outstanding.  Vector memory counts complete in issue order on gfx9, and
+// the N instructions issued after this row's DMAs are the later rows' DMAs
+// (plus any stores, which only makes the wait longer).  The compiler does not
+// see these LDS reads, so it inserts no full wait for the outstanding DMAs of
+// the other slots; the block waits for its own reads before it ends.
+template <int N, bool X>
+__device__ __forceinline__ void fslot_read(unsigned a16, unsigned a8, unsigned ae, fv4_t &r, fv4_t &p, fv4_t &xv,
+                                           uv2_t &gw, fv4_t &e4, unsigned &e1)
+{
+    if constexpr (X) {
+        asm volatile(
+            "s_waitcnt vmcnt(%[n])\n\t"
+            "ds_read_b128 %[r], %[a16]\n\t"
+            "ds_read_b128 %[p], %[a16] offset:1024\n\t"
+            "ds_read_b128 %[x], %[a16] offset:2048\n\t"
+            "ds_read_b64 %[g], %[a8]\n\t"
+            "ds_read_b128 %[e], %[ae]\n\t"
+            "ds_read_b32 %[e1], %[ae] offset:16\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : [r] "=&v"(r), [p] "=&v"(p), [x] "=&v"(xv), [g] "=&v"(gw), [e] "=&v"(e4), [e1] "=&v"(e1)
+            : [a16] "v"(a16), [a8] "v"(a8), [ae] "v"(ae), [n] "i"(N)
+            : "memory");
+    } else {
+        asm volatile(
+            "s_waitcnt vmcnt(%[n])\n\t"
+            "ds_read_b128 %[r], %[a16]\n\t"
+            "ds_read_b128 %[p], %[a16] offset:1024\n\t"
+            "ds_read_b64 %[g], %[a8]\n\t"
+            "ds_read_b128 %[e], %[ae]\n\t"
+            "ds_read_b32 %[e1], %[ae] offset:16\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : [r] "=&v"(r), [p] "=&v"(p), [g] "=&v"(gw), [e] "=&v"(e4), [e1] "=&v"(e1)
+            : [a16] "v"(a16), [a8] "v"(a8), [ae] "v"(ae), [n] "i"(N)
+            : "memory");
+        xv = fv4_t{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
+// the wait for a row with rem later rows requested, G = 4 DMA instructions
+// per row (5 when x is staged too)
+template <int DEPTH>
+__device__ __forceinline__ void fslot_wait_read(int rem, bool X, unsigned a16, unsigned a8, unsigned ae, fv4_t &r,
+                                                fv4_t &p, fv4_t &xv, uv2_t &gw, fv4_t &e4, unsigned &e1)
+{
+    if (X) {
+        if (DEPTH >= 4 && rem >= 3)
+            fslot_read<15, true>(a16, a8, ae, r, p, xv, gw, e4, e1);
+        else if (DEPTH >= 3 && rem == 2)
+            fslot_read<10, true>(a16, a8, ae, r, p, xv, gw, e4, e1);
+        else if (rem == 1)
+            fslot_read<5, true>(a16, a8, ae, r, p, xv, gw, e4, e1);
+        else
+            fslot_read<0, true>(a16, a8, ae, r, p, xv, gw, e4, e1);
+    } else {
+        if (DEPTH >= 4 && rem >= 3)
+            fslot_read<12, false>(a16, a8, ae, r, p, xv, gw, e4, e1);
+        else if (DEPTH >= 3 && rem == 2)
+            fslot_read<8, false>(a16, a8, ae, r, p, xv, gw, e4, e1);
+        else if (rem == 1)
+            fslot_read<4, false>(a16, a8, ae, r, p, xv, gw, e4, e1);
+        else
+            fslot_read<0, false>(a16, a8, ae, r, p, xv, gw, e4, e1);
+    }
+}
+
+template <int RB, int FMT, int DEPTH>
 __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, const float *__restrict__ rin,
                                                          float *__restrict__ rout, const float *__restrict__ pin,
                                                          float *__restrict__ pout, float *__restrict__ x,
@@ -1608,10 +1698,47 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, c
         }
     };
 
+    // LDS staging (DEPTH > 0): this wave's ring, the per-lane sources of the
+    // coefficient and outside-column gathers
+    static_assert(DEPTH == 0 || RB == 1, "LDS staging works row by row");
+    static_assert(DEPTH <= 4, "waits are spelled out for up to 3 rows ahead");
+    __shared__ FSlot ring[DEPTH > 0 ? TBV / 64 : 1][DEPTH > 0 ? DEPTH : 1];
+    const int xw = x0 - 4 * lane;                    // first column of the wave
+    const int gcol = min(xw + 8 * lane, ((g.nx - 1) / 8) * 8);
+    const bool glane = lane < 32;
+    const bool elane = lane < 5 || (lane >= 8 && lane < 13);
+    const char *ebase;
+    long epitch;
+    {
+        const int k = lane & 7;
+        int ec = lane < 8 ? (xw > 0 ? xw - 2 : 0) : (xw + 256 < g.nx ? xw + 256 : 0);
+        if (k < 4) ec += k & 1;
+        ebase = k < 2 ? (const char *)(rin + at(g, ec, 0))
+                      : k < 4 ? (const char *)(pin + at(g, ec, 0)) : (const char *)(gc + at(g, ec, 0));
+        epitch = g.pitch * (k < 4 ? (long)sizeof(float) : (long)sizeof(tl_gc_t));
+    }
+    FSlot *const myring = &ring[0][0] + (DEPTH > 0 ? (threadIdx.x >> 6) * DEPTH : 0);
+    auto dma_row = [&](int s, int yb, bool withx) {
+        const int yy = min(max(yb, 0), g.ny - 1);
+        const long i = at(g, cl.xl, yy);
+        FSlot &S = myring[s];
+        __builtin_amdgcn_global_load_lds(rin + i, TL_LDS(S.r), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(pin + i, TL_LDS(S.p), 16, 0, 0);
+        if (withx) __builtin_amdgcn_global_load_lds(x + i, TL_LDS(S.x), 16, 0, 0);
+        if (glane) __builtin_amdgcn_global_load_lds(gc + at(g, gcol, yy), TL_LDS(S.g), 16, 0, 0);
+        if (elane) __builtin_amdgcn_global_load_lds(ebase + (long)yy * epitch, TL_LDS(S.e), 4, 0, 0);
+    };
+
     // the band's first rows are requested before the block sums the previous
     // iteration's partials, so the reduction overlaps their latency
     FBatch<RB> B0, B1;
-    load_batch(B0, cl.y0 - 2);
+    if constexpr (DEPTH == 0) {
+        load_batch(B0, cl.y0 - 2);
+    } else {
+#pragma unroll
+        for (int k = 0; k < DEPTH - 1; ++k)
+            if (cl.y0 - 2 + k <= cl.y1 + 1) dma_row(k, cl.y0 - 2 + k, xupd);
+    }
     double s4[4];
     sum_partials<4, TBV>(sin, nbin, g.nbmax, s4);
     const double gam = s4[0];
@@ -1638,7 +1765,11 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, c
             }
         }
     }
-    if (why != WHY_NONE || !ok) return;  // every block derives the same decision
+    if (why != WHY_NONE || !ok) {  // every block derives the same decision
+        // rows already requested into the LDS land before the wave ends
+        if constexpr (DEPTH > 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        return;
+    }
     const float beta = (float)bd, alpha = (float)ad;
     const float alpha_prev = xupd ? (float)ctrl->alphas[(t - 1) & 1] : 0.0f;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};     // gamma, delta, eps, pi of iteration t+1
@@ -1815,15 +1946,49 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, c
                 }
             }
         };
-        // rows y0-2 .. y1+1 in batches of RB, the next batch's loads issued
-        // before the current batch is computed (the first one above)
-        for (int yb = y0 - 2; yb < y1 + 2; yb += 2 * RB) {
-            const bool more = yb + RB < y1 + 2;
-            if (more) load_batch(B1, yb + RB);
-            compute_batch(B0, yb);
-            if (!more) break;
-            if (yb + 2 * RB < y1 + 2) load_batch(B0, yb + 2 * RB);
-            compute_batch(B1, yb + RB);
+        if constexpr (DEPTH == 0) {
+            // rows y0-2 .. y1+1 in batches of RB, the next batch's loads issued
+            // before the current batch is computed (the first one above)
+            for (int yb = y0 - 2; yb < y1 + 2; yb += 2 * RB) {
+                const bool more = yb + RB < y1 + 2;
+                if (more) load_batch(B1, yb + RB);
+                compute_batch(B0, yb);
+                if (!more) break;
+                if (yb + 2 * RB < y1 + 2) load_batch(B0, yb + 2 * RB);
+                compute_batch(B1, yb + RB);
+            }
+        } else {
+            // row L: request row L+DEPTH-1 into the slot row L-1 left, wait
+            // for row L (the rows after it still in flight), compute it
+            const unsigned ring0 = (unsigned)(size_t)TL_LDS(myring);
+            const unsigned o16 = 16u * lane, o8 = FSLOT_G + 8u * lane, oe = FSLOT_E + (lane == 63 ? 32u : 0u);
+            int sr = 0;  // slot of row L
+            for (int L = y0 - 2; L <= y1 + 1; ++L) {
+                const int ahead = L + DEPTH - 1;
+                if (ahead <= y1 + 1) dma_row(sr == 0 ? DEPTH - 1 : sr - 1, ahead, xupd);
+                const int rem = min(DEPTH - 1, y1 + 1 - L);  // rows requested after row L
+                const unsigned base = ring0 + (unsigned)sr * (unsigned)sizeof(FSlot);
+                fv4_t rv, pv, xv, e4;
+                uv2_t gw;
+                unsigned e1;
+                fslot_wait_read<DEPTH>(rem, xupd, base + o16, base + o8, base + oe, rv, pv, xv, gw, e4, e1);
+                FBatch<1> &B = B0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    B.r[0][j] = rv[j];
+                    B.p[0][j] = pv[j];
+                    B.x[0][j] = xv[j];
+                }
+                B.g[0][0] = gw[0];
+                B.g[0][1] = gw[1];
+                B.er[0][0] = eneed ? e4[0] : 0.f;
+                B.er[0][1] = eneed ? e4[1] : 0.f;
+                B.ep[0][0] = eneed ? e4[2] : 0.f;
+                B.ep[0][1] = eneed ? e4[3] : 0.f;
+                B.eg[0] = eneed ? e1 : 0u;
+                compute_batch(B, L);
+                sr = sr == DEPTH - 1 ? 0 : sr + 1;
+            }
         }
     }
     write_partials<4, TBV>(acc, sout, g.nbmax);
@@ -2727,7 +2892,7 @@ hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double
 hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t)
 {
     const bool prof = prof_start(c, TL_K_CHEBY64);
-    k_vs64<VS_CHEBY, TL_RV_VS64, true><<<nb2(c), TBV, 0, c->stream>>>(
+    k_vs64<VS_CHEBY, TL_RV_VS64, TL_PIPE_VS64 != 0><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, t, u_in, u_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_U0], c->f[TL_F_P], a, b, norm,
         c->ctrl, part_m(c));
     if (prof) prof_stop(c);
@@ -2746,7 +2911,7 @@ hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t)
 hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t)
 {
     const bool prof = prof_start(c, TL_K_PPCG64);
-    k_vs64<VS_PPCG, TL_RV_VS64, true><<<nb2(c), TBV, 0, c->stream>>>(
+    k_vs64<VS_PPCG, TL_RV_VS64, TL_PIPE_VS64 != 0><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, t, sd_in, sd_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_R], c->f[TL_F_U], a, b, last,
         c->ctrl, part_r(c, t));
     if (prof) prof_stop(c);
@@ -2763,7 +2928,7 @@ hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, in
         k_jacobi<<<sgrid(c->g), TB, 0, c->stream>>>(c->g, t, u_in, u_out, c->f[TL_F_U0], c->f[TL_F_KX],
                                                     c->f[TL_F_KY], c->ctrl, part_m(c));
     } else {
-        k_vs64<VS_JACOBI, TL_RV_VS64, true><<<nb2(c), TBV, 0, c->stream>>>(
+        k_vs64<VS_JACOBI, TL_RV_VS64, TL_PIPE_VS64 != 0><<<nb2(c), TBV, 0, c->stream>>>(
             c->g, c->g.ty2, t, u_in, u_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_U0], nullptr, 0.0, 0.0, 1,
             c->ctrl, part_m(c));
     }
@@ -2847,7 +3012,7 @@ static inline unsigned nbf(const tl_ctx *c) { return (unsigned)(c->g.gx4 * c->g.
 hipError_t launch_cgf(tl_ctx *c, int t, int tref)
 {
     const bool prof = prof_start(c, TL_K_CGF32);
-    auto kern = c->gc_fmt == GC_F16 ? k_cgf<TL_RV_F32, GC_F16> : k_cgf<TL_RV_F32, GC_BF16>;
+    auto kern = c->gc_fmt == GC_F16 ? k_cgf<TL_RV_F32, GC_F16, TL_CGF_DEPTH> : k_cgf<TL_RV_F32, GC_BF16, TL_CGF_DEPTH>;
     kern<<<nbf(c), TBV, 0, c->stream>>>(c->g, c->g.tyf, targ(t, tref), cgf_r(c, t - 1), cgf_r(c, t),
                                         cgf_p(c, t - 1), cgf_p(c, t), c->x32, c->gcell, c->hx32, c->hy32, c->ctrl,
                                         part_f(c, t - 1), c->nb_f[(t - 1) & 1], part_f(c, t));

@@ -313,6 +313,31 @@ def kernel_repeat(n, seed, kid, reps):
     print(json.dumps({"kernel": _lib.KERNEL_NAMES[kid], "reps": reps, "n": n}))
 
 
+def unet_time(n, seed, reps):
+    """The learned start's inference alone on the n^2 benchmark problem: after
+    one FP32-refined timestep, `reps` calls of tl_unet_predict (face
+    coefficients, features, the U-Net forward pass, the guess), each timed on
+    the host; one JSON line with the mean and the fastest call."""
+    s = make_settings(n, seed, "fp32_ir", "learned")
+    chunk = T.initialiseapp(s)
+    from tealeaf_hip import unet
+    unet.load_weights(chunk, WEIGHTS)
+    T.step(chunk, s)
+    L = _lib.load()
+    rx, ry = s.dtinit / (s.dx * s.dx), s.dtinit / (s.dy * s.dy)
+    times = []
+    for i in range(reps + 1):
+        t0 = time.perf_counter()
+        # returns after a synchronisation of the context's stream
+        _lib.check(L.tl_unet_predict(chunk._ctx, rx, ry, s.coefficient), chunk._ctx, "tl_unet_predict")
+        if i:  # the first call allocates the activations
+            times.append((time.perf_counter() - t0) * 1e3)
+        log(f"[unet] call {i + 1}: {(time.perf_counter() - t0) * 1e3:.2f} ms")
+    chunk.close()
+    print(json.dumps({"unet_predict_ms_mean": round(float(np.mean(times)), 3),
+                      "unet_predict_ms_min": round(float(np.min(times)), 3), "calls": reps, "n": n}), flush=True)
+
+
 # algorithmic bytes per interior cell of the kernels kernel_table measures
 # (FP64 solver kernels: arrays read once, written once; k_residual in the form
 # tl_kernel_repeat runs it, writing the FP64 r)
@@ -397,6 +422,8 @@ def main():
                     help="FP32-refined CG: refresh the FP64 residual when the FP32 one falls by this factor")
     ap.add_argument("--solvers", action="store_true",
                     help="CG, Chebyshev and PPCG on the 3-state reference deck at --n (default 4096 here)")
+    ap.add_argument("--unet-time", type=int, default=0, metavar="REPS",
+                    help="time REPS calls of the learned start's inference alone at --n")
     ap.add_argument("--kernel-table", type=int, default=0, metavar="REPS",
                     help="per-kernel HIP-event timing of every hot kernel, REPS launches each, at --n")
     a = ap.parse_args()
@@ -407,6 +434,9 @@ def main():
         sys.exit(2)
     if a.kernel_table:
         kernel_table(a.n, a.seed, a.kernel_table)
+        return
+    if a.unet_time:
+        unet_time(a.n, a.seed, a.unet_time)
         return
     if a.kernel_repeat:
         kernel_repeat(a.n, a.seed, a.kernel_repeat[0], a.kernel_repeat[1])

@@ -75,9 +75,12 @@ constexpr int TBV = TL_TBV;
 #endif
 // rows per load batch of the vectorised FP64 Chebyshev / PPCG / Jacobi
 // kernels, and whether the next batch's loads are issued before the current
-// one is computed
+// one is computed.  Measured at 8192^2 (A/B in one box): 4 rows pipelined
+// (236 VGPRs, 2 waves per SIMD) against 2 rows: Chebyshev 5.28 / 5.18 TB/s,
+// PPCG 5.18 / 5.11, Jacobi 5.39 / 5.11; 4 rows unpipelined, 3 rows, and
+// 12288 or 16384 blocks no better
 #ifndef TL_RV_VS64
-#define TL_RV_VS64 2
+#define TL_RV_VS64 4
 #endif
 #ifndef TL_PIPE_VS64
 #define TL_PIPE_VS64 1

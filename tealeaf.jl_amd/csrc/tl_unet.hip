@@ -46,6 +46,26 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16;
 
+// output rows per wave of the convolutions: the band of rows a wave walks
+// (fewer on small tensors, so that a launch still has about TL_CONV_BLOCKS
+// blocks); 16 and 64 measured within 2% of 32 at 8192^2
+#ifndef TL_CONV_ROWS
+#define TL_CONV_ROWS 32
+#endif
+#ifndef TL_CONV_BLOCKS
+#define TL_CONV_BLOCKS 4096
+#endif
+// occupancy targets (waves per SIMD) of the convolutions with 16 and 32
+// output channels per launch; the compiler otherwise spends registers down to
+// 2 and 1 waves.  Measured at 8192^2: 16-channel launches at 2 waves (168
+// VGPRs) make the inference 6% faster than at 3 (a few registers spilled)
+#ifndef TL_CONV_WPE1
+#define TL_CONV_WPE1 2
+#endif
+#ifndef TL_CONV_WPE2
+#define TL_CONV_WPE2 2
+#endif
+
 constexpr int NL = 11;
 struct LayerDef {
     int cin, cinp, cout, coutp, relu;
@@ -77,75 +97,151 @@ struct TV {
 // ------------------------------------------------------------- kernels
 
 // Convolution 3x3, zero padding, as implicit GEMM on MFMA with the output
-// channels on the M side: a wave computes 64 pixels (4 groups of 16) of row y
-// for all NT*16 output channels, a block 4 waves side by side.  Per 16-channel
-// chunk of the input and tap: A = W[tap][co][ci] (16 co x 16 ci; lane (i, kq)
-// holds ci 4kq..4kq+3 of co i), B = in[pixel][ci] (16 ci x 16 pixels; lane
-// (j, kq) holds ci 4kq..4kq+3 of pixel j), D[co][pixel]: lane (j, kq) holds
-// output channels 4kq..4kq+3 of pixel j, stored as one 8-byte vector.  Each
-// weight fragment serves the 4 pixel groups, and the 9 taps' input fragments
-// are requested before their products.
+// channels on the M side.  A wave owns 64 pixels (4 groups of 16) of a column
+// strip and walks a band of R output rows down the strip; a block is 4 waves
+// side by side.  Each input row is loaded once per 16-channel chunk (its 3
+// column shifts x 4 pixel groups: 12 fragments), the next (row, chunk)'s
+// fragments requested before this one's products, and added into the three
+// output rows it borders, kept in three rolling accumulator sets: input row r
+// is row dy = 2 of output r-1's window (complete after it: stored), dy = 1 of
+// output r and dy = 0 of output r+1.  Per chunk and tap: A = W[tap][co][ci]
+// (16 co x 16 ci; lane (i, kq) holds ci 4kq..4kq+3 of co i), B =
+// in[pixel][ci] (16 ci x 16 pixels; lane (j, kq) holds ci 4kq..4kq+3 of pixel
+// j), D[co][pixel]: lane (j, kq) holds output channels 4kq..4kq+3 of pixel j,
+// stored as one 8-byte vector.  A launch computes output channels co0 ..
+// co0 + 16 NT - 1 of coutp on the pixel grid of `out`.
+// Input channels [0, csplit) come from `in`, [csplit, cinp) from `skip`
+// (channel c - csplit): a decoder's concatenation [up | skip] read in place.
+// With upin, `in` has half the resolution and is read at (y/2, x/2), the
+// nearest-neighbour upsample (its zero border lands on the output's border).
 template <int NT>
-__global__ __launch_bounds__(256) void k_conv(TV in, TV out, const bf16 *__restrict__ wgt,
-                                              const float *__restrict__ bias, int cinp, int relu)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 1 ? TL_CONV_WPE1 : TL_CONV_WPE2))) void k_conv(
+    TV in, TV skip, int csplit, int upin, TV out, const bf16 *__restrict__ wgt, const float *__restrict__ bias,
+    int cinp, int coutp, int co0, int relu, int R)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = lane & 15, kq = lane >> 4;
-    const int y = blockIdx.y, b = blockIdx.z;
+    const int y0 = blockIdx.y * R, b = blockIdx.z;
     const int x0 = (blockIdx.x * 4 + wave) * 64;
-    if (x0 >= in.W) return;
-    int xa[4];  // this lane's pixel of each group (clamped: pixels past W are never stored)
-#pragma unroll
-    for (int m = 0; m < 4; ++m) xa[m] = min(x0 + 16 * m + j, in.W - 1);
-    f4 acc[4][NT];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < NT; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
-    const int coutp = NT * 16;
-    for (int kc = 0; kc < cinp; kc += 16) {
-        s4 bv[9][4];
-#pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-#pragma unroll
-            for (int m = 0; m < 4; ++m) bv[tap][m] = *(const s4 *)(in.p + in.idx(b, y + dy, xa[m] + dx) + kc + kq * 4);
-        }
-#pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-#pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                const s4 av = *(const s4 *)(wgt + ((long)tap * coutp + n * 16 + j) * cinp + kc + kq * 4);
-#pragma unroll
-                for (int m = 0; m < 4; ++m)
-                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[tap][m], acc[m][n], 0, 0, 0);
-            }
+    // this launch's weights, [tap][16 NT output channels][cinp], copied into
+    // the LDS once per block: every wave reads all of them for every row, and
+    // from the caches they would be fetched by all CUs from the same lines
+    extern __shared__ __attribute__((aligned(16))) bf16 wl[];
+    {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        const int per = cinp / 8, nvec = 9 * 16 * NT * per;  // 16-byte vectors
+        for (int v = threadIdx.x; v < nvec; v += 256) {
+            const int rowi = v / per, c8 = (v % per) * 8;
+            const int tap = rowi / (16 * NT), co = rowi % (16 * NT);
+            *(u4 *)(wl + (long)rowi * cinp + c8) = *(const u4 *)(wgt + ((long)tap * coutp + co0 + co) * cinp + c8);
         }
     }
+    __syncthreads();
+    if (x0 >= out.W) return;
+    const int yout = min(y0 + R, out.H);  // output rows y0 .. yout-1
+    int xa[4];  // this lane's pixel of each group (clamped: pixels past W are never stored)
 #pragma unroll
-    for (int n = 0; n < NT; ++n) {
-        const int co = n * 16 + 4 * kq;
-        float bb[4];
+    for (int m = 0; m < 4; ++m) xa[m] = min(x0 + 16 * m + j, out.W - 1);
+    float bb[NT][4];  // requested first: their latency overlaps the main loop
 #pragma unroll
-        for (int i = 0; i < 4; ++i) bb[i] = bias ? bias[co + i] : 0.f;
+    for (int n = 0; n < NT; ++n)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int x = x0 + 16 * m + j;
-            if (x < out.W) {
-                uint32_t w2[2];
+        for (int i = 0; i < 4; ++i) bb[n][i] = bias ? bias[co0 + n * 16 + 4 * kq + i] : 0.f;
+    f4 a0[4][NT], a1[4][NT], a2[4][NT];
+    auto zero = [&](f4 (&a)[4][NT]) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    float v0 = acc[m][n][2 * h] + bb[2 * h], v1 = acc[m][n][2 * h + 1] + bb[2 * h + 1];
-                    if (relu) {
-                        v0 = fmaxf(v0, 0.f);
-                        v1 = fmaxf(v1, 0.f);
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) a[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+    };
+    zero(a0);
+    zero(a1);
+    zero(a2);
+    auto store = [&](const f4 (&a)[4][NT], int y) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int co = co0 + n * 16 + 4 * kq;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int x = x0 + 16 * m + j;
+                if (x < out.W) {
+                    uint32_t w2[2];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        float v0 = a[m][n][2 * h] + bb[n][2 * h], v1 = a[m][n][2 * h + 1] + bb[n][2 * h + 1];
+                        if (relu) {
+                            v0 = fmaxf(v0, 0.f);
+                            v1 = fmaxf(v1, 0.f);
+                        }
+                        w2[h] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
                     }
-                    w2[h] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+                    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+                    *(u2 *)(out.p + out.idx(b, y, x) + co) = u2{w2[0], w2[1]};
                 }
-                typedef uint32_t u2 __attribute__((ext_vector_type(2)));
-                *(u2 *)(out.p + out.idx(b, y, x) + co) = u2{w2[0], w2[1]};
             }
         }
+    };
+    // the 12 fragments of input row r, chunk kc
+    auto load = [&](s4 (&f)[3][4], int r, int kc) {
+        if (kc < csplit) {
+            const int ry = upin ? r >> 1 : r;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int x = xa[m] + dx - 1;
+                    f[dx][m] = *(const s4 *)(in.p + in.idx(b, ry, upin ? x >> 1 : x) + kc + kq * 4);
+                }
+        } else {
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    f[dx][m] = *(const s4 *)(skip.p + skip.idx(b, r, xa[m] + dx - 1) + kc - csplit + kq * 4);
+        }
+    };
+    s4 fc[3][4], fn[3][4];
+    load(fc, y0 - 1, 0);
+    // input row r into the outputs r-1 (up), r (mid), r+1 (dn); then output
+    // r-1 is complete
+    auto row = [&](int r, f4 (&up)[4][NT], f4 (&mid)[4][NT], f4 (&dn)[4][NT]) {
+        for (int kc = 0; kc < cinp; kc += 16) {
+            // the next step: the next chunk of this row, or the next row's first
+            const bool last = kc + 16 >= cinp;
+            if (!last || r + 1 <= yout) load(fn, last ? r + 1 : r, last ? 0 : kc + 16);
+            // keeps the following steps' loads below this step's products
+            asm volatile("" ::: "memory");
+            s4 wv[9][NT];
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+                for (int n = 0; n < NT; ++n) wv[tap][n] = *(const s4 *)(wl + (tap * 16 * NT + n * 16 + j) * cinp + kc + kq * 4);
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        dn[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv[dx][n], fc[dx][m], dn[m][n], 0, 0, 0);
+                        mid[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv[3 + dx][n], fc[dx][m], mid[m][n], 0, 0, 0);
+                        up[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv[6 + dx][n], fc[dx][m], up[m][n], 0, 0, 0);
+                    }
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) fc[dx][m] = fn[dx][m];
+        }
+        if (r - 1 >= y0 && r - 1 < yout) store(up, r - 1);
+        zero(up);
+    };
+    // input rows y0-1 .. yout (row yout may be the zero bottom border); the
+    // roles of the three accumulator sets rotate with the row
+    for (int r = y0 - 1; r <= yout; r += 3) {
+        row(r, a0, a1, a2);
+        if (r + 1 > yout) break;
+        row(r + 1, a1, a2, a0);
+        if (r + 2 > yout) break;
+        row(r + 2, a2, a0, a1);
     }
 }
 
@@ -612,6 +708,7 @@ struct Acts {
     bool grads = false;
     // forward
     Tensor x0, e1a, c1, p1, e2a, c2, p2, e3a, e3b, d2a, d2b, d1a, d1b, out;
+    Tensor e1b, e2b;  // the skip tensors on their own (forward without concatenation)
     // backward
     Tensor g_out, g_d1b, g_d1a, g_c1, g_d2b, g_d2a, g_c2, g_e3b, g_e3a, g_p2, g_e2a, g_p1, g_e1a, g_e1b, g_e2b, dz;
 };
@@ -721,7 +818,7 @@ static void acts_free(Acts &a)
     Tensor *all[] = {&a.x0,    &a.e1a,   &a.c1,    &a.p1,    &a.e2a,   &a.c2,    &a.p2,    &a.e3a,
                      &a.e3b,   &a.d2a,   &a.d2b,   &a.d1a,   &a.d1b,   &a.out,   &a.g_out, &a.g_d1b,
                      &a.g_d1a, &a.g_c1,  &a.g_d2b, &a.g_d2a, &a.g_c2,  &a.g_e3b, &a.g_e3a, &a.g_p2,
-                     &a.g_e2a, &a.g_p1,  &a.g_e1a, &a.g_e1b, &a.g_e2b, &a.dz};
+                     &a.g_e2a, &a.g_p1,  &a.g_e1a, &a.g_e1b, &a.g_e2b, &a.dz,    &a.e1b,   &a.e2b};
     for (Tensor *t : all) tfree(*t);
     a.B = a.H = a.W = 0;
 }
@@ -743,7 +840,7 @@ static int acts_ensure(tl_ctx *c, Unet *u, int B, int H, int W, bool grads)
     } f[] = {{&a.x0, H, W, 16},   {&a.e1a, H, W, 16},    {&a.c1, H, W, 48},    {&a.p1, H2, W2, 16},
              {&a.e2a, H2, W2, 32}, {&a.c2, H2, W2, 96},  {&a.p2, H4, W4, 32},  {&a.e3a, H4, W4, 64},
              {&a.e3b, H4, W4, 64}, {&a.d2a, H2, W2, 32}, {&a.d2b, H2, W2, 32}, {&a.d1a, H, W, 16},
-             {&a.d1b, H, W, 16},   {&a.out, H, W, 16}};
+             {&a.d1b, H, W, 16},   {&a.out, H, W, 16},   {&a.e1b, H, W, 16},   {&a.e2b, H2, W2, 32}};
     for (auto &q : f)
         if (int rc = talloc(c, *q.t, B, q.h, q.w, q.ch)) {
             acts_free(a);
@@ -771,17 +868,28 @@ static unsigned blocks(long n, int bs = 256) { return (unsigned)((n + bs - 1) / 
 // so full-grid tensors of 16384^2 x 64 channels need the loop
 static unsigned gs_blocks(long n, int bs = 256) { return (unsigned)std::min<long>((n + bs - 1) / bs, 1L << 20); }
 
+// out = conv(in) (same size); or, with skip, out = conv([up2(in) | skip])
+// with csplit channels from in (upsampled 2x when upin)
 static hipError_t conv(tl_ctx *c, const TV &in, const TV &out, const bf16 *w, const float *bias, int cinp, int coutp,
-                       int relu)
+                       int relu, const TV *skip = nullptr, int csplit = 0, int upin = 0)
 {
-    const dim3 grid((in.W + 255) / 256, in.H, in.B);
-    switch (coutp) {
-    case 16: k_conv<1><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
-    case 32: k_conv<2><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
-    case 48: k_conv<3><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
-    case 64: k_conv<4><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
-    case 96: k_conv<6><<<grid, 256, 0, c->stream>>>(in, out, w, bias, cinp, relu); break;
-    default: return hipErrorInvalidValue;
+    if (coutp % 16 || coutp < 16 || cinp % 16 || cinp < 16 || cinp > 96) return hipErrorInvalidValue;
+    if (skip && (csplit % 16 || csplit <= 0 || csplit >= cinp)) return hipErrorInvalidValue;
+    // 32 output channels per launch (16 for the last 16 of 48): the three
+    // rolling accumulator sets stay at 96 registers
+    const int gx = (out.W + 255) / 256;
+    int R = TL_CONV_ROWS;
+    while (R > 4 && (long)gx * ((out.H + R - 1) / R) * out.B < TL_CONV_BLOCKS) R /= 2;
+    const dim3 grid(gx, (out.H + R - 1) / R, out.B);
+    const TV &sk = skip ? *skip : in;
+    const int cs = skip ? csplit : cinp;
+    for (int co0 = 0; co0 < coutp; co0 += 32) {
+        const int nt = coutp - co0 >= 32 ? 2 : 1;
+        const size_t lds = (size_t)9 * 16 * nt * cinp * sizeof(bf16);  // at most 55 KB (96 -> 32)
+        if (nt == 2)
+            k_conv<2><<<grid, 256, lds, c->stream>>>(in, sk, cs, upin, out, w, bias, cinp, coutp, co0, relu, R);
+        else
+            k_conv<1><<<grid, 256, lds, c->stream>>>(in, sk, cs, upin, out, w, bias, cinp, coutp, co0, relu, R);
     }
     return hipGetLastError();
 }
@@ -802,23 +910,46 @@ static int refresh_weights(tl_ctx *c, Unet *u)
 }
 
 // forward pass over acts.x0
-static int forward(tl_ctx *c, Unet *u)
+// The forward pass.  With concat (a backward pass follows), the decoders'
+// inputs are materialised as the concatenations c1 = [up(d2b) | e1b] and
+// c2 = [up(e3b) | e2b] that the weight gradients read; without, the decoder
+// convolutions read the low-resolution tensor (upsampled on the fly) and the
+// skip tensor in place, and the pools read the skip tensors whole.
+static int forward(tl_ctx *c, Unet *u, bool concat)
 {
     Acts &a = u->acts;
     const long n1 = (long)a.B * a.H * a.W, n2 = n1 / 4;
     UCHK(c, fwd_layer(c, u, 0, a.x0.v(), a.e1a.v()));
-    UCHK(c, fwd_layer(c, u, 1, a.e1a.v(), a.c1.v(32)));
-    k_pool<<<gs_blocks(n2 * 2), 256, 0, c->stream>>>(a.c1.v(32), a.p1.v(), 16);
-    UCHK(c, fwd_layer(c, u, 2, a.p1.v(), a.e2a.v()));
-    UCHK(c, fwd_layer(c, u, 3, a.e2a.v(), a.c2.v(64)));
-    k_pool<<<gs_blocks(n2 / 4 * 4), 256, 0, c->stream>>>(a.c2.v(64), a.p2.v(), 32);
+    if (concat) {
+        UCHK(c, fwd_layer(c, u, 1, a.e1a.v(), a.c1.v(32)));
+        k_pool<<<gs_blocks(n2 * 2), 256, 0, c->stream>>>(a.c1.v(32), a.p1.v(), 16);
+        UCHK(c, fwd_layer(c, u, 2, a.p1.v(), a.e2a.v()));
+        UCHK(c, fwd_layer(c, u, 3, a.e2a.v(), a.c2.v(64)));
+        k_pool<<<gs_blocks(n2 / 4 * 4), 256, 0, c->stream>>>(a.c2.v(64), a.p2.v(), 32);
+    } else {
+        UCHK(c, fwd_layer(c, u, 1, a.e1a.v(), a.e1b.v()));
+        k_pool<<<gs_blocks(n2 * 2), 256, 0, c->stream>>>(a.e1b.v(), a.p1.v(), 16);
+        UCHK(c, fwd_layer(c, u, 2, a.p1.v(), a.e2a.v()));
+        UCHK(c, fwd_layer(c, u, 3, a.e2a.v(), a.e2b.v()));
+        k_pool<<<gs_blocks(n2 / 4 * 4), 256, 0, c->stream>>>(a.e2b.v(), a.p2.v(), 32);
+    }
     UCHK(c, fwd_layer(c, u, 4, a.p2.v(), a.e3a.v()));
     UCHK(c, fwd_layer(c, u, 5, a.e3a.v(), a.e3b.v()));
-    k_up<<<gs_blocks(n2 * 8), 256, 0, c->stream>>>(a.e3b.v(), a.c2.v(0), 64);
-    UCHK(c, fwd_layer(c, u, 6, a.c2.v(), a.d2a.v()));
+    if (concat) {
+        k_up<<<gs_blocks(n2 * 8), 256, 0, c->stream>>>(a.e3b.v(), a.c2.v(0), 64);
+        UCHK(c, fwd_layer(c, u, 6, a.c2.v(), a.d2a.v()));
+    } else {
+        const TV sk = a.e2b.v();
+        UCHK(c, conv(c, a.e3b.v(), a.d2a.v(), u->wb[6], u->b[6], LAY[6].cinp, LAY[6].coutp, LAY[6].relu, &sk, 64, 1));
+    }
     UCHK(c, fwd_layer(c, u, 7, a.d2a.v(), a.d2b.v()));
-    k_up<<<gs_blocks(n1 * 4), 256, 0, c->stream>>>(a.d2b.v(), a.c1.v(0), 32);
-    UCHK(c, fwd_layer(c, u, 8, a.c1.v(), a.d1a.v()));
+    if (concat) {
+        k_up<<<gs_blocks(n1 * 4), 256, 0, c->stream>>>(a.d2b.v(), a.c1.v(0), 32);
+        UCHK(c, fwd_layer(c, u, 8, a.c1.v(), a.d1a.v()));
+    } else {
+        const TV sk = a.e1b.v();
+        UCHK(c, conv(c, a.d2b.v(), a.d1a.v(), u->wb[8], u->b[8], LAY[8].cinp, LAY[8].coutp, LAY[8].relu, &sk, 32, 1));
+    }
     UCHK(c, fwd_layer(c, u, 9, a.d1a.v(), a.d1b.v()));
     UCHK(c, fwd_layer(c, u, 10, a.d1b.v(), a.out.v()));
     return TL_OK;
@@ -962,7 +1093,7 @@ int unet_predict_guess(tl_ctx *c, double, double, int, double *guess_dev)
                                                         c->f[TL_F_KX], c->f[TL_F_KY], c->u_alt, u->scale,
                                                         u->acts.x0.v(), 0, 0, 0);
     UCHK(c, hipGetLastError());
-    if (int rc = forward(c, u)) return rc;
+    if (int rc = forward(c, u, false)) return rc;
     k_guess<<<dim3(blocks(g.nx), g.ny), 256, 0, c->stream>>>(g, c->f[TL_F_U], u->acts.out.v(), u->scale, guess_dev);
     UCHK(c, hipGetLastError());
     return TL_OK;
@@ -1183,7 +1314,7 @@ extern "C" int tl_unet_train(tl_ctx *c, const tl_train_params *tp, float *loss_o
             UCHK(c, hipMemcpyAsync(u->bt_m + b * px, u->ds_m + (size_t)s * px, px * sizeof(float),
                                    hipMemcpyDeviceToDevice, c->stream));
         }
-        if (int rc = forward(c, u)) return rc;
+        if (int rc = forward(c, u, true)) return rc;
         // scale = 1 / (number of scored pixels) (from the masks of the batch)
         UCHK(c, hipMemcpyAsync(mk.data(), u->bt_m, npx * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         UCHK(c, hipStreamSynchronize(c->stream));
@@ -1265,7 +1396,7 @@ extern "C" int tl_unet_eval(tl_ctx *c, const float *x, const float *target, int 
                     hx[((((size_t)b * (H + 2) + y + 1) * (W + 2)) + xx + 1) * 16 + ch] =
                         tobf(x[(((size_t)b * H + y) * W + xx) * 6 + ch]);
     UCHK(c, hipMemcpyAsync(u->acts.x0.p, hx.data(), hx.size() * sizeof(bf16), hipMemcpyHostToDevice, c->stream));
-    if (int rc = forward(c, u)) return rc;
+    if (int rc = forward(c, u, want_grad)) return rc;
     std::vector<bf16> ho(u->acts.out.bytes() / sizeof(bf16));
     UCHK(c, hipMemcpyAsync(ho.data(), u->acts.out.p, ho.size() * sizeof(bf16), hipMemcpyDeviceToHost, c->stream));
     UCHK(c, hipStreamSynchronize(c->stream));

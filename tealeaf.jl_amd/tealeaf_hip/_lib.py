@@ -46,7 +46,7 @@ TL_WARM_POLICY_BACKOFF, TL_WARM_POLICY_EVERY = 0, 1
 KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, false>", 2: "k_cg_p<double>",
                 3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
                 5: "(fused into k_cgg<4, false, 2, true, 1>)",
-                6: "k_vs64<0, 2, true>", 7: "k_vs64<1, 2, true>", 8: "k_vs64<2, 2, true>", 9: "k_residual",
+                6: "k_vs64<0, 4, true>", 7: "k_vs64<1, 4, true>", 8: "k_vs64<2, 4, true>", 9: "k_residual",
                 10: "k_cgf<1, 1>",
                 -1: "k_copy_full", -2: "k_copy_vec"}
 

@@ -47,17 +47,21 @@ KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, fa
                 3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
                 5: "(fused into k_cgg<4, false, 2, true, 1>)",
                 6: "k_vs64<0, 4, true>", 7: "k_vs64<1, 4, true>", 8: "k_vs64<2, 4, true>", 9: "k_residual",
-                10: "k_cgf<1, 1>",
+                10: "k_cgf<1, 1, 0>",
                 -1: "k_copy_full", -2: "k_copy_vec"}
 
 
 def kernel_name(kid, coef_format=None):
     """Symbol of profiled kernel `kid`.  The FP32 kernels' last template
     argument is the cell coefficient's 16-bit format (1 = scaled half,
-    0 = bfloat16), which the solver picks per solve (`SolveStats.coef_format`);
-    without a format the half instantiation is named."""
+    0 = bfloat16; the fused kernel's second, before its LDS staging depth,
+    0 in the default build), which the solver picks per solve
+    (`SolveStats.coef_format`); without a format the half instantiation is
+    named."""
     name = KERNEL_NAMES[kid]
-    if coef_format == TL_COEF_BF16 and kid in (TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32, TL_K_CGF32):
+    if coef_format == TL_COEF_BF16 and kid == TL_K_CGF32:
+        name = name.replace("<1, 1,", "<1, 0,")
+    elif coef_format == TL_COEF_BF16 and kid in (TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32):
         name = name.replace(", 1>", ", 0>")
     return name
 

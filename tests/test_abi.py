@@ -38,6 +38,34 @@ def test_struct_sizes_match_header():
     assert p.eps == 1e-15 and p.epslim == 1e-5 and p.rel_tol == 1e-10 and p.check_result == 1
 
 
+def test_struct_layouts_match_header(tmp_path):
+    # every member offset and struct size of the ctypes mirrors against the
+    # header as a C compiler lays it out (gcc, the same ABI as the library)
+    mirrors = {"tl_solve_params": _lib.SolveParams, "tl_solve_stats": _lib.SolveStats, "tl_state": _lib.State,
+               "tl_deck": _lib.Deck, "tl_train_params": _lib.TrainParams}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tealeaf_hip.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            cname, fname, v = ln.split()
+            got[(cname, fname)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
 @pytest.mark.parametrize("text", [
     decks.reference_deck(512, 10),
     decks.reference_deck(4096, 20, solver="ppcg", presteps=8),
@@ -147,6 +175,8 @@ def test_kernel_names_follow_coefficient_format():
     assert _lib.kernel_name(_lib.TL_K_CG_UR32, _lib.TL_COEF_BF16) == "k_cgg<4, true, 4, true, 0>"
     assert _lib.kernel_name(_lib.TL_K_CG_W32, _lib.TL_COEF_BF16) == "k_cgg<4, false, 2, true, 0>"
     assert _lib.kernel_name(_lib.TL_K_CG_UR64, _lib.TL_COEF_BF16) == _lib.KERNEL_NAMES[_lib.TL_K_CG_UR64]
+    assert _lib.kernel_name(_lib.TL_K_CGF32) == "k_cgf<1, 1, 0>"
+    assert _lib.kernel_name(_lib.TL_K_CGF32, _lib.TL_COEF_BF16) == "k_cgf<1, 0, 0>"
 
 
 def test_benchmark_problem_is_fixed():

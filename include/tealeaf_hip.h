@@ -109,7 +109,8 @@ typedef struct {
     int launch;           /* TL_LAUNCH_* */
     int fp32_kernels;     /* TL_FP32_IR: TL_FP32_KERNELS_* */
     int warm_policy;      /* TL_WARM_LEARNED: TL_WARM_POLICY_* */
-    int reserved[5];
+    int ppcg_pairs;       /* TL_PPCG: TL_PPCG_PAIRS_* */
+    int reserved[4];
 } tl_solve_params;
 
 /* tl_solve_params.warm_policy: when a TL_WARM_LEARNED solve runs the network.
@@ -119,6 +120,11 @@ typedef struct {
                                     prediction; after a rejection the next attempt waits 2, 4, 8,
                                     then 16 solves (a non-attempted solve starts from u) */
 #define TL_WARM_POLICY_EVERY 1   /* every solve */
+
+/* tl_solve_params.ppcg_pairs: PPCG's inner Chebyshev steps (PPCG.jl:78-83)
+ * two per sweep over the grid, or one.  Bit-identical results. */
+#define TL_PPCG_PAIRS_AUTO 0 /* two steps per sweep when ppcg_inner_steps >= 4 (an odd last step alone) */
+#define TL_PPCG_PAIRS_OFF 1  /* one step per sweep */
 
 /* tl_solve_params.fp32_kernels: how a TL_FP32_IR iteration is split into kernels.
  * Both reach the same FP64 tolerance; their FP32 roundings differ. */
@@ -296,7 +302,8 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 #define TL_K_JACOBI64 8 /* Jacobi sweep (the sweeps that do not also form the residual) */
 #define TL_K_RESID64 9 /* residual! r = u0 - A u (also the FP64 refresh of TL_FP32_IR) */
 #define TL_K_CGF32 10  /* FP32 inner CG iteration as one kernel (TL_FP32_KERNELS_FUSED) */
-#define TL_K_COUNT 11
+#define TL_K_PPCG2_64 11 /* two PPCG inner steps in one sweep (TL_PPCG_PAIRS_AUTO) */
+#define TL_K_COUNT 12
 /* Bracket every period-th launch of each kernel in kernel_mask (bit =
  * 1 << TL_K_*; period >= 1) with HIP events on the context's stream (an event
  * pair costs a few microseconds of GPU time, so a period of about 16 keeps the

@@ -154,7 +154,7 @@ extern "C" void tl_destroy(tl_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     unet_destroy(c);
-    void *bufs[] = {c->u_alt, c->sd_alt, c->p_alt, c->p32b, c->guess, c->gcell, c->gc_range, c->p32, c->r32, c->x32,
+    void *bufs[] = {c->u_alt, c->sd_alt, c->r_alt1, c->r_alt2, c->p_alt, c->p32b, c->guess, c->gcell, c->gc_range, c->p32, c->r32, c->x32,
                     c->r32b, c->part_f, c->partials, c->ctrl, c->hist_a, c->hist_b, c->states_d};
     for (int i = 0; i < TL_NFIELDS; ++i)
         if (c->f[i]) (void)hipFree(c->f[i]);
@@ -849,6 +849,14 @@ static int solve_ppcg(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
     HIPCHK(c, launch_residual(c));
     HIPCHK(c, launch_set_mode(c, MODE_PPCG, p->max_iters));
     double *sd0 = c->f[TL_F_SD], *sd1 = c->sd_alt;
+    // two inner steps per sweep (r through two spare buffers, back in r's
+    // own at the end of every outer iteration), an odd last step alone
+    const bool pairs = p->ppcg_pairs == TL_PPCG_PAIRS_AUTO && inner >= 4;
+    if (pairs) {
+        if (!c->r_alt1) RETIF(alloc_d(c, &c->r_alt1, c->g.elems));
+        if (!c->r_alt2) RETIF(alloc_d(c, &c->r_alt2, c->g.elems));
+    }
+    const int sd_moves = pairs ? inner / 2 + (inner & 1) : inner;  // sweeps per outer iteration
     int t = cgiters;
     RETIF(run_until_stop(c, 16, (long)p->max_iters + 16,
                          [&]() -> hipError_t {
@@ -856,7 +864,19 @@ static int solve_ppcg(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
                              hipError_t e = launch_cg_w64(c, t);
                              if (e == hipSuccess) e = launch_ppcg_ur_sd(c, theta, t);
                              double *a = sd0, *b = sd1;
-                             for (int pp = 0; pp < inner && e == hipSuccess; ++pp) {
+                             int pp = 0;
+                             if (pairs) {
+                                 const int np = inner / 2;
+                                 const double *rs = c->f[TL_F_R];
+                                 for (int q = 0; q < np && e == hipSuccess; ++q, pp += 2) {
+                                     double *rd = q == np - 1 ? c->f[TL_F_R] : (q & 1) ? c->r_alt2 : c->r_alt1;
+                                     e = launch_ppcg_inner2(c, ca[pp], cb[pp], ca[pp + 1], cb[pp + 1], a, b, rs, rd,
+                                                            pp + 2 == inner, t);
+                                     std::swap(a, b);
+                                     rs = rd;
+                                 }
+                             }
+                             for (; pp < inner && e == hipSuccess; ++pp) {
                                  e = launch_ppcg_inner(c, ca[pp], cb[pp], a, b, pp == inner - 1, t);
                                  std::swap(a, b);
                              }
@@ -865,7 +885,7 @@ static int solve_ppcg(tl_ctx *c, const tl_solve_params *p, double rx, double ry,
                              return e;
                          },
                          &k));
-    if (inner & 1) std::swap(c->f[TL_F_SD], c->sd_alt);
+    if (sd_moves & 1) std::swap(c->f[TL_F_SD], c->sd_alt);
     st->cheby_iters = k.stop_iter - cgiters;
     st->iterations = k.stop_iter;
     fill_stats(k, st);
@@ -1051,12 +1071,19 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
         // one FP64 solver kernel, `reps` launches on the CG-initialised fields
         // (u, u0, p, r, kx, ky) with fixed coefficients; never stopped
         HIPCHK(c, launch_set_mode(c, MODE_JACOBI, 1 << 30));
+        if (kernel_id == TL_K_PPCG2_64 && !c->r_alt1) RETIF(alloc_d(c, &c->r_alt1, c->g.elems));
         double *a = c->f[TL_F_U], *b = c->u_alt, *s0 = c->f[TL_F_SD], *s1 = c->sd_alt;
+        double *r0 = c->f[TL_F_R], *r1 = c->r_alt1;
         for (int i = 0; i < reps; ++i) {
             const int t = 1 + i;
             switch (kernel_id) {
             case TL_K_CHEBY64: HIPCHK(c, launch_cheby_iter(c, 0.5, 1e-3, a, b, 0, t)); std::swap(a, b); break;
             case TL_K_PPCG64: HIPCHK(c, launch_ppcg_inner(c, 0.5, 1e-3, s0, s1, 0, t)); std::swap(s0, s1); break;
+            case TL_K_PPCG2_64:
+                HIPCHK(c, launch_ppcg_inner2(c, 0.5, 1e-3, 0.5, 1e-3, s0, s1, r0, r1, 0, t));
+                std::swap(s0, s1);
+                std::swap(r0, r1);
+                break;
             case TL_K_JACOBI64: HIPCHK(c, launch_jacobi(c, a, b, t, 0)); std::swap(a, b); break;
             default: HIPCHK(c, launch_residual(c)); break;
             }
@@ -1123,6 +1150,10 @@ extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double r
     }
     if (p->warm_policy != TL_WARM_POLICY_BACKOFF && p->warm_policy != TL_WARM_POLICY_EVERY) {
         c->err = "tl_solve: warm_policy must be TL_WARM_POLICY_BACKOFF or TL_WARM_POLICY_EVERY";
+        return TL_E_INVALID;
+    }
+    if (p->ppcg_pairs != TL_PPCG_PAIRS_AUTO && p->ppcg_pairs != TL_PPCG_PAIRS_OFF) {
+        c->err = "tl_solve: ppcg_pairs must be TL_PPCG_PAIRS_AUTO or TL_PPCG_PAIRS_OFF";
         return TL_E_INVALID;
     }
     if (p->warm_start != TL_WARM_NONE && p->solver != TL_CG) {

@@ -230,6 +230,7 @@ struct tl_ctx {
     // FP64 fields (reference Chunk fields; u and sd have ping-pong twins)
     double *f[TL_NFIELDS] = {};
     double *u_alt = nullptr, *sd_alt = nullptr, *guess = nullptr;
+    double *r_alt1 = nullptr, *r_alt2 = nullptr;  // PPCG: r between two-step sweeps
     // FP32 fields of the mixed-precision solve
     // FP32 inner CG: cell coefficient gc = 1/w (tl_gc_t) and its face scale
     // (rx/2, ry/2), search direction, residual, correction
@@ -336,6 +337,8 @@ hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double
 hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t);
 hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t);
 hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t);
+hipError_t launch_ppcg_inner2(tl_ctx *c, double a1, double b1, double a2, double b2, const double *sd_in,
+                              double *sd_out, const double *r_in, double *r_out, int last, int t);
 hipError_t launch_jacobi(tl_ctx *c, const double *u_in, double *u_out, int t, int defer);
 hipError_t launch_ir_prepare(tl_ctx *c, double rx, double ry, int coefficient);
 hipError_t launch_ir_accumulate(tl_ctx *c, int T, const float *pT = nullptr);

@@ -1567,7 +1567,9 @@ struct alignas(16) FSlot {
     unsigned e[16];                 // outside columns (words 0-4 left, 8-12 right)
 };
 constexpr unsigned FSLOT_P = 1024, FSLOT_X = 2048, FSLOT_G = 3072, FSLOT_E = 3584;
-static_assert(sizeof(FSlot) == 3648, "slot layout");
+static_assert(sizeof(FSlot) == 3648 && offsetof(FSlot, p) == FSLOT_P && offsetof(FSlot, x) == FSLOT_X &&
+                  offsetof(FSlot, g) == FSLOT_G && offsetof(FSlot, e) == FSLOT_E,
+              "slot layout (the reads below use these offsets)");
 #define TL_LDS(ptr) ((__attribute__((address_space(3))) void *)(ptr))
 
 typedef float fv4_t __attribute__((ext_vector_type(4)));
@@ -2427,6 +2429,182 @@ __global__ __launch_bounds__(TBV) void k_vs64(Geo g, int ty, int t, const double
     if (sums) write_partials<1, TBV>(acc, part, g.nbmax);
 }
 
+// Two PPCG inner steps (PPCG.jl:78-83, sd ping-ponged) in one sweep.  The
+// inner steps have no reduction between them, so step 2 can follow step 1
+// a row behind in the same pass: a thread walks its band keeping sd0 at rows
+// L-2..L and sd1 at rows L-3..L-1 (with the column next to the wave for the
+// lanes at its edges), and at row L
+//   step 1 at row M = L-1:  r1 = r0 - A sd0,  sd1 = a1 sd0 + b1 r1
+//   step 2 at row N = L-2:  r2 = r1 - A sd1,  sd2 = a2 sd1 + b2 r2,
+//                           u2 = (u0 + sd0) + sd1
+// with the single step's operations and association (k_vs64, VS_PPCG), so
+// the result is bit-identical to two launches of it.  Lanes 0 / 63 compute
+// step 1 also at the column next to the wave (hcol), from sd0 one column
+// further (fcol), as the neighbouring wave does for its own column.  Reads
+// sd0, r0, u, kx, ky once and writes sd2, r2, u2 once: 64 B/cell for two
+// steps instead of 128.  r is read at neighbours' rows and columns by other
+// threads, so it is not updated in place (rin -> rout); u is (own cell only).
+__global__ __launch_bounds__(TBV) void k_ppcg2(Geo g, int ty, int t, const double *__restrict__ sdin,
+                                               double *__restrict__ sdout, const double *__restrict__ rin,
+                                               double *__restrict__ rout, double *__restrict__ u,
+                                               const double *__restrict__ kx, const double *__restrict__ ky,
+                                               double a1, double b1, double a2, double b2, int sums,
+                                               const Ctrl *__restrict__ ctrl, double *__restrict__ part)
+{
+    constexpr int V = 2;
+    typedef double T;
+    if (t > ctrl->stop_iter) return;
+    const VCol<V> cl = vcolumn<V>(g, ty);
+    const int xl = cl.xl, lane = cl.lane;
+    double acc[1] = {0.0};
+    if (cl.y0 < g.ny) {
+        const bool eleft = lane == 0 && xl > 0;
+        const bool eright = lane == 63 && xl + V < g.nx;
+        const bool edge = eleft || eright;
+        const int hc = lane == 0 ? xl - 1 : xl + V;                       // hcol
+        const int hcc = edge ? hc : xl;                                     // a valid column to load
+        const int fcc = eleft ? xl - 2 : eright ? min(xl + V + 1, g.nx - 1) : xl;
+        // one row's loads: sd0 at row L (own, hcol, fcol); r0, kx at row L-1;
+        // ky at row L (the face below it); u at row L-2
+        struct Row {
+            T s[V], r[V], kx[V], ky[V], u[V];
+            T es, ef, er, ekx0, ekx1, eky;
+        };
+        auto load_row = [&](Row &B, int L) {
+            const int ys = min(max(L, 0), g.ny - 1), yr = min(max(L - 1, 0), g.ny - 1);
+            const int yk = min(max(L, 0), g.ny), yu = min(max(L - 2, 0), g.ny - 1);
+            vload<T, V>(B.s, sdin + at(g, xl, ys));
+            vload<T, V>(B.r, rin + at(g, xl, yr));
+            vload<T, V>(B.kx, kx + at(g, xl, yr));
+            vload<T, V>(B.ky, ky + at(g, xl, yk));
+            vload<T, V>(B.u, u + at(g, xl, yu));
+            B.es = B.ef = B.er = B.ekx1 = B.eky = (T)0;
+            // lane 63's right face of its last column, as the single step loads it
+            B.ekx0 = lane == 63 ? kx[at(g, xl, yr) + V] : (T)0;
+            if (edge) {
+                B.es = sdin[at(g, hcc, ys)];
+                B.ef = sdin[at(g, fcc, ys)];
+                B.er = rin[at(g, hcc, yr)];
+                B.eky = ky[at(g, hcc, yk)];
+                if (lane == 0) B.ekx0 = kx[at(g, xl - 1, yr)];               // left face of hcol
+                else B.ekx1 = kx[at(g, min(xl + V + 1, g.nx), yr)];          // right face of hcol
+            }
+        };
+        // the window at row L, before step 1 at row M: sd0 at rows N, M (own,
+        // hcol) and fcol at M; kx at N, ky at N and M; sd1 at rows N-1, N
+        // (own; hcol at N) and r1 at N, from the two previous rows
+        T S0m[V], S0c[V], SHm = 0, SHc = 0, EFc = 0;
+        T KXm[V], EKXm = 0, KYm[V], KYc[V], EKYc = 0;
+        T S1b[V], S1c[V], H1c = 0, R1c[V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) S0m[j] = S0c[j] = KXm[j] = KYm[j] = KYc[j] = S1b[j] = S1c[j] = R1c[j] = (T)0;
+        const int y0 = cl.y0, y1 = cl.y1;
+        auto step = [&](const Row &B, int L) {
+            const int M = L - 1, N = L - 2;
+            // step 1 at row M (own columns, and hcol on the edge lanes)
+            T s1[V], r1[V], h1 = 0;
+            {
+                const bool ybot = M <= 0, ytop = M + 1 >= g.ny;
+                T lv = lane_up1<false>(S0c[V - 1]), rv = lane_down1<false>(S0c[0]);
+                if (lane == 0) lv = xl > 0 ? SHc : S0c[0];
+                if (lane == 63) rv = xl + V < g.nx ? SHc : S0c[V - 1];
+                if (xl == 0) lv = S0c[0];
+                T kxn = lane_down1<false>(B.kx[0]);
+                if (lane == 63) kxn = B.ekx0;
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const T kxl = B.kx[j], kxr = j < V - 1 ? B.kx[j + 1] : kxn;
+                    const T dn = ybot ? S0c[j] : S0m[j], up = ytop ? S0c[j] : B.s[j];
+                    const T w = vsmvp<T, V>(g, xl, j, S0c, lv, rv, dn, up, kxl, kxr, KYc[j], B.ky[j]);
+                    r1[j] = B.r[j] - w;
+                    s1[j] = a1 * S0c[j] + b1 * r1[j];
+                }
+                if (edge) {
+                    const T c = SHc;
+                    T l, r, kxl, kxr;
+                    if (lane == 0) {
+                        l = hc == 0 ? c : EFc;
+                        r = S0c[0];
+                        kxl = B.ekx0;
+                        kxr = B.kx[0];
+                    } else {
+                        l = S0c[V - 1];
+                        r = hc + 1 >= g.nx ? c : EFc;
+                        kxl = B.ekx0;
+                        kxr = B.ekx1;
+                    }
+                    const T dn = ybot ? c : SHm, up = ytop ? c : B.es;
+                    const T w = smvp5(c, l, r, dn, up, kxl, kxr, EKYc, B.eky);
+                    const T rh = B.er - w;
+                    h1 = a1 * c + b1 * rh;
+                }
+            }
+            // step 2 at row N (own columns), from sd1 at rows N-1, N, M
+            if (L >= y0 + 2 && N < y1) {
+                const bool ybot = N <= 0, ytop = N + 1 >= g.ny;
+                T lv = lane_up1<false>(S1c[V - 1]), rv = lane_down1<false>(S1c[0]);
+                if (lane == 0) lv = xl > 0 ? H1c : S1c[0];
+                if (lane == 63) rv = xl + V < g.nx ? H1c : S1c[V - 1];
+                if (xl == 0) lv = S1c[0];
+                T kxn = lane_down1<false>(KXm[0]);
+                if (lane == 63) kxn = EKXm;
+                T o1[V], o2[V], o3[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const T kxl = KXm[j], kxr = j < V - 1 ? KXm[j + 1] : kxn;
+                    const T dn = ybot ? S1c[j] : S1b[j], up = ytop ? S1c[j] : s1[j];
+                    const T w = vsmvp<T, V>(g, xl, j, S1c, lv, rv, dn, up, kxl, kxr, KYm[j], KYc[j]);
+                    const T rr = R1c[j] - w;
+                    o1[j] = a2 * S1c[j] + b2 * rr;          // sd2
+                    o2[j] = rr;                             // r2
+                    o3[j] = (B.u[j] + S0m[j]) + S1c[j];     // u2
+                }
+                if (cl.active) {
+                    const long ic = at(g, xl, N);
+                    const int nv = g.nx - cl.x0;
+                    vstore_n<T, V>(sdout + ic, o1, nv);
+                    vstore_n<T, V>(rout + ic, o2, nv);
+                    vstore_n<T, V>(u + ic, o3, nv);
+#pragma unroll
+                    for (int j = 0; j < V; ++j)
+                        if (j < nv) acc[0] += o2[j] * o2[j];
+                }
+            }
+            // shift the window by one row
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                S0m[j] = S0c[j];
+                S0c[j] = B.s[j];
+                KXm[j] = B.kx[j];
+                KYm[j] = KYc[j];
+                KYc[j] = B.ky[j];
+                S1b[j] = S1c[j];
+                S1c[j] = s1[j];
+                R1c[j] = r1[j];
+            }
+            SHm = SHc;
+            SHc = B.es;
+            EFc = B.ef;
+            EKXm = B.ekx0;
+            EKYc = B.eky;
+            H1c = h1;
+        };
+        // rows y0-2 .. y1+1, the next row's loads issued before the current
+        // row is computed
+        Row B0, B1;
+        load_row(B0, y0 - 2);
+        for (int L = y0 - 2; L <= y1 + 1; L += 2) {
+            const bool more = L + 1 <= y1 + 1;
+            if (more) load_row(B1, L + 1);
+            step(B0, L);
+            if (!more) break;
+            if (L + 2 <= y1 + 1) load_row(B0, L + 2);
+            step(B1, L + 1);
+        }
+    }
+    if (sums) write_partials<1, TBV>(acc, part, g.nbmax);
+}
+
 // --------------------------------------------------------------- Jacobi
 
 // Jacobi.iterate!, Jacobi.jl:59-80 (u' in a separate buffer, the reference's
@@ -2914,6 +3092,20 @@ hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in,
     k_vs64<VS_PPCG, TL_RV_VS64, TL_PIPE_VS64 != 0><<<nb2(c), TBV, 0, c->stream>>>(
         c->g, c->g.ty2, t, sd_in, sd_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_R], c->f[TL_F_U], a, b, last,
         c->ctrl, part_r(c, t));
+    if (prof) prof_stop(c);
+    if (last) nbr(c, t) = nb2(c);
+    return hipGetLastError();
+}
+
+// two inner steps (coefficients a1, b1 then a2, b2): sd_in -> sd_out,
+// r_in -> r_out (distinct buffers), u in place
+hipError_t launch_ppcg_inner2(tl_ctx *c, double a1, double b1, double a2, double b2, const double *sd_in,
+                              double *sd_out, const double *r_in, double *r_out, int last, int t)
+{
+    const bool prof = prof_start(c, TL_K_PPCG2_64);
+    k_ppcg2<<<nb2(c), TBV, 0, c->stream>>>(c->g, c->g.ty2, t, sd_in, sd_out, r_in, r_out, c->f[TL_F_U],
+                                           c->f[TL_F_KX], c->f[TL_F_KY], a1, b1, a2, b2, last, c->ctrl,
+                                           part_r(c, t));
     if (prof) prof_stop(c);
     if (last) nbr(c, t) = nb2(c);
     return hipGetLastError();

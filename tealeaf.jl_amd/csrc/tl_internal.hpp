@@ -107,6 +107,12 @@ constexpr int TBV = TL_TBV;
 // iteration against 0.311 ms for register loads; depth 2 (5 waves per SIMD)
 // and 4 (2 waves) 2-5% slower.  A band-walking copy with the same arrays and
 // geometry streams at 5.0 TB/s, so more rows in flight have little to win.
+// blocks of the two-step PPCG kernel on large grids (its launches that do
+// not form the residual sum; the one that does keeps the FP64 kernels'
+// bands, so that the sum is the single step's bit for bit)
+#ifndef TL_VEC_BLOCKS_PPCG2
+#define TL_VEC_BLOCKS_PPCG2 768
+#endif
 #ifndef TL_CGF_DEPTH
 #define TL_CGF_DEPTH 0
 #endif
@@ -177,6 +183,7 @@ struct Geo {
     int gyp4, typ4;            // FP32 PW kernel: gx4*gyp4 blocks of 4*TB columns x typ4 rows
     int gx2, gy2, ty2;         // FP64 vector kernels: blocks of 2*TB columns x ty2 rows
     int gyf, tyf;              // fused FP32 kernel: gx4*gyf blocks of 4*TB columns x tyf rows
+    int gyq, tyq;              // two-step PPCG kernel: gx2*gyq blocks of 2*TB columns x tyq rows
     int nbmax;                 // stride of the partial-sum arrays (max block count)
     size_t elems;              // elements per allocation
 };

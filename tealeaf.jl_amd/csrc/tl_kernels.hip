@@ -92,6 +92,15 @@ Geo make_geo(int nx, int ny, int hd)
         g.tyf = std::min(t2, 1024);
         g.gyf = (ny + g.tyf - 1) / g.tyf;
     }
+    {
+        // two-step PPCG kernel: about TL_VEC_BLOCKS_PPCG2 blocks (each band
+        // re-reads 2 rows below and 2 above it), bands of at least 16 rows
+        const long r2 = (long)ny * g.gx2;
+        int t2 = (int)((r2 + TL_VEC_BLOCKS_PPCG2 - 1) / TL_VEC_BLOCKS_PPCG2);
+        t2 = std::max(t2, std::min(16, ny));
+        g.tyq = std::min(t2, 1024);
+        g.gyq = (ny + g.tyq - 1) / g.tyq;
+    }
     g.nbmax = std::max(std::max(g.gx * g.gy, g.gx4 * g.gyp4), std::max(g.gx4 * g.gy4, g.gx2 * g.gy2));
     g.nbmax = std::max(g.nbmax, g.gx4 * g.gyf);
     return g;
@@ -3103,7 +3112,9 @@ hipError_t launch_ppcg_inner2(tl_ctx *c, double a1, double b1, double a2, double
                               double *sd_out, const double *r_in, double *r_out, int last, int t)
 {
     const bool prof = prof_start(c, TL_K_PPCG2_64);
-    k_ppcg2<<<nb2(c), TBV, 0, c->stream>>>(c->g, c->g.ty2, t, sd_in, sd_out, r_in, r_out, c->f[TL_F_U],
+    // the launch that forms the residual sum uses the FP64 kernels' bands
+    const unsigned nb = last ? nb2(c) : (unsigned)(c->g.gx2 * c->g.gyq);
+    k_ppcg2<<<nb, TBV, 0, c->stream>>>(c->g, last ? c->g.ty2 : c->g.tyq, t, sd_in, sd_out, r_in, r_out, c->f[TL_F_U],
                                            c->f[TL_F_KX], c->f[TL_F_KY], a1, b1, a2, b2, last, c->ctrl,
                                            part_r(c, t));
     if (prof) prof_stop(c);

@@ -343,7 +343,8 @@ def unet_time(n, seed, reps):
 # tl_kernel_repeat runs it, writing the FP64 r)
 TABLE_BYTES = dict(KERNEL_BYTES)
 TABLE_BYTES.update({_lib.TL_K_CHEBY64: 56, _lib.TL_K_PPCG64: 64, _lib.TL_K_JACOBI64: 40, _lib.TL_K_RESID64: 40,
-                    _lib.TL_K_PPCG2_64: 64})  # two PPCG inner steps: 64 B/cell per launch
+                    _lib.TL_K_PPCG2_64: 64,   # two PPCG inner steps: 64 B/cell per launch
+                    _lib.TL_K_CHEBY2_64: 56})  # two Chebyshev iterations: 56 B/cell per launch
 
 
 def kernel_table(n, seed, reps):
@@ -357,7 +358,7 @@ def kernel_table(n, seed, reps):
               (_lib.TL_K_CG_W32, (_lib.TL_K_CG_W32, _lib.TL_K_CG_UR32)),
               (_lib.TL_K_CGF32, (_lib.TL_K_CGF32,)),
               (_lib.TL_K_CHEBY64, (_lib.TL_K_CHEBY64,)), (_lib.TL_K_PPCG64, (_lib.TL_K_PPCG64,)),
-              (_lib.TL_K_PPCG2_64, (_lib.TL_K_PPCG2_64,)),
+              (_lib.TL_K_PPCG2_64, (_lib.TL_K_PPCG2_64,)), (_lib.TL_K_CHEBY2_64, (_lib.TL_K_CHEBY2_64,)),
               (_lib.TL_K_JACOBI64, (_lib.TL_K_JACOBI64,)), (_lib.TL_K_RESID64, (_lib.TL_K_RESID64,))]
     s = make_settings(n, seed, "fp64", "none")
     chunk = T.initialiseapp(s)

@@ -110,7 +110,8 @@ typedef struct {
     int fp32_kernels;     /* TL_FP32_IR: TL_FP32_KERNELS_* */
     int warm_policy;      /* TL_WARM_LEARNED: TL_WARM_POLICY_* */
     int ppcg_pairs;       /* TL_PPCG: TL_PPCG_PAIRS_* */
-    int reserved[4];
+    int cheby_pairs;      /* TL_CHEBY: TL_CHEBY_PAIRS_* */
+    int reserved[3];
 } tl_solve_params;
 
 /* tl_solve_params.warm_policy: when a TL_WARM_LEARNED solve runs the network.
@@ -125,6 +126,11 @@ typedef struct {
  * two per sweep over the grid, or one.  Bit-identical results. */
 #define TL_PPCG_PAIRS_AUTO 0 /* two steps per sweep when ppcg_inner_steps >= 4 (an odd last step alone) */
 #define TL_PPCG_PAIRS_OFF 1  /* one step per sweep */
+
+/* tl_solve_params.cheby_pairs: Chebyshev iterations (Cheby.jl:93-107) two per
+ * sweep over the grid, or one.  Bit-identical results. */
+#define TL_CHEBY_PAIRS_AUTO 0 /* two iterations per sweep where the first needs no residual norm */
+#define TL_CHEBY_PAIRS_OFF 1  /* one iteration per sweep */
 
 /* tl_solve_params.fp32_kernels: how a TL_FP32_IR iteration is split into kernels.
  * Both reach the same FP64 tolerance; their FP32 roundings differ. */
@@ -303,7 +309,8 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 #define TL_K_RESID64 9 /* residual! r = u0 - A u (also the FP64 refresh of TL_FP32_IR) */
 #define TL_K_CGF32 10  /* FP32 inner CG iteration as one kernel (TL_FP32_KERNELS_FUSED) */
 #define TL_K_PPCG2_64 11 /* two PPCG inner steps in one sweep (TL_PPCG_PAIRS_AUTO) */
-#define TL_K_COUNT 12
+#define TL_K_CHEBY2_64 12 /* two Chebyshev iterations in one sweep (TL_CHEBY_PAIRS_AUTO) */
+#define TL_K_COUNT 13
 /* Bracket every period-th launch of each kernel in kernel_mask (bit =
  * 1 << TL_K_*; period >= 1) with HIP events on the context's stream (an event
  * pair costs a few microseconds of GPU time, so a period of about 16 keeps the

@@ -783,28 +783,49 @@ static int solve_cheby(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     }
     HIPCHK(c, launch_set_mode(c, MODE_JACOBI, maxiters));
     double *ucur = c->f[TL_F_U], *unext = c->u_alt;
+    double *pcur = c->f[TL_F_P], *pnext = c->p_alt;  // p moves only in two-iteration launches
+    const bool pairs = p->cheby_pairs == TL_CHEBY_PAIRS_AUTO;
+    // u (and p) move between two buffers and either may end up as the
+    // field: the cells no kernel writes (the outer halo) start equal in both
+    HIPCHK(c, launch_copy_full(c, ucur, unext));
+    if (pairs) HIPCHK(c, launch_copy_full(c, pcur, pnext));
     // Cheby.init! and the first main step (Cheby.jl:40-45); iteration t = tt
     int tt = cgiters + 1;
     int chebyiters = 1;
     HIPCHK(c, launch_cheby_init(c, theta, ucur, unext));
     std::swap(ucur, unext);
-    HIPCHK(c, launch_cheby_iter(c, ca[2], cb[2], ucur, unext, 1, tt));
+    HIPCHK(c, launch_cheby_iter(c, ca[2], cb[2], ucur, unext, pcur, 1, tt));
     std::swap(ucur, unext);
     RETIF(read_ctrl(c, &k));
     const int estiter = tl_cheby_calciter(eigmin, eigmax, k.err, k.bb);
     st->est_iters = estiter;
     if (k.stop_iter == INT_MAX) {
-        // main loop (Cheby.jl:46-53); every launch swaps the u buffers
+        // main loop (Cheby.jl:46-53); every launch swaps the u buffers.  Two
+        // iterations per launch (TL_CHEBY_PAIRS_AUTO) unless the first needs
+        // the residual norm, so that a stop can only follow a launch's last
+        // iteration
         while (tt < maxiters) {
             int nb = 0;
             int norm = 0;
             while (tt < maxiters && nb < 64) {
-                ++tt;
-                ++chebyiters;
-                norm = (chebyiters >= estiter) && ((tt + 1) % 10 == 0);
-                HIPCHK(c, launch_cheby_iter(c, ca[chebyiters + 1], cb[chebyiters + 1], ucur, unext, norm, tt));
+                const bool norm1 = (chebyiters + 1 >= estiter) && ((tt + 2) % 10 == 0);
+                if (pairs && !norm1 && tt + 2 <= maxiters) {
+                    norm = (chebyiters + 2 >= estiter) && ((tt + 3) % 10 == 0);
+                    HIPCHK(c, launch_cheby_iter2(c, ca[chebyiters + 2], cb[chebyiters + 2], ca[chebyiters + 3],
+                                                 cb[chebyiters + 3], ucur, unext, pcur, pnext, norm, tt + 1));
+                    std::swap(pcur, pnext);
+                    tt += 2;
+                    chebyiters += 2;
+                    nb += 2;
+                } else {
+                    ++tt;
+                    ++chebyiters;
+                    norm = (chebyiters >= estiter) && ((tt + 1) % 10 == 0);
+                    HIPCHK(c, launch_cheby_iter(c, ca[chebyiters + 1], cb[chebyiters + 1], ucur, unext, pcur, norm,
+                                                tt));
+                    ++nb;
+                }
                 std::swap(ucur, unext);
-                ++nb;
                 if (norm) break;
             }
             if (norm) {
@@ -813,6 +834,11 @@ static int solve_cheby(tl_ctx *c, const tl_solve_params *p, double rx, double ry
             }
         }
         RETIF(read_ctrl(c, &k));
+        if (pairs && k.stop_iter != INT_MAX && k.stop_iter != tt) {
+            // stops follow norm iterations, which end a launch
+            c->err = "Chebyshev: stop inside a two-iteration launch";
+            return TL_E_STATE;
+        }
         if (k.stop_iter != INT_MAX && ((tt - k.stop_iter) & 1)) std::swap(ucur, unext);
         if (k.stop_iter != INT_MAX) {
             chebyiters -= tt - k.stop_iter;
@@ -820,6 +846,7 @@ static int solve_cheby(tl_ctx *c, const tl_solve_params *p, double rx, double ry
         }
     }
     if (ucur != c->f[TL_F_U]) std::swap(c->f[TL_F_U], c->u_alt);
+    if (pcur != c->f[TL_F_P]) std::swap(c->f[TL_F_P], c->p_alt);
     st->iterations = tt;
     st->cheby_iters = chebyiters;
     st->error = k.err;
@@ -1073,11 +1100,19 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
         HIPCHK(c, launch_set_mode(c, MODE_JACOBI, 1 << 30));
         if (kernel_id == TL_K_PPCG2_64 && !c->r_alt1) RETIF(alloc_d(c, &c->r_alt1, c->g.elems));
         double *a = c->f[TL_F_U], *b = c->u_alt, *s0 = c->f[TL_F_SD], *s1 = c->sd_alt;
-        double *r0 = c->f[TL_F_R], *r1 = c->r_alt1;
+        double *r0 = c->f[TL_F_R], *r1 = c->r_alt1, *q0 = c->f[TL_F_P], *q1 = c->p_alt;
         for (int i = 0; i < reps; ++i) {
             const int t = 1 + i;
             switch (kernel_id) {
-            case TL_K_CHEBY64: HIPCHK(c, launch_cheby_iter(c, 0.5, 1e-3, a, b, 0, t)); std::swap(a, b); break;
+            case TL_K_CHEBY64:
+                HIPCHK(c, launch_cheby_iter(c, 0.5, 1e-3, a, b, c->f[TL_F_P], 0, t));
+                std::swap(a, b);
+                break;
+            case TL_K_CHEBY2_64:
+                HIPCHK(c, launch_cheby_iter2(c, 0.5, 1e-3, 0.5, 1e-3, a, b, q0, q1, 0, t));
+                std::swap(a, b);
+                std::swap(q0, q1);
+                break;
             case TL_K_PPCG64: HIPCHK(c, launch_ppcg_inner(c, 0.5, 1e-3, s0, s1, 0, t)); std::swap(s0, s1); break;
             case TL_K_PPCG2_64:
                 HIPCHK(c, launch_ppcg_inner2(c, 0.5, 1e-3, 0.5, 1e-3, s0, s1, r0, r1, 0, t));
@@ -1154,6 +1189,10 @@ extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double r
     }
     if (p->ppcg_pairs != TL_PPCG_PAIRS_AUTO && p->ppcg_pairs != TL_PPCG_PAIRS_OFF) {
         c->err = "tl_solve: ppcg_pairs must be TL_PPCG_PAIRS_AUTO or TL_PPCG_PAIRS_OFF";
+        return TL_E_INVALID;
+    }
+    if (p->cheby_pairs != TL_CHEBY_PAIRS_AUTO && p->cheby_pairs != TL_CHEBY_PAIRS_OFF) {
+        c->err = "tl_solve: cheby_pairs must be TL_CHEBY_PAIRS_AUTO or TL_CHEBY_PAIRS_OFF";
         return TL_E_INVALID;
     }
     if (p->warm_start != TL_WARM_NONE && p->solver != TL_CG) {

@@ -12,7 +12,7 @@
 //   precision = fp64 | fp32_ir      stop_rule = reference | relative
 //   rel_tol = <float>               warm_start = none | learned
 //   ir_delta = <float>              fp32_kernels = auto | two | fused
-//   warm_policy = backoff | every    ppcg_pairs = auto | off
+//   warm_policy = backoff | every    ppcg_pairs = auto | off    cheby_pairs = auto | off
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -220,6 +220,14 @@ extern "C" int tl_deck_parse(const char *text, tl_deck *d)
                 d->params.warm_policy = TL_WARM_POLICY_BACKOFF;
             else if (v == "every")
                 d->params.warm_policy = TL_WARM_POLICY_EVERY;
+            else
+                ok = false;
+        } else if (key == "chebypairs") {
+            const std::string v = strip(val);
+            if (v == "auto")
+                d->params.cheby_pairs = TL_CHEBY_PAIRS_AUTO;
+            else if (v == "off")
+                d->params.cheby_pairs = TL_CHEBY_PAIRS_OFF;
             else
                 ok = false;
         } else if (key == "ppcgpairs") {

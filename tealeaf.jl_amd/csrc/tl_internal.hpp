@@ -341,7 +341,10 @@ hipError_t launch_field_summary(tl_ctx *c);
 hipError_t launch_sumsq_slot(tl_ctx *c, const double *v, int slot);
 hipError_t launch_sumsq_rr(tl_ctx *c, const double *v, int t);
 hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double *u_out);
-hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t);
+hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, double *p, int norm,
+                             int t);
+hipError_t launch_cheby_iter2(tl_ctx *c, double a1, double b1, double a2, double b2, const double *u_in,
+                              double *u_out, const double *p_in, double *p_out, int norm, int t);
 hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t);
 hipError_t launch_ppcg_inner(tl_ctx *c, double a, double b, const double *sd_in, double *sd_out, int last, int t);
 hipError_t launch_ppcg_inner2(tl_ctx *c, double a1, double b1, double a2, double b2, const double *sd_in,

@@ -2614,6 +2614,181 @@ __global__ __launch_bounds__(TBV) void k_ppcg2(Geo g, int ty, int t, const doubl
     if (sums) write_partials<1, TBV>(acc, part, g.nbmax);
 }
 
+// Two Chebyshev steps (Cheby.iterate!, Cheby.jl:93-107) in one sweep, as
+// k_ppcg2 does for PPCG: step 1 at row M = L-1, step 2 at row N = L-2,
+//   step 1:  r1 = u0 - A u,   p1 = a1 p0 + b1 r1,  u1 = u + p1
+//   step 2:  r2 = u0 - A u1,  p2 = a2 p1 + b2 r2,  u2 = u1 + p2
+// with the single step's operations and association (k_vs64, VS_CHEBY), so
+// the result is bit-identical to two launches of it; with norm, the partial
+// sums of r2^2 as the single step forms them for its residual norm.  u and p
+// are read at the column next to the wave by its edge lanes, so both move to
+// other buffers (u_in -> u_out, p_in -> p_out).  Reads u, u0, p, kx, ky once
+// and writes u2, p2 once: 56 B/cell for two steps instead of 112.
+__global__ __launch_bounds__(TBV) void k_cheby2(Geo g, int ty, int t, const double *__restrict__ uin,
+                                                double *__restrict__ uout, const double *__restrict__ u0,
+                                                const double *__restrict__ pin, double *__restrict__ pout,
+                                                const double *__restrict__ kx, const double *__restrict__ ky,
+                                                double a1, double b1, double a2, double b2, int norm,
+                                                const Ctrl *__restrict__ ctrl, double *__restrict__ part)
+{
+    constexpr int V = 2;
+    typedef double T;
+    if (t > ctrl->stop_iter) return;
+    const VCol<V> cl = vcolumn<V>(g, ty);
+    const int xl = cl.xl, lane = cl.lane;
+    double acc[1] = {0.0};
+    if (cl.y0 < g.ny) {
+        const bool eleft = lane == 0 && xl > 0;
+        const bool eright = lane == 63 && xl + V < g.nx;
+        const bool edge = eleft || eright;
+        const int hc = lane == 0 ? xl - 1 : xl + V;                       // hcol
+        const int hcc = edge ? hc : xl;                                     // a valid column to load
+        const int fcc = eleft ? xl - 2 : eright ? min(xl + V + 1, g.nx - 1) : xl;
+        // one row's loads: u at row L (own, hcol, fcol); u0, p0, kx at row
+        // L-1; ky at row L (the face below it)
+        struct Row {
+            T u[V], u0[V], p[V], kx[V], ky[V];
+            T eu, ef, eu0, ep, ekx0, ekx1, eky;
+        };
+        auto load_row = [&](Row &B, int L) {
+            const int ys = min(max(L, 0), g.ny - 1), yr = min(max(L - 1, 0), g.ny - 1);
+            const int yk = min(max(L, 0), g.ny);
+            vload<T, V>(B.u, uin + at(g, xl, ys));
+            vload<T, V>(B.u0, u0 + at(g, xl, yr));
+            vload<T, V>(B.p, pin + at(g, xl, yr));
+            vload<T, V>(B.kx, kx + at(g, xl, yr));
+            vload<T, V>(B.ky, ky + at(g, xl, yk));
+            B.eu = B.ef = B.eu0 = B.ep = B.ekx1 = B.eky = (T)0;
+            // lane 63's right face of its last column, as the single step loads it
+            B.ekx0 = lane == 63 ? kx[at(g, xl, yr) + V] : (T)0;
+            if (edge) {
+                B.eu = uin[at(g, hcc, ys)];
+                B.ef = uin[at(g, fcc, ys)];
+                B.eu0 = u0[at(g, hcc, yr)];
+                B.ep = pin[at(g, hcc, yr)];
+                B.eky = ky[at(g, hcc, yk)];
+                if (lane == 0) B.ekx0 = kx[at(g, xl - 1, yr)];               // left face of hcol
+                else B.ekx1 = kx[at(g, min(xl + V + 1, g.nx), yr)];          // right face of hcol
+            }
+        };
+        // the window at row L, before step 1 at row M: u at rows N, M (own,
+        // hcol) and fcol at M; u0, kx at N, ky at N and M; u1 at rows N-1, N
+        // (own; hcol at N) and p1 at N, from the two previous rows
+        T Um[V], Uc[V], UHm = 0, UHc = 0, EFc = 0;
+        T U0m[V], KXm[V], EKXm = 0, KYm[V], KYc[V], EKYc = 0;
+        T V1b[V], V1c[V], H1c = 0, P1c[V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) Um[j] = Uc[j] = U0m[j] = KXm[j] = KYm[j] = KYc[j] = V1b[j] = V1c[j] = P1c[j] = (T)0;
+        const int y0 = cl.y0, y1 = cl.y1;
+        auto step = [&](const Row &B, int L) {
+            const int M = L - 1, N = L - 2;
+            // step 1 at row M (own columns, and hcol on the edge lanes)
+            T v1[V], p1[V], h1 = 0;
+            {
+                const bool ybot = M <= 0, ytop = M + 1 >= g.ny;
+                T lv = lane_up1<false>(Uc[V - 1]), rv = lane_down1<false>(Uc[0]);
+                if (lane == 0) lv = xl > 0 ? UHc : Uc[0];
+                if (lane == 63) rv = xl + V < g.nx ? UHc : Uc[V - 1];
+                if (xl == 0) lv = Uc[0];
+                T kxn = lane_down1<false>(B.kx[0]);
+                if (lane == 63) kxn = B.ekx0;
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const T kxl = B.kx[j], kxr = j < V - 1 ? B.kx[j + 1] : kxn;
+                    const T dn = ybot ? Uc[j] : Um[j], up = ytop ? Uc[j] : B.u[j];
+                    const T w = vsmvp<T, V>(g, xl, j, Uc, lv, rv, dn, up, kxl, kxr, KYc[j], B.ky[j]);
+                    const T rr = B.u0[j] - w;
+                    p1[j] = a1 * B.p[j] + b1 * rr;
+                    v1[j] = Uc[j] + p1[j];
+                }
+                if (edge) {
+                    const T c = UHc;
+                    T l, r, kxl, kxr;
+                    if (lane == 0) {
+                        l = hc == 0 ? c : EFc;
+                        r = Uc[0];
+                        kxl = B.ekx0;
+                        kxr = B.kx[0];
+                    } else {
+                        l = Uc[V - 1];
+                        r = hc + 1 >= g.nx ? c : EFc;
+                        kxl = B.ekx0;
+                        kxr = B.ekx1;
+                    }
+                    const T dn = ybot ? c : UHm, up = ytop ? c : B.eu;
+                    const T w = smvp5(c, l, r, dn, up, kxl, kxr, EKYc, B.eky);
+                    const T rr = B.eu0 - w;
+                    const T ph = a1 * B.ep + b1 * rr;
+                    h1 = c + ph;
+                }
+            }
+            // step 2 at row N (own columns), from u1 at rows N-1, N, M
+            if (L >= y0 + 2 && N < y1) {
+                const bool ybot = N <= 0, ytop = N + 1 >= g.ny;
+                T lv = lane_up1<false>(V1c[V - 1]), rv = lane_down1<false>(V1c[0]);
+                if (lane == 0) lv = xl > 0 ? H1c : V1c[0];
+                if (lane == 63) rv = xl + V < g.nx ? H1c : V1c[V - 1];
+                if (xl == 0) lv = V1c[0];
+                T kxn = lane_down1<false>(KXm[0]);
+                if (lane == 63) kxn = EKXm;
+                T o1[V], o2[V], o3[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const T kxl = KXm[j], kxr = j < V - 1 ? KXm[j + 1] : kxn;
+                    const T dn = ybot ? V1c[j] : V1b[j], up = ytop ? V1c[j] : v1[j];
+                    const T w = vsmvp<T, V>(g, xl, j, V1c, lv, rv, dn, up, kxl, kxr, KYm[j], KYc[j]);
+                    const T rr = U0m[j] - w;                // u0 - A u1
+                    const T pv = a2 * P1c[j] + b2 * rr;
+                    o1[j] = V1c[j] + pv;                    // u2
+                    o2[j] = pv;                             // p2
+                    o3[j] = rr;
+                }
+                if (cl.active) {
+                    const long ic = at(g, xl, N);
+                    const int nv = g.nx - cl.x0;
+                    vstore_n<T, V>(uout + ic, o1, nv);
+                    vstore_n<T, V>(pout + ic, o2, nv);
+#pragma unroll
+                    for (int j = 0; j < V; ++j)
+                        if (j < nv) acc[0] += o3[j] * o3[j];
+                }
+            }
+            // shift the window by one row
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                Um[j] = Uc[j];
+                Uc[j] = B.u[j];
+                U0m[j] = B.u0[j];
+                KXm[j] = B.kx[j];
+                KYm[j] = KYc[j];
+                KYc[j] = B.ky[j];
+                V1b[j] = V1c[j];
+                V1c[j] = v1[j];
+                P1c[j] = p1[j];
+            }
+            UHm = UHc;
+            UHc = B.eu;
+            EFc = B.ef;
+            EKXm = B.ekx0;
+            EKYc = B.eky;
+            H1c = h1;
+        };
+        // rows y0-2 .. y1+1, the next row's loads issued before the current
+        // row is computed
+        Row B0, B1;
+        load_row(B0, y0 - 2);
+        for (int L = y0 - 2; L <= y1 + 1; L += 2) {
+            const bool more = L + 1 <= y1 + 1;
+            if (more) load_row(B1, L + 1);
+            step(B0, L);
+            if (!more) break;
+            if (L + 2 <= y1 + 1) load_row(B0, L + 2);
+            step(B1, L + 1);
+        }
+    }
+    if (norm) write_partials<1, TBV>(acc, part, g.nbmax);
+}
+
 // --------------------------------------------------------------- Jacobi
 
 // Jacobi.iterate!, Jacobi.jl:59-80 (u' in a separate buffer, the reference's
@@ -3076,15 +3251,32 @@ hipError_t launch_cheby_init(tl_ctx *c, double theta, const double *u_in, double
     return e == hipSuccess ? fin(c, part_m(c), nbh(c), 1, 0, FK_BB, 0) : e;
 }
 
-hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, int norm, int t)
+hipError_t launch_cheby_iter(tl_ctx *c, double a, double b, const double *u_in, double *u_out, double *p, int norm,
+                             int t)
 {
     const bool prof = prof_start(c, TL_K_CHEBY64);
     k_vs64<VS_CHEBY, TL_RV_VS64, TL_PIPE_VS64 != 0><<<nb2(c), TBV, 0, c->stream>>>(
-        c->g, c->g.ty2, t, u_in, u_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_U0], c->f[TL_F_P], a, b, norm,
-        c->ctrl, part_m(c));
+        c->g, c->g.ty2, t, u_in, u_out, c->f[TL_F_KX], c->f[TL_F_KY], c->f[TL_F_U0], p, a, b, norm, c->ctrl,
+        part_m(c));
     if (prof) prof_stop(c);
     hipError_t e = hipGetLastError();
     return (e == hipSuccess && norm) ? fin(c, part_m(c), nb2(c), 1, 0, FK_CHEBY, t) : e;
+}
+
+// two Chebyshev steps, iterations t and t+1 (coefficients a1, b1 then a2,
+// b2): u_in -> u_out, p_in -> p_out; with norm, the residual norm of t+1
+hipError_t launch_cheby_iter2(tl_ctx *c, double a1, double b1, double a2, double b2, const double *u_in,
+                              double *u_out, const double *p_in, double *p_out, int norm, int t)
+{
+    const bool prof = prof_start(c, TL_K_CHEBY2_64);
+    // the launch that forms the residual norm uses the single step's bands
+    const unsigned nb = norm ? nb2(c) : (unsigned)(c->g.gx2 * c->g.gyq);
+    k_cheby2<<<nb, TBV, 0, c->stream>>>(c->g, norm ? c->g.ty2 : c->g.tyq, t, u_in, u_out, c->f[TL_F_U0], p_in,
+                                        p_out, c->f[TL_F_KX], c->f[TL_F_KY], a1, b1, a2, b2, norm, c->ctrl,
+                                        part_m(c));
+    if (prof) prof_stop(c);
+    hipError_t e = hipGetLastError();
+    return (e == hipSuccess && norm) ? fin(c, part_m(c), nb2(c), 1, 0, FK_CHEBY, t + 1) : e;
 }
 
 hipError_t launch_ppcg_ur_sd(tl_ctx *c, double theta, int t)

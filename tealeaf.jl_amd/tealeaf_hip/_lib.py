@@ -39,16 +39,17 @@ EXPORTS = [
 ]
 TL_START_PREVIOUS, TL_START_WARM = 0, 1
 (TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32, TL_K_CHEBY64, TL_K_PPCG64,
- TL_K_JACOBI64, TL_K_RESID64, TL_K_CGF32, TL_K_PPCG2_64) = range(12)
-TL_K_COUNT = 12
+ TL_K_JACOBI64, TL_K_RESID64, TL_K_CGF32, TL_K_PPCG2_64, TL_K_CHEBY2_64) = range(13)
+TL_K_COUNT = 13
 TL_FP32_KERNELS_AUTO, TL_FP32_KERNELS_TWO, TL_FP32_KERNELS_FUSED = 0, 1, 2
 TL_WARM_POLICY_BACKOFF, TL_WARM_POLICY_EVERY = 0, 1
 TL_PPCG_PAIRS_AUTO, TL_PPCG_PAIRS_OFF = 0, 1
+TL_CHEBY_PAIRS_AUTO, TL_CHEBY_PAIRS_OFF = 0, 1
 KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, false>", 2: "k_cg_p<double>",
                 3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
                 5: "(fused into k_cgg<4, false, 2, true, 1>)",
                 6: "k_vs64<0, 4, true>", 7: "k_vs64<1, 4, true>", 8: "k_vs64<2, 4, true>", 9: "k_residual",
-                10: "k_cgf<1, 1, 0>", 11: "k_ppcg2",
+                10: "k_cgf<1, 1, 0>", 11: "k_ppcg2", 12: "k_cheby2",
                 -1: "k_copy_full", -2: "k_copy_vec"}
 
 
@@ -73,7 +74,7 @@ class SolveParams(C.Structure):
         ("ppcg_inner_steps", C.c_int), ("error_switch", C.c_int), ("eps", C.c_double), ("epslim", C.c_double),
         ("precision", C.c_int), ("stop_rule", C.c_int), ("rel_tol", C.c_double), ("warm_start", C.c_int),
         ("check_result", C.c_int), ("ir_delta", C.c_double), ("launch", C.c_int), ("fp32_kernels", C.c_int),
-        ("warm_policy", C.c_int), ("ppcg_pairs", C.c_int), ("reserved", C.c_int * 4),
+        ("warm_policy", C.c_int), ("ppcg_pairs", C.c_int), ("cheby_pairs", C.c_int), ("reserved", C.c_int * 3),
     ]
 
 

@@ -110,9 +110,11 @@ def test_parser_mixed_precision_keys():
     assert T.Settings.from_text(base + "warm_policy=every\n")._deck.params.warm_policy == _lib.TL_WARM_POLICY_EVERY
     assert s._deck.params.ppcg_pairs == _lib.TL_PPCG_PAIRS_AUTO
     assert T.Settings.from_text(base + "ppcg_pairs=off\n")._deck.params.ppcg_pairs == _lib.TL_PPCG_PAIRS_OFF
+    assert s._deck.params.cheby_pairs == _lib.TL_CHEBY_PAIRS_AUTO
+    assert T.Settings.from_text(base + "cheby_pairs=off\n")._deck.params.cheby_pairs == _lib.TL_CHEBY_PAIRS_OFF
     n0 = T.Settings.from_text(base).n_warnings
     for bad in ("precision=fp16\n", "ir_delta=abc\n", "stop_rule=sometimes\n", "warm_start=maybe\n",
-                "fp32_kernels=three\n", "warm_policy=sometimes\n", "ppcg_pairs=3\n"):
+                "fp32_kernels=three\n", "warm_policy=sometimes\n", "ppcg_pairs=3\n", "cheby_pairs=2\n"):
         assert T.Settings.from_text(base + bad).n_warnings == n0 + 1, bad  # warned and ignored
 
 

@@ -165,12 +165,22 @@ typedef float tl_gc_t;
 enum GcFmt : int { GC_BF16 = 0, GC_F16 = 1 };
 // blocks of the coefficient range reduction (k_gc_range)
 #define TL_RANGE_BLOCKS 256
-// non-temporal (streaming) vector loads / stores in those kernels
+// non-temporal (streaming) vector loads / stores in the vectorised stencil
+// kernels: each sweep reads and writes fields far larger than the caches.
+// Measured at 8192^2 (A/B in one box, loads and stores together against
+// neither): fused FP32 CG 4.68 / 4.78 -> 5.14 / 5.04 TB/s (odd / even t), FP32
+// PW 4.65 -> 5.37, FP32 UR 5.03 / 5.09 -> 6.08 / 5.98, FP64 PW 5.23 -> 5.33,
+// FP64 UR 5.17 -> 5.56, Chebyshev / PPCG / Jacobi steps +6-7%; loads alone
+// made the fused kernel 4% slower.  The two-step solver kernels ran 2-4%
+// slower with them (TL_NT_2STEP).
 #ifndef TL_NT_LOAD
-#define TL_NT_LOAD 0
+#define TL_NT_LOAD 1
 #endif
 #ifndef TL_NT_STORE
-#define TL_NT_STORE 0
+#define TL_NT_STORE 1
+#endif
+#ifndef TL_NT_2STEP
+#define TL_NT_2STEP 0
 #endif
 
 struct Geo {

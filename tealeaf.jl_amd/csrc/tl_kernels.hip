@@ -799,39 +799,40 @@ __global__ __launch_bounds__(TB) void k_cg_p(Geo g, int t, T *__restrict__ p, co
 // last vector of a row reads into the right halo/padding (inside the
 // allocation) and stores only its interior columns.
 
-template <typename T, int V>
+// NT: non-temporal (streaming) access, the default for the stencil kernels'
+// field sweeps (TL_NT_LOAD / TL_NT_STORE)
+template <typename T, int V, bool NT = (TL_NT_LOAD != 0)>
 __device__ __forceinline__ void vload(T (&d)[V], const T *__restrict__ p)
 {
     typedef T vec __attribute__((ext_vector_type(V)));
-#if TL_NT_LOAD
-    const vec v = __builtin_nontemporal_load(reinterpret_cast<const vec *>(p));
-#else
-    const vec v = *reinterpret_cast<const vec *>(p);
-#endif
+    vec v;
+    if constexpr (NT)
+        v = __builtin_nontemporal_load(reinterpret_cast<const vec *>(p));
+    else
+        v = *reinterpret_cast<const vec *>(p);
 #pragma unroll
     for (int j = 0; j < V; ++j) d[j] = v[j];
 }
 
-template <typename T, int V>
+template <typename T, int V, bool NT = (TL_NT_STORE != 0)>
 __device__ __forceinline__ void vstore(T *__restrict__ p, const T (&s)[V])
 {
     typedef T vec __attribute__((ext_vector_type(V)));
     vec v;
 #pragma unroll
     for (int j = 0; j < V; ++j) v[j] = s[j];
-#if TL_NT_STORE
-    __builtin_nontemporal_store(v, reinterpret_cast<vec *>(p));
-#else
-    *reinterpret_cast<vec *>(p) = v;
-#endif
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, reinterpret_cast<vec *>(p));
+    else
+        *reinterpret_cast<vec *>(p) = v;
 }
 
 // store of the first n (< V: the grid's last, partial vector) or all V elements
-template <typename T, int V>
+template <typename T, int V, bool NT = (TL_NT_STORE != 0)>
 __device__ __forceinline__ void vstore_n(T *__restrict__ p, const T (&s)[V], int n)
 {
     if (n >= V) {
-        vstore<T, V>(p, s);
+        vstore<T, V, NT>(p, s);
     } else {
 #pragma unroll
         for (int j = 0; j < V; ++j)
@@ -1681,6 +1682,7 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, c
             vload<float, 4>(B.r[q], rin + i);
             vload<float, 4>(B.p[q], pin + i);
             {
+                // a cached load: the non-temporal form made the kernel 6-7% slower
                 typedef unsigned uv2 __attribute__((ext_vector_type(2)));
                 const uv2 w = *reinterpret_cast<const uv2 *>(gc + i);
                 B.g[q][0] = w[0];
@@ -2482,11 +2484,11 @@ __global__ __launch_bounds__(TBV) void k_ppcg2(Geo g, int ty, int t, const doubl
         auto load_row = [&](Row &B, int L) {
             const int ys = min(max(L, 0), g.ny - 1), yr = min(max(L - 1, 0), g.ny - 1);
             const int yk = min(max(L, 0), g.ny), yu = min(max(L - 2, 0), g.ny - 1);
-            vload<T, V>(B.s, sdin + at(g, xl, ys));
-            vload<T, V>(B.r, rin + at(g, xl, yr));
-            vload<T, V>(B.kx, kx + at(g, xl, yr));
-            vload<T, V>(B.ky, ky + at(g, xl, yk));
-            vload<T, V>(B.u, u + at(g, xl, yu));
+            vload<T, V, TL_NT_2STEP != 0>(B.s, sdin + at(g, xl, ys));
+            vload<T, V, TL_NT_2STEP != 0>(B.r, rin + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP != 0>(B.kx, kx + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP != 0>(B.ky, ky + at(g, xl, yk));
+            vload<T, V, TL_NT_2STEP != 0>(B.u, u + at(g, xl, yu));
             B.es = B.ef = B.er = B.ekx1 = B.eky = (T)0;
             // lane 63's right face of its last column, as the single step loads it
             B.ekx0 = lane == 63 ? kx[at(g, xl, yr) + V] : (T)0;
@@ -2571,9 +2573,9 @@ __global__ __launch_bounds__(TBV) void k_ppcg2(Geo g, int ty, int t, const doubl
                 if (cl.active) {
                     const long ic = at(g, xl, N);
                     const int nv = g.nx - cl.x0;
-                    vstore_n<T, V>(sdout + ic, o1, nv);
-                    vstore_n<T, V>(rout + ic, o2, nv);
-                    vstore_n<T, V>(u + ic, o3, nv);
+                    vstore_n<T, V, TL_NT_2STEP != 0>(sdout + ic, o1, nv);
+                    vstore_n<T, V, TL_NT_2STEP != 0>(rout + ic, o2, nv);
+                    vstore_n<T, V, TL_NT_2STEP != 0>(u + ic, o3, nv);
 #pragma unroll
                     for (int j = 0; j < V; ++j)
                         if (j < nv) acc[0] += o2[j] * o2[j];
@@ -2653,11 +2655,11 @@ __global__ __launch_bounds__(TBV) void k_cheby2(Geo g, int ty, int t, const doub
         auto load_row = [&](Row &B, int L) {
             const int ys = min(max(L, 0), g.ny - 1), yr = min(max(L - 1, 0), g.ny - 1);
             const int yk = min(max(L, 0), g.ny);
-            vload<T, V>(B.u, uin + at(g, xl, ys));
-            vload<T, V>(B.u0, u0 + at(g, xl, yr));
-            vload<T, V>(B.p, pin + at(g, xl, yr));
-            vload<T, V>(B.kx, kx + at(g, xl, yr));
-            vload<T, V>(B.ky, ky + at(g, xl, yk));
+            vload<T, V, TL_NT_2STEP != 0>(B.u, uin + at(g, xl, ys));
+            vload<T, V, TL_NT_2STEP != 0>(B.u0, u0 + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP != 0>(B.p, pin + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP != 0>(B.kx, kx + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP != 0>(B.ky, ky + at(g, xl, yk));
             B.eu = B.ef = B.eu0 = B.ep = B.ekx1 = B.eky = (T)0;
             // lane 63's right face of its last column, as the single step loads it
             B.ekx0 = lane == 63 ? kx[at(g, xl, yr) + V] : (T)0;
@@ -2746,8 +2748,8 @@ __global__ __launch_bounds__(TBV) void k_cheby2(Geo g, int ty, int t, const doub
                 if (cl.active) {
                     const long ic = at(g, xl, N);
                     const int nv = g.nx - cl.x0;
-                    vstore_n<T, V>(uout + ic, o1, nv);
-                    vstore_n<T, V>(pout + ic, o2, nv);
+                    vstore_n<T, V, TL_NT_2STEP != 0>(uout + ic, o1, nv);
+                    vstore_n<T, V, TL_NT_2STEP != 0>(pout + ic, o2, nv);
 #pragma unroll
                     for (int j = 0; j < V; ++j)
                         if (j < nv) acc[0] += o3[j] * o3[j];

@@ -171,16 +171,20 @@ enum GcFmt : int { GC_BF16 = 0, GC_F16 = 1 };
 // neither): fused FP32 CG 4.68 / 4.78 -> 5.14 / 5.04 TB/s (odd / even t), FP32
 // PW 4.65 -> 5.37, FP32 UR 5.03 / 5.09 -> 6.08 / 5.98, FP64 PW 5.23 -> 5.33,
 // FP64 UR 5.17 -> 5.56, Chebyshev / PPCG / Jacobi steps +6-7%; loads alone
-// made the fused kernel 4% slower.  The two-step solver kernels ran 2-4%
-// slower with them (TL_NT_2STEP).
+// made the fused kernel 4% slower.  The two-step solver kernels have their
+// own switches: with both they ran 2-4% slower, with loads alone 6-8% slower,
+// with stores alone within +-2%.
 #ifndef TL_NT_LOAD
 #define TL_NT_LOAD 1
 #endif
 #ifndef TL_NT_STORE
 #define TL_NT_STORE 1
 #endif
-#ifndef TL_NT_2STEP
-#define TL_NT_2STEP 0
+#ifndef TL_NT_2STEP_LD
+#define TL_NT_2STEP_LD 0
+#endif
+#ifndef TL_NT_2STEP_ST
+#define TL_NT_2STEP_ST 0
 #endif
 
 struct Geo {

@@ -2484,11 +2484,11 @@ __global__ __launch_bounds__(TBV) void k_ppcg2(Geo g, int ty, int t, const doubl
         auto load_row = [&](Row &B, int L) {
             const int ys = min(max(L, 0), g.ny - 1), yr = min(max(L - 1, 0), g.ny - 1);
             const int yk = min(max(L, 0), g.ny), yu = min(max(L - 2, 0), g.ny - 1);
-            vload<T, V, TL_NT_2STEP != 0>(B.s, sdin + at(g, xl, ys));
-            vload<T, V, TL_NT_2STEP != 0>(B.r, rin + at(g, xl, yr));
-            vload<T, V, TL_NT_2STEP != 0>(B.kx, kx + at(g, xl, yr));
-            vload<T, V, TL_NT_2STEP != 0>(B.ky, ky + at(g, xl, yk));
-            vload<T, V, TL_NT_2STEP != 0>(B.u, u + at(g, xl, yu));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.s, sdin + at(g, xl, ys));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.r, rin + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.kx, kx + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.ky, ky + at(g, xl, yk));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.u, u + at(g, xl, yu));
             B.es = B.ef = B.er = B.ekx1 = B.eky = (T)0;
             // lane 63's right face of its last column, as the single step loads it
             B.ekx0 = lane == 63 ? kx[at(g, xl, yr) + V] : (T)0;
@@ -2573,9 +2573,9 @@ __global__ __launch_bounds__(TBV) void k_ppcg2(Geo g, int ty, int t, const doubl
                 if (cl.active) {
                     const long ic = at(g, xl, N);
                     const int nv = g.nx - cl.x0;
-                    vstore_n<T, V, TL_NT_2STEP != 0>(sdout + ic, o1, nv);
-                    vstore_n<T, V, TL_NT_2STEP != 0>(rout + ic, o2, nv);
-                    vstore_n<T, V, TL_NT_2STEP != 0>(u + ic, o3, nv);
+                    vstore_n<T, V, TL_NT_2STEP_ST != 0>(sdout + ic, o1, nv);
+                    vstore_n<T, V, TL_NT_2STEP_ST != 0>(rout + ic, o2, nv);
+                    vstore_n<T, V, TL_NT_2STEP_ST != 0>(u + ic, o3, nv);
 #pragma unroll
                     for (int j = 0; j < V; ++j)
                         if (j < nv) acc[0] += o2[j] * o2[j];
@@ -2655,11 +2655,11 @@ __global__ __launch_bounds__(TBV) void k_cheby2(Geo g, int ty, int t, const doub
         auto load_row = [&](Row &B, int L) {
             const int ys = min(max(L, 0), g.ny - 1), yr = min(max(L - 1, 0), g.ny - 1);
             const int yk = min(max(L, 0), g.ny);
-            vload<T, V, TL_NT_2STEP != 0>(B.u, uin + at(g, xl, ys));
-            vload<T, V, TL_NT_2STEP != 0>(B.u0, u0 + at(g, xl, yr));
-            vload<T, V, TL_NT_2STEP != 0>(B.p, pin + at(g, xl, yr));
-            vload<T, V, TL_NT_2STEP != 0>(B.kx, kx + at(g, xl, yr));
-            vload<T, V, TL_NT_2STEP != 0>(B.ky, ky + at(g, xl, yk));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.u, uin + at(g, xl, ys));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.u0, u0 + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.p, pin + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.kx, kx + at(g, xl, yr));
+            vload<T, V, TL_NT_2STEP_LD != 0>(B.ky, ky + at(g, xl, yk));
             B.eu = B.ef = B.eu0 = B.ep = B.ekx1 = B.eky = (T)0;
             // lane 63's right face of its last column, as the single step loads it
             B.ekx0 = lane == 63 ? kx[at(g, xl, yr) + V] : (T)0;
@@ -2748,8 +2748,8 @@ __global__ __launch_bounds__(TBV) void k_cheby2(Geo g, int ty, int t, const doub
                 if (cl.active) {
                     const long ic = at(g, xl, N);
                     const int nv = g.nx - cl.x0;
-                    vstore_n<T, V, TL_NT_2STEP != 0>(uout + ic, o1, nv);
-                    vstore_n<T, V, TL_NT_2STEP != 0>(pout + ic, o2, nv);
+                    vstore_n<T, V, TL_NT_2STEP_ST != 0>(uout + ic, o1, nv);
+                    vstore_n<T, V, TL_NT_2STEP_ST != 0>(pout + ic, o2, nv);
 #pragma unroll
                     for (int j = 0; j < V; ++j)
                         if (j < nv) acc[0] += o3[j] * o3[j];

@@ -91,7 +91,9 @@ constexpr int TBV = TL_TBV;
 // on one box): 1-row batches pipelined (143 VGPRs, 3 waves per SIMD) in 768
 // blocks -- one block per 3 waves of every SIMD, a single round -- take
 // 0.314 ms per iteration; 2-row batches (188 VGPRs) 0.317 ms with 1024
-// blocks, 0.352 ms with 768; 4-row batches 0.36 ms
+// blocks, 0.352 ms with 768; 4-row batches 0.36 ms.  With non-temporal
+// sweeps: 768 blocks 0.287 ms, 1024 0.35, 1152 0.36-0.38, 1536 0.295;
+// 4 waves per SIMD forced (TL_WPE_F32=4: 128 VGPRs, 36 spilled) 0.37-0.38
 #ifndef TL_RV_F32
 #define TL_RV_F32 1
 #endif

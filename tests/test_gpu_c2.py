@@ -3,7 +3,17 @@ with 10 inner steps; dt 0.004, eps 1e-15, each solver's own stop rule)
 against the oracle's per-step answers committed in tests/golden/c2_4096.json:
 all four field-summary quantities within 1e-10 at every step the fixture
 holds, the CG pre-step count of Chebyshev and PPCG exactly, and the
-iteration counts as in the other parity tests."""
+iteration counts as in the other parity tests.
+
+The oracle's PPCG takes about half an hour per 4096^2 step, so the fixture
+holds its first two steps only.  PPCG runs the deck's 20 steps here all the
+same, and from step 3 on its summaries are checked against the oracle's CG
+answer for the same step.  PPCG stops on its own error measure, which leaves
+a larger true residual than CG's rule, and each step starts from the last
+one's answer, so the two drift apart steadily: the oracle's PPCG temperature
+lies 1.12e-11 below its CG's after step 1 and 2.09e-11 after step 2.  The
+bound at step t is 1.5 t times that drift per step (the fixture's step-2
+difference over 2), at least RTOL."""
 import json
 import os
 
@@ -26,14 +36,30 @@ def _fixture():
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("solver", ["cg", "cheby", "ppcg"])
 def test_reference_deck_4096_every_step_matches_oracle(solver):
-    g = _fixture()[solver]
+    fx = _fixture()
+    g = fx[solver]
     n = len(g["summary"])
-    print(f"case: {solver} 4096^2 reference deck, {n} steps vs the oracle fixture", flush=True)
+    steps = len(fx["cg"]["summary"]) if solver == "ppcg" else n
+    print(f"case: {solver} 4096^2 reference deck, {n} steps vs the oracle fixture"
+          + (f", steps {n + 1}-{steps} vs the oracle's CG" if steps > n else ""), flush=True)
+    if steps > n:
+        # per-step drift of the oracle's own PPCG from its CG over the steps it holds
+        a, b = np.array(g["summary"][n - 1]), np.array(fx["cg"]["summary"][n - 1])
+        drift = np.max(np.abs(a - b) / np.abs(b)) / n
+        assert 0 < drift < 1e-10, drift
     s = T.Settings.from_text(g["deck"])
     c = T.initialiseapp(s)
-    for t in range(n):
+    for t in range(steps):
         st = T.step(c, s)
         got = T.fieldsummary(c)
+        if t >= n:
+            want = np.array(fx["cg"]["summary"][t])
+            rel = np.max(np.abs(got - want) / np.abs(want))
+            bound = max(RTOL, 1.5 * drift * (t + 1))
+            print(f"  step {t + 1}: {st['iterations']} iterations, cg {st['cg_iters']}, "
+                  f"max rel diff vs the oracle's CG {rel:.2e} (bound {bound:.2e})", flush=True)
+            assert rel < bound, (t, got, want)
+            continue
         want = np.array(g["summary"][t])
         ost = g["stats"][t]
         rel = np.max(np.abs(got - want) / np.abs(want))

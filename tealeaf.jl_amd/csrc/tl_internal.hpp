@@ -94,6 +94,9 @@ constexpr int TBV = TL_TBV;
 // blocks, 0.352 ms with 768; 4-row batches 0.36 ms.  With non-temporal
 // sweeps: 768 blocks 0.287 ms, 1024 0.35, 1152 0.36-0.38, 1536 0.295;
 // 4 waves per SIMD forced (TL_WPE_F32=4: 128 VGPRs, 36 spilled) 0.37-0.38
+// The wave-edge loads (lanes 0 / 63: r, p and the coefficients of the two
+// columns outside the wave) cost about 4% of the iteration; one 8-byte load
+// by lanes 0-5 passed on by readlane instead of three ran no faster.
 #ifndef TL_RV_F32
 #define TL_RV_F32 1
 #endif

@@ -53,10 +53,15 @@ PROFILE_PERIOD = 15  # odd: both parities of the alternating FP32 UR kernel are 
 #          reads r(t-1), p(t-1) (4 B each), gc (2 B), writes r(t), p(t)       (18 B)
 #          and at even iterations reads and writes x                      (+8 B)
 #          i.e. 22 B per launch (= per iteration) on average
+#   FP32 inner CG as pairs of iterations (two sweeps per two iterations):
+#          first sweep reads r(t-1), p(t-1), gc                           (10 B)
+#          second sweep reads r(t-1), p(t-1), gc, x, writes r(t+1), p(t+1), x (26 B)
+#          i.e. 18 B per iteration
 # P64 is only the final p update of an FP64 solve.
 KERNEL_BYTES = {
     _lib.TL_K_CG_W64: 48, _lib.TL_K_CG_UR64: 48, _lib.TL_K_CG_P64: 24,
     _lib.TL_K_CG_W32: 14, _lib.TL_K_CG_UR32: 20, _lib.TL_K_CG_P32: 0, _lib.TL_K_CGF32: 22,
+    _lib.TL_K_CGP_G32: 10, _lib.TL_K_CGP_U32: 26,
 }
 # kernels whose odd and even iterations move different bytes: (odd, even) B/cell
 ALTERNATING = {_lib.TL_K_CG_UR32: (14, 26), _lib.TL_K_CGF32: (18, 26)}
@@ -84,7 +89,7 @@ def make_settings(n, seed, precision, warm):
 
 
 FP32_KERNELS = {"auto": _lib.TL_FP32_KERNELS_AUTO, "two": _lib.TL_FP32_KERNELS_TWO,
-                "fused": _lib.TL_FP32_KERNELS_FUSED}
+                "fused": _lib.TL_FP32_KERNELS_FUSED, "pairs": _lib.TL_FP32_KERNELS_PAIRS}
 
 
 def run_method(name, n, seed, precision, warm, steps, warmup, profile_mask=0, launch="stream", ir_delta=None,
@@ -357,6 +362,7 @@ def kernel_table(n, seed, reps):
     groups = [(_lib.TL_K_CG_W64, (_lib.TL_K_CG_W64, _lib.TL_K_CG_UR64)),
               (_lib.TL_K_CG_W32, (_lib.TL_K_CG_W32, _lib.TL_K_CG_UR32)),
               (_lib.TL_K_CGF32, (_lib.TL_K_CGF32,)),
+              (_lib.TL_K_CGP_G32, (_lib.TL_K_CGP_G32, _lib.TL_K_CGP_U32)),
               (_lib.TL_K_CHEBY64, (_lib.TL_K_CHEBY64,)), (_lib.TL_K_PPCG64, (_lib.TL_K_PPCG64,)),
               (_lib.TL_K_PPCG2_64, (_lib.TL_K_PPCG2_64,)), (_lib.TL_K_CHEBY2_64, (_lib.TL_K_CHEBY2_64,)),
               (_lib.TL_K_JACOBI64, (_lib.TL_K_JACOBI64,)), (_lib.TL_K_RESID64, (_lib.TL_K_RESID64,))]
@@ -460,7 +466,8 @@ def main():
                           "rel_resid": [st["rel_resid"] for st in base["stats"]],
                           "summaries": [list(map(float, x)) for x in base["summaries"]]}))
         return
-    mask = (1 << _lib.TL_K_CG_W32) | (1 << _lib.TL_K_CG_UR32) | (1 << _lib.TL_K_CGF32)
+    mask = ((1 << _lib.TL_K_CG_W32) | (1 << _lib.TL_K_CG_UR32) | (1 << _lib.TL_K_CGF32) | (1 << _lib.TL_K_CGP_G32)
+            | (1 << _lib.TL_K_CGP_U32))
     flag = run_method("fp32-ir" + ("+learned" if learned else ""), a.n, a.seed, "fp32_ir", warm, a.steps,
                       a.warmup, mask if a.launch == "stream" else 0, a.launch, a.ir_delta, a.fp32_kernels)
     base = None

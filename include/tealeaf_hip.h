@@ -137,6 +137,8 @@ typedef struct {
 #define TL_FP32_KERNELS_AUTO 0   /* fused when nx is a multiple of 4, else two kernels */
 #define TL_FP32_KERNELS_TWO 1    /* two kernels and two reductions per iteration (34 B/cell) */
 #define TL_FP32_KERNELS_FUSED 2  /* one kernel and one reduction per iteration (22 B/cell); needs nx % 4 == 0 */
+#define TL_FP32_KERNELS_PAIRS 3  /* two iterations per two kernels and two reductions (s-step CG, s = 2;
+                                   18 B/cell per iteration); needs nx % 4 == 0; stream launches */
 
 /* replaces the reference's @info lines (CG.jl:27, Cheby.jl:60, PPCG.jl:54, Jacobi.jl:29) */
 typedef struct {
@@ -310,7 +312,9 @@ int tl_get_cg_history(tl_ctx *ctx, double *alpha, double *beta, int n);
 #define TL_K_CGF32 10  /* FP32 inner CG iteration as one kernel (TL_FP32_KERNELS_FUSED) */
 #define TL_K_PPCG2_64 11 /* two PPCG inner steps in one sweep (TL_PPCG_PAIRS_AUTO) */
 #define TL_K_CHEBY2_64 12 /* two Chebyshev iterations in one sweep (TL_CHEBY_PAIRS_AUTO) */
-#define TL_K_COUNT 13
+#define TL_K_CGP_G32 13 /* TL_FP32_KERNELS_PAIRS: a pair's first sweep (the six sums) */
+#define TL_K_CGP_U32 14 /* TL_FP32_KERNELS_PAIRS: a pair's second sweep (r, p, x updates) */
+#define TL_K_COUNT 15
 /* Bracket every period-th launch of each kernel in kernel_mask (bit =
  * 1 << TL_K_*; period >= 1) with HIP events on the context's stream (an event
  * pair costs a few microseconds of GPU time, so a period of about 16 keeps the
@@ -329,7 +333,9 @@ int tl_profile_end(tl_ctx *ctx);
  * one cell per lane), -2 a 16-byte-per-lane copy of one field's whole
  * allocation; TL_K_CHEBY64, TL_K_PPCG64, TL_K_JACOBI64 and TL_K_RESID64 run
  * `reps` launches of that kernel on the CG-initialised fields with fixed
- * coefficients (values not meaningful: for timing and counters only) */
+ * coefficients (values not meaningful: for timing and counters only);
+ * TL_K_CGP_G32 and TL_K_CGP_U32 run `reps` pairs of FP32 inner iterations
+ * (both sweeps of each pair, after two warm-up pairs) */
 int tl_kernel_repeat(tl_ctx *ctx, int kernel_id, int reps);
 
 /* --------------------------------------------------- learned warm start */

@@ -968,9 +968,14 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     HIPCHK(c, launch_ir_prepare(c, rx, ry, p->coefficient));
     const double delta = p->ir_delta > 0 && p->ir_delta < 1 ? p->ir_delta : 0.1;
     HIPCHK(c, launch_init_ctrl(c, MODE_IR, p->max_iters, p->eps, p->rel_tol, delta, 0, 0, p->epslim));
-    const bool fused = p->fp32_kernels == TL_FP32_KERNELS_FUSED ||
-                       (p->fp32_kernels == TL_FP32_KERNELS_AUTO && fused_fits(c));
-    if (fused) RETIF(ensure_fused(c));
+    // pairs of iterations (k_cgp) run as stream launches; with TL_LAUNCH_GRAPH
+    // the automatic choice takes the fused kernel
+    const bool pairs = p->fp32_kernels == TL_FP32_KERNELS_PAIRS ||
+                       (p->fp32_kernels == TL_FP32_KERNELS_AUTO && TL_CGP_AUTO && fused_fits(c) &&
+                        p->launch != TL_LAUNCH_GRAPH);
+    const bool fused = !pairs && (p->fp32_kernels == TL_FP32_KERNELS_FUSED ||
+                                  (p->fp32_kernels == TL_FP32_KERNELS_AUTO && fused_fits(c)));
+    if (fused || pairs) RETIF(ensure_fused(c));
     const double *guess = nullptr;
     RETIF(choose_guess(c, p, rx, ry, &guess, st));
     // u0 = b = energy*density; u = guess or b; bb, tol_abs
@@ -981,6 +986,7 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     // iteration 1 (r and p in the buffers of iteration 0)
     HIPCHK(c, launch_ir_residual(c, 1, t + 1));
     if (fused) HIPCHK(c, launch_cgf_dots(c, 0));
+    int tfirst = 1;  // pairs: the first pair after a start or a refresh
     Ctrl k;
     RETIF(read_ctrl(c, &k));
     st->init_rel_resid = k.bb > 0 ? std::sqrt(k.sums[0] / k.bb) : 0.0;
@@ -988,7 +994,13 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
     while (k.stop_iter == INT_MAX || k.why == WHY_REFRESH) {
         if (k.stop_iter != INT_MAX && k.why == WHY_REFRESH) {
             t = k.stop_iter;
-            if (fused) {
+            if (pairs) {
+                // u += x (complete after every pair), the FP64 residual into
+                // r(t)'s buffer, p(t) kept
+                HIPCHK(c, launch_ir_accumulate(c, t, nullptr));
+                HIPCHK(c, launch_ir_residual(c, 0, t + 1, cgp_r(c, t)));
+                tfirst = t + 1;
+            } else if (fused) {
                 // u += x (with the pending update of an odd t), the FP64
                 // residual into r(t)'s buffer, p(t) kept, and the four sums
                 HIPCHK(c, launch_ir_accumulate(c, t, cgf_p(c, t)));
@@ -1002,7 +1014,15 @@ static int solve_cg_ir(tl_ctx *c, const tl_solve_params *p, double rx, double ry
             RETIF(read_ctrl(c, &k));
             continue;
         }
-        if (fused)
+        if (pairs)
+            RETIF(run_until_stop(c, 32, (long)p->max_iters / 2 + 16,
+                                 [&]() {
+                                     const hipError_t e = launch_cgp(c, t + 1, t + 1 == tfirst);
+                                     t += 2;
+                                     return e;
+                                 },
+                                 &k));
+        else if (fused)
             RETIF(run_until_stop(c, 64, (long)p->max_iters + 16, [&]() { return launch_cgf(c, ++t); }, &k,
                                  p->launch == TL_LAUNCH_GRAPH ? GRAPH_CGF : -1, &t));
         else
@@ -1094,6 +1114,18 @@ extern "C" int tl_kernel_repeat(tl_ctx *c, int kernel_id, int reps)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return TL_OK;
     }
+    if (kernel_id == TL_K_CGP_G32 || kernel_id == TL_K_CGP_U32) {
+        // `reps` pairs of FP32 CG iterations (after two warm-up pairs); both
+        // sweeps of a pair run whichever of the two is named
+        if (!fused_fits(c)) return TL_E_INVALID;
+        RETIF(ensure_fp32(c));
+        RETIF(ensure_fused(c));
+        HIPCHK(c, launch_ir_prepare(c, 1.0, 1.0, 1));
+        HIPCHK(c, launch_ir_residual(c, 1, 1));
+        for (int i = 0; i < 2 + reps; ++i) HIPCHK(c, launch_cgp(c, 1 + 2 * i, i == 0));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return TL_OK;
+    }
     if (kernel_id >= TL_K_CHEBY64) {
         // one FP64 solver kernel, `reps` launches on the CG-initialised fields
         // (u, u0, p, r, kx, ky) with fixed coefficients; never stopped
@@ -1174,9 +1206,11 @@ extern "C" int tl_solve(tl_ctx *c, const tl_solve_params *p, double rx, double r
         c->err = "TL_FP32_IR needs the CG solver and the relative stop rule";
         return TL_E_INVALID;
     }
-    if (p->fp32_kernels < TL_FP32_KERNELS_AUTO || p->fp32_kernels > TL_FP32_KERNELS_FUSED ||
-        (p->precision == TL_FP32_IR && p->fp32_kernels == TL_FP32_KERNELS_FUSED && !fused_fits(c))) {
-        c->err = "tl_solve: fp32_kernels must be TL_FP32_KERNELS_AUTO, _TWO or _FUSED (fused needs nx % 4 == 0)";
+    if (p->fp32_kernels < TL_FP32_KERNELS_AUTO || p->fp32_kernels > TL_FP32_KERNELS_PAIRS ||
+        (p->precision == TL_FP32_IR &&
+         (p->fp32_kernels == TL_FP32_KERNELS_FUSED || p->fp32_kernels == TL_FP32_KERNELS_PAIRS) && !fused_fits(c))) {
+        c->err = "tl_solve: fp32_kernels must be TL_FP32_KERNELS_AUTO, _TWO, _FUSED or _PAIRS (fused and pairs need "
+                 "nx % 4 == 0)";
         return TL_E_INVALID;
     }
     if (p->launch != TL_LAUNCH_STREAM && p->launch != TL_LAUNCH_GRAPH) {

@@ -212,6 +212,8 @@ extern "C" int tl_deck_parse(const char *text, tl_deck *d)
                 d->params.fp32_kernels = TL_FP32_KERNELS_TWO;
             else if (v == "fused")
                 d->params.fp32_kernels = TL_FP32_KERNELS_FUSED;
+            else if (v == "pairs")
+                d->params.fp32_kernels = TL_FP32_KERNELS_PAIRS;
             else
                 ok = false;
         } else if (key == "warmpolicy") {

@@ -121,6 +121,11 @@ constexpr int TBV = TL_TBV;
 #ifndef TL_CGF_DEPTH
 #define TL_CGF_DEPTH 0
 #endif
+// TL_FP32_KERNELS_AUTO takes the pairs of iterations (k_cgp) where the fused
+// kernel fits and launches are stream launches (1), or the fused kernel (0)
+#ifndef TL_CGP_AUTO
+#define TL_CGP_AUTO 0
+#endif
 // neighbour exchange of the FP32 PW kernel by DPP wave shifts (else LDS shuffles)
 #ifndef TL_DPP
 #define TL_DPP 1
@@ -354,6 +359,9 @@ hipError_t launch_cgf(tl_ctx *c, int t, int tref = 0);
 hipError_t launch_cgf_dots(tl_ctx *c, int T);
 float *cgf_r(tl_ctx *c, int t);
 float *cgf_p(tl_ctx *c, int t);
+hipError_t launch_cgp(tl_ctx *c, int t, bool first, int tref = 0);
+float *cgp_r(tl_ctx *c, int T);
+float *cgp_p(tl_ctx *c, int T);
 hipError_t launch_apply(tl_ctx *c, const double *src, double *dst);
 hipError_t launch_finalise(tl_ctx *c);
 hipError_t launch_field_summary(tl_ctx *c);

@@ -2007,6 +2007,332 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, c
     write_partials<4, TBV>(acc, sout, g.nbmax);
 }
 
+// Two CG iterations per pair of sweeps (s-step CG with s = 2, monomial
+// basis).  From r = r(t-1) and p' = p(t-1), with p = r + beta(t) p', the two
+// iterations t and t+1 lie in the span of p, A p, A^2 p, r and A r:
+//   alpha0 = gamma / (p, A p)                        gamma = (r, r)
+//   r1 = r - alpha0 A p,        gamma1 = gamma - 2 alpha0 (r, A p) + alpha0^2 (A p, A p)
+//   beta0 = gamma1 / gamma,     p1 = r1 + beta0 p,   A p1 = A r - alpha0 A^2 p + beta0 A p
+//   (p1, A p1) = (r, A r) - 2 alpha0 (A p, A r) + alpha0^2 (A p, A^2 p)
+//              + 2 beta0 ((r, A p) - alpha0 (A p, A p)) + beta0^2 (p, A p)
+//   alpha1 = gamma1 / (p1, A p1)
+//   x += alpha0 p + alpha1 p1,  r(t+1) = r1 - alpha1 A p1,  p(t+1) = p1.
+// The first sweep (UPD false) forms p, A p, A^2 p and A r and sums the six
+// products (p, Ap), (r, Ap), (Ap, Ap), (r, Ar), (Ap, Ar), (Ap, A^2 p); it
+// stores nothing.  The second (UPD true) forms the same vectors bit for bit,
+// derives the scalars from the six sums and writes r(t+1), p(t+1) and x,
+// with (r(t+1), r(t+1)) for the next pair.  Bytes per cell for the two
+// iterations: 10 (r, p', 16-bit gc) + 26 (also x; writes r, p, x) = 36, or
+// 18 per iteration against 22 for k_cgf.  The walk is k_cgf's: rows L-3 .. L
+// in registers (p at L-2..L, r at L-2..L, A p at L-3..L-1, A r at L-2, L-1),
+// A p and A r at row L-1, A^2 p and the row's results at L-2; the edge lanes
+// compute A p at the column next to the wave from the two outside columns.
+// Stops and refreshes are decided between pairs: the first sweep of pair t
+// tests gamma(t-1) as k_cgf does; when the pair's second step cannot be
+// taken (gamma1 or (p1, A p1) not positive, or t + 1 beyond the iteration
+// limit) the second sweep applies the first step only.
+template <int FMT, bool UPD>
+__global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, const float *__restrict__ rin,
+                                                         float *__restrict__ rout, const float *__restrict__ pin,
+                                                         float *__restrict__ pout, float *__restrict__ x,
+                                                         const tl_gc_t *__restrict__ gc, float hx, float hy,
+                                                         Ctrl *__restrict__ ctrl, const double *__restrict__ sin,
+                                                         unsigned nbin, double *__restrict__ sout)
+{
+    constexpr int NS = UPD ? 1 : 6;
+    t = iter_of(t, ctrl);
+    if (t > ctrl->stop_iter) return;
+    const VCol<4> cl = vcolumn<4>(g, ty);
+    const int x0 = cl.x0, lane = cl.lane;
+    const bool eleft = lane == 0 && x0 > 0 && cl.active;
+    const bool eright = lane == 63 && x0 + 4 < g.nx && cl.active;
+    const bool eneed = eleft || eright;
+    const long eoff = lane == 0 ? -2 : 4;
+    const int hk = lane == 0 ? 1 : 0;
+    const bool at_left = x0 == 0, at_right = x0 + 4 >= g.nx;
+
+    auto load_batch = [&](FBatch<1> &B, int yb) {
+        const int yy = min(max(yb, 0), g.ny - 1);
+        const long i = at(g, cl.xl, yy);
+        vload<float, 4>(B.r[0], rin + i);
+        vload<float, 4>(B.p[0], pin + i);
+        {
+            typedef unsigned uv2 __attribute__((ext_vector_type(2)));
+            const uv2 w = *reinterpret_cast<const uv2 *>(gc + i);
+            B.g[0][0] = w[0];
+            B.g[0][1] = w[1];
+        }
+        if constexpr (UPD) {  // x of the row the sweep finishes, two rows behind
+            const int yx = min(max(yb - 2, 0), g.ny - 1);
+            vload<float, 4>(B.x[0], x + at(g, cl.xl, yx));
+        }
+        B.er[0][0] = B.er[0][1] = B.ep[0][0] = B.ep[0][1] = 0.f;
+        B.eg[0] = 0u;
+        if (eneed) {
+            typedef float fv2 __attribute__((ext_vector_type(2)));
+            const long ie = i + eoff;
+            const fv2 er = *reinterpret_cast<const fv2 *>(rin + ie);
+            const fv2 ep = *reinterpret_cast<const fv2 *>(pin + ie);
+            B.er[0][0] = er[0];
+            B.er[0][1] = er[1];
+            B.ep[0][0] = ep[0];
+            B.ep[0][1] = ep[1];
+            B.eg[0] = *reinterpret_cast<const unsigned *>(gc + ie);
+        }
+    };
+
+    FBatch<1> B0, B1;
+    load_batch(B0, cl.y0 - 2);
+    // scalars of the pair, identical in every block
+    double bd, a0 = 0.0, b0 = 0.0, a1 = 0.0;
+    if constexpr (!UPD) {
+        double gam;
+        if (sin) {
+            double s1[1];
+            sum_partials<1, TBV>(sin, nbin, g.nbmax, s1);
+            gam = s1[0];
+        } else {
+            gam = ctrl->rrs[(t - 1) & 1];  // after a start or an FP64 refresh
+        }
+        const int why = decide_why(ctrl, t - 1, gam);
+        bd = t >= 2 ? gam / ctrl->rrs[(t - 2) & 1] : 0.0;
+        if (block_id() == 0 && threadIdx.x == 0) {
+            ctrl->rrs[(t - 1) & 1] = gam;
+            ctrl->rro = gam;
+            ctrl->err = gam;
+            ctrl->beta = bd;
+            ctrl->iters = t - 1;
+            record_stop(ctrl, t - 1, why);
+        }
+        if (why != WHY_NONE) return;
+    } else {
+        double s6[6];
+        sum_partials<6, TBV>(sin, nbin, g.nbmax, s6);
+        const double gam = ctrl->rrs[(t - 1) & 1];
+        bd = ctrl->beta;
+        const double pap = s6[0], rap = s6[1], apap = s6[2], rar = s6[3], apar = s6[4], apa2p = s6[5];
+        a0 = gam / pap;
+        const bool ok = pap > 0.0 && isfinite(a0);
+        const double g1 = gam - 2.0 * a0 * rap + a0 * a0 * apap;
+        b0 = g1 / gam;
+        const double p1ap1 =
+            (rar - 2.0 * a0 * apar + a0 * a0 * apa2p) + 2.0 * b0 * (rap - a0 * apap) + b0 * b0 * pap;
+        a1 = g1 / p1ap1;
+        const bool cut = t + 1 > ctrl->maxiters;
+        const bool two = ok && !cut && g1 > 0.0 && p1ap1 > 0.0 && isfinite(a1);
+        if (!two) {
+            a1 = 0.0;
+            b0 = 0.0;
+        }
+        if (block_id() == 0 && threadIdx.x == 0) {
+            ctrl->pw = pap;
+            ctrl->alpha = two ? a1 : a0;
+            ctrl->alphas[t & 1] = a0;
+            ctrl->alphas[(t + 1) & 1] = a1;
+            ctrl->rrs[t & 1] = g1 > 0.0 ? g1 : gam;
+            ctrl->iters = cut ? t : t + 1;
+            if (!ok) {  // non-positive curvature: stop before this pair's update
+                ctrl->status = TL_E_BREAKDOWN;
+                ctrl->why = WHY_BREAKDOWN;
+                ctrl->iters = t - 1;
+                ctrl->stop_iter = t - 1;
+            } else if (cut) {
+                record_stop(ctrl, t, WHY_MAXITERS);
+            }
+        }
+        if (!ok) return;
+    }
+    const float beta = (float)bd;
+    const float fa0 = (float)a0, fb0 = (float)b0, fa1 = (float)a1;
+    double acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+
+    {
+        const int y0 = cl.y0, y1 = cl.y1;
+        float P0[4], P1[4], P2[4], P0h = 0.f, P1h = 0.f, P1f = 0.f, P2h = 0.f, P2f = 0.f;
+        float R0[4], R1[4], R2[4], R1h = 0.f, R2h = 0.f;
+        float N0[4], N1[4], N2[4], N1h = 0.f, N2h = 0.f;  // A p at L-3, L-2, L-1
+        float AR1[4], AR2[4];                            // A r at L-2, L-1
+        float Gc[4], Gch = 0.f, Gcf = 0.f;
+        unsigned Gw[2] = {0u, 0u};
+        float KX1[5], KX2[5], KX1o = 0.f;
+        float KY0[4], KY1[4], KY2[4], KY1h = 0.f, KY2h = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            P0[j] = P1[j] = P2[j] = R0[j] = R1[j] = R2[j] = N0[j] = N1[j] = N2[j] = 0.f;
+            AR1[j] = AR2[j] = Gc[j] = KY0[j] = KY1[j] = KY2[j] = 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) KX1[j] = KX2[j] = 0.f;
+        (void)P0h;
+
+        auto nbr_l = [&](float src3, float c0, float hval) -> float {
+            float v = dpp_up1f(src3);
+            if (eleft) v = hval;
+            if (at_left) v = c0;
+            return v;
+        };
+        auto nbr_r = [&](float src0, float c3, float hval) -> float {
+            float v = dpp_down1f(src0);
+            if (eright) v = hval;
+            if (at_right) v = c3;
+            return v;
+        };
+
+        auto compute_row = [&](const FBatch<1> &B, int L) {
+            if (L > y1 + 1) return;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                P0[j] = P1[j];
+                P1[j] = P2[j];
+                R0[j] = R1[j];
+                R1[j] = R2[j];
+                N0[j] = N1[j];
+                N1[j] = N2[j];
+                AR1[j] = AR2[j];
+                KY0[j] = KY1[j];
+                KY1[j] = KY2[j];
+            }
+#pragma unroll
+            for (int j = 0; j < 5; ++j) KX2[j] = KX1[j];
+            P0h = P1h;
+            P1h = P2h;
+            P1f = P2f;
+            R1h = R2h;
+            N1h = N2h;
+            KY1h = KY2h;
+            float gL[4];
+            const unsigned wL[2] = {B.g[0][0], B.g[0][1]};
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                gL[2 * k] = gc_word<FMT>(wL[k], 0);
+                gL[2 * k + 1] = gc_word<FMT>(wL[k], 1);
+            }
+            const float gLh = gc_word<FMT>(B.eg[0], hk), gLf = gc_word<FMT>(B.eg[0], 1 - hk);
+            {
+                float gl = gc_word<FMT>(dpp_up1u(Gw[1]), 1), gr = gc_word<FMT>(dpp_down1u(Gw[0]), 0);
+                if (eleft) gl = Gch;
+                if (eright) gr = Gch;
+                KX1[0] = hx * (gl + Gc[0]);
+#pragma unroll
+                for (int j = 1; j < 4; ++j) KX1[j] = hx * (Gc[j - 1] + Gc[j]);
+                KX1[4] = hx * (Gc[3] + gr);
+                KX1o = lane == 0 ? hx * (Gcf + Gch) : hx * (Gch + Gcf);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) KY2[j] = hy * (Gc[j] + gL[j]);
+                KY2h = hy * (Gch + gLh);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Gc[j] = gL[j];
+            Gch = gLh;
+            Gcf = gLf;
+            Gw[0] = wL[0];
+            Gw[1] = wL[1];
+            // row L: r and p = r + beta p'
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                R2[j] = B.r[0][j];
+                P2[j] = __builtin_fmaf(beta, B.p[0][j], B.r[0][j]);
+            }
+            {
+                const float rh = B.er[0][hk], rf = B.er[0][1 - hk];
+                P2h = __builtin_fmaf(beta, B.ep[0][hk], rh);
+                P2f = __builtin_fmaf(beta, B.ep[0][1 - hk], rf);
+                R2h = rh;
+            }
+            // A p and A r on row L-1
+            if (L >= y0) {
+                const int y = L - 1;
+                const bool ybot = y <= 0, ytop = y >= g.ny - 1;
+                const float pl = nbr_l(P1[3], P1[0], P1h), pr = nbr_r(P1[0], P1[3], P1h);
+                const float rl = nbr_l(R1[3], R1[0], R1h), rr = nbr_r(R1[0], R1[3], R1h);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float c = P1[j];
+                    const float l = j > 0 ? P1[j - 1] : pl;
+                    const float r = j < 3 ? P1[j + 1] : pr;
+                    const float d = ybot ? c : P0[j];
+                    const float u = ytop ? c : P2[j];
+                    N2[j] = smvp5f(c, l, r, d, u, KX1[j], KX1[j + 1], KY1[j], KY2[j]);
+                    const float cr = R1[j];
+                    const float lr = j > 0 ? R1[j - 1] : rl;
+                    const float rrr = j < 3 ? R1[j + 1] : rr;
+                    const float dr = ybot ? cr : R0[j];
+                    const float ur = ytop ? cr : R2[j];
+                    AR2[j] = smvp5f(cr, lr, rrr, dr, ur, KX1[j], KX1[j + 1], KY1[j], KY2[j]);
+                }
+                {
+                    // A p at the column next to the wave (edge lanes)
+                    const float c = P1h;
+                    const float own = lane == 0 ? P1[0] : P1[3];
+                    const float l = lane == 0 ? P1f : own, r = lane == 0 ? own : P1f;
+                    const float kxl = lane == 0 ? KX1o : KX1[4], kxr = lane == 0 ? KX1[0] : KX1o;
+                    const float d = ybot ? c : P0h, u = ytop ? c : P2h;
+                    N2h = smvp5f(c, l, r, d, u, kxl, kxr, KY1h, KY2h);
+                }
+            }
+            // A^2 p on row L-2 and the row's sums or updates
+            if (L >= y0 + 2) {
+                const int y = L - 2;
+                if (y < y1 && cl.active) {
+                    const bool ybot = y <= 0, ytop = y >= g.ny - 1;
+                    const float nl = nbr_l(N1[3], N1[0], N1h), nr = nbr_r(N1[0], N1[3], N1h);
+                    float o_r[4], o_p[4], o_x[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float c = N1[j];
+                        const float l = j > 0 ? N1[j - 1] : nl;
+                        const float r = j < 3 ? N1[j + 1] : nr;
+                        const float d = ybot ? c : N0[j];
+                        const float u = ytop ? c : N2[j];
+                        const float a2p = smvp5f(c, l, r, d, u, KX2[j], KX2[j + 1], KY0[j], KY1[j]);
+                        const float pv = P0[j], apv = N1[j], rv = R0[j], arv = AR1[j];
+                        if constexpr (!UPD) {
+                            const double ap = (double)apv;
+                            acc[0] += (double)pv * ap;
+                            acc[1] += (double)rv * ap;
+                            acc[2] += ap * ap;
+                            acc[3] += (double)rv * (double)arv;
+                            acc[4] += ap * (double)arv;
+                            acc[5] += ap * (double)a2p;
+                            (void)o_r;
+                            (void)o_p;
+                            (void)o_x;
+                        } else {
+                            const float r1 = __builtin_fmaf(-fa0, apv, rv);
+                            const float p1 = __builtin_fmaf(fb0, pv, r1);
+                            const float ar1 = __builtin_fmaf(-fa0, a2p, arv);
+                            const float ap1 = __builtin_fmaf(fb0, apv, ar1);
+                            o_x[j] = __builtin_fmaf(fa1, p1, __builtin_fmaf(fa0, pv, B.x[0][j]));
+                            o_r[j] = __builtin_fmaf(-fa1, ap1, r1);
+                            o_p[j] = p1;
+                            acc[0] += (double)o_r[j] * (double)o_r[j];
+                        }
+                    }
+                    if constexpr (UPD) {
+                        const long i = at(g, x0, y);
+                        vstore<float, 4>(rout + i, o_r);
+                        vstore<float, 4>(pout + i, o_p);
+                        vstore<float, 4>(x + i, o_x);
+                    }
+                }
+            }
+        };
+        // rows y0-2 .. y1+1, the next row's loads issued before the current
+        // row is computed (the first one above)
+        for (int L = y0 - 2; L < y1 + 2; L += 2) {
+            const bool more = L + 1 < y1 + 2;
+            if (more) load_batch(B1, L + 1);
+            compute_row(B0, L);
+            if (!more) break;
+            if (L + 2 < y1 + 2) load_batch(B0, L + 2);
+            compute_row(B1, L + 1);
+        }
+    }
+    write_partials<NS, TBV>(acc, sout, g.nbmax);
+}
+
 // The four sums k_cgf needs at its first iteration and after an FP64 refresh,
 // for the residual r and the kept direction p: (r, r), (r, A r), (r, A p),
 // (p, A p), with the same FP32 operator (flux form, clamped neighbours).  One
@@ -3414,6 +3740,34 @@ hipError_t launch_cgf(tl_ctx *c, int t, int tref)
                                         cgf_p(c, t - 1), cgf_p(c, t), c->x32, c->gcell, c->hx32, c->hy32, c->ctrl,
                                         part_f(c, t - 1), c->nb_f[(t - 1) & 1], part_f(c, t));
     c->nb_f[t & 1] = nbf(c);
+    if (prof) prof_stop(c);
+    return hipGetLastError();
+}
+
+// pairs of FP32 inner CG iterations (k_cgp): after iteration T (even), r(T)
+// and p(T) in buffer (T / 2) & 1; the six sums of a pair's first sweep in
+// part_f[0 .. 6 stride), the residual sum of its second in part_f[6 stride ..)
+float *cgp_r(tl_ctx *c, int T) { return ((T >> 1) & 1) ? c->r32b : c->r32; }
+float *cgp_p(tl_ctx *c, int T) { return ((T >> 1) & 1) ? c->p32b : c->p32; }
+
+// the pair t, t+1 (t odd); first: the first pair after a start or an FP64
+// refresh, whose gamma(t-1) is in Ctrl::rrs
+hipError_t launch_cgp(tl_ctx *c, int t, bool first, int tref)
+{
+    double *const pg = c->part_f, *const pu = c->part_f + 6 * stride(c);
+    const unsigned nb = nbf(c);
+    const bool fp16 = c->gc_fmt == GC_F16;
+    bool prof = prof_start(c, TL_K_CGP_G32);
+    auto kg = fp16 ? k_cgp<GC_F16, false> : k_cgp<GC_BF16, false>;
+    kg<<<nb, TBV, 0, c->stream>>>(c->g, c->g.tyf, targ(t, tref), cgp_r(c, t - 1), nullptr, cgp_p(c, t - 1), nullptr,
+                                  nullptr, c->gcell, c->hx32, c->hy32, c->ctrl, first ? nullptr : pu, nb, pg);
+    if (prof) prof_stop(c);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    prof = prof_start(c, TL_K_CGP_U32);
+    auto ku = fp16 ? k_cgp<GC_F16, true> : k_cgp<GC_BF16, true>;
+    ku<<<nb, TBV, 0, c->stream>>>(c->g, c->g.tyf, targ(t, tref), cgp_r(c, t - 1), cgp_r(c, t + 1), cgp_p(c, t - 1),
+                                  cgp_p(c, t + 1), c->x32, c->gcell, c->hx32, c->hy32, c->ctrl, pg, nb, pu);
     if (prof) prof_stop(c);
     return hipGetLastError();
 }

@@ -39,9 +39,9 @@ EXPORTS = [
 ]
 TL_START_PREVIOUS, TL_START_WARM = 0, 1
 (TL_K_CG_W64, TL_K_CG_UR64, TL_K_CG_P64, TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32, TL_K_CHEBY64, TL_K_PPCG64,
- TL_K_JACOBI64, TL_K_RESID64, TL_K_CGF32, TL_K_PPCG2_64, TL_K_CHEBY2_64) = range(13)
-TL_K_COUNT = 13
-TL_FP32_KERNELS_AUTO, TL_FP32_KERNELS_TWO, TL_FP32_KERNELS_FUSED = 0, 1, 2
+ TL_K_JACOBI64, TL_K_RESID64, TL_K_CGF32, TL_K_PPCG2_64, TL_K_CHEBY2_64, TL_K_CGP_G32, TL_K_CGP_U32) = range(15)
+TL_K_COUNT = 15
+TL_FP32_KERNELS_AUTO, TL_FP32_KERNELS_TWO, TL_FP32_KERNELS_FUSED, TL_FP32_KERNELS_PAIRS = 0, 1, 2, 3
 TL_WARM_POLICY_BACKOFF, TL_WARM_POLICY_EVERY = 0, 1
 TL_PPCG_PAIRS_AUTO, TL_PPCG_PAIRS_OFF = 0, 1
 TL_CHEBY_PAIRS_AUTO, TL_CHEBY_PAIRS_OFF = 0, 1
@@ -49,7 +49,7 @@ KERNEL_NAMES = {0: "k_cg_pwv<double, 2, 4, true>", 1: "k_cg_urv<double, 2, 4, fa
                 3: "k_cgg<4, false, 2, true, 1>", 4: "k_cgg<4, true, 4, true, 1>",
                 5: "(fused into k_cgg<4, false, 2, true, 1>)",
                 6: "k_vs64<0, 4, true>", 7: "k_vs64<1, 4, true>", 8: "k_vs64<2, 4, true>", 9: "k_residual",
-                10: "k_cgf<1, 1, 0>", 11: "k_ppcg2", 12: "k_cheby2",
+                10: "k_cgf<1, 1, 0>", 11: "k_ppcg2", 12: "k_cheby2", 13: "k_cgp<1, false>", 14: "k_cgp<1, true>",
                 -1: "k_copy_full", -2: "k_copy_vec"}
 
 
@@ -57,12 +57,14 @@ def kernel_name(kid, coef_format=None):
     """Symbol of profiled kernel `kid`.  The FP32 kernels' last template
     argument is the cell coefficient's 16-bit format (1 = scaled half,
     0 = bfloat16; the fused kernel's second, before its LDS staging depth,
-    0 in the default build), which the solver picks per solve
+    0 in the default build; the pair kernels' first), which the solver picks per solve
     (`SolveStats.coef_format`); without a format the half instantiation is
     named."""
     name = KERNEL_NAMES[kid]
     if coef_format == TL_COEF_BF16 and kid == TL_K_CGF32:
         name = name.replace("<1, 1,", "<1, 0,")
+    elif coef_format == TL_COEF_BF16 and kid in (TL_K_CGP_G32, TL_K_CGP_U32):
+        name = name.replace("<1,", "<0,")
     elif coef_format == TL_COEF_BF16 and kid in (TL_K_CG_W32, TL_K_CG_UR32, TL_K_CG_P32):
         name = name.replace(", 1>", ", 0>")
     return name

@@ -134,7 +134,8 @@ typedef struct {
 
 /* tl_solve_params.fp32_kernels: how a TL_FP32_IR iteration is split into kernels.
  * Both reach the same FP64 tolerance; their FP32 roundings differ. */
-#define TL_FP32_KERNELS_AUTO 0   /* fused when nx is a multiple of 4, else two kernels */
+#define TL_FP32_KERNELS_AUTO 0   /* nx a multiple of 4: pairs with stream launches, fused with
+                                   TL_LAUNCH_GRAPH; else two kernels */
 #define TL_FP32_KERNELS_TWO 1    /* two kernels and two reductions per iteration (34 B/cell) */
 #define TL_FP32_KERNELS_FUSED 2  /* one kernel and one reduction per iteration (22 B/cell); needs nx % 4 == 0 */
 #define TL_FP32_KERNELS_PAIRS 3  /* two iterations per two kernels and two reductions (s-step CG, s = 2;

@@ -2081,6 +2081,8 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
         }
     };
 
+    // the band's first row is requested before the block sums the previous
+    // sweep's partials
     FBatch<1> B0, B1;
     load_batch(B0, cl.y0 - 2);
     // scalars of the pair, identical in every block
@@ -2241,35 +2243,28 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                 P2f = __builtin_fmaf(beta, B.ep[0][1 - hk], rf);
                 R2h = rh;
             }
-            // A p and A r on row L-1
+            // A p and A r on row L-1.  The rows outside the grid were loaded
+            // clamped, so p and r there equal the edge row's, bit for bit:
+            // the zero-flux neighbours need no select here (A^2 p below, from
+            // A p of rows outside the grid, does)
             if (L >= y0) {
-                const int y = L - 1;
-                const bool ybot = y <= 0, ytop = y >= g.ny - 1;
                 const float pl = nbr_l(P1[3], P1[0], P1h), pr = nbr_r(P1[0], P1[3], P1h);
                 const float rl = nbr_l(R1[3], R1[0], R1h), rr = nbr_r(R1[0], R1[3], R1h);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const float c = P1[j];
                     const float l = j > 0 ? P1[j - 1] : pl;
                     const float r = j < 3 ? P1[j + 1] : pr;
-                    const float d = ybot ? c : P0[j];
-                    const float u = ytop ? c : P2[j];
-                    N2[j] = smvp5f(c, l, r, d, u, KX1[j], KX1[j + 1], KY1[j], KY2[j]);
-                    const float cr = R1[j];
+                    N2[j] = smvp5f(P1[j], l, r, P0[j], P2[j], KX1[j], KX1[j + 1], KY1[j], KY2[j]);
                     const float lr = j > 0 ? R1[j - 1] : rl;
                     const float rrr = j < 3 ? R1[j + 1] : rr;
-                    const float dr = ybot ? cr : R0[j];
-                    const float ur = ytop ? cr : R2[j];
-                    AR2[j] = smvp5f(cr, lr, rrr, dr, ur, KX1[j], KX1[j + 1], KY1[j], KY2[j]);
+                    AR2[j] = smvp5f(R1[j], lr, rrr, R0[j], R2[j], KX1[j], KX1[j + 1], KY1[j], KY2[j]);
                 }
                 {
                     // A p at the column next to the wave (edge lanes)
-                    const float c = P1h;
                     const float own = lane == 0 ? P1[0] : P1[3];
                     const float l = lane == 0 ? P1f : own, r = lane == 0 ? own : P1f;
                     const float kxl = lane == 0 ? KX1o : KX1[4], kxr = lane == 0 ? KX1[0] : KX1o;
-                    const float d = ybot ? c : P0h, u = ytop ? c : P2h;
-                    N2h = smvp5f(c, l, r, d, u, kxl, kxr, KY1h, KY2h);
+                    N2h = smvp5f(P1h, l, r, P0h, P2h, kxl, kxr, KY1h, KY2h);
                 }
             }
             // A^2 p on row L-2 and the row's sums or updates
@@ -2289,13 +2284,15 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                         const float a2p = smvp5f(c, l, r, d, u, KX2[j], KX2[j + 1], KY0[j], KY1[j]);
                         const float pv = P0[j], apv = N1[j], rv = R0[j], arv = AR1[j];
                         if constexpr (!UPD) {
-                            const double ap = (double)apv;
-                            acc[0] += (double)pv * ap;
-                            acc[1] += (double)rv * ap;
-                            acc[2] += ap * ap;
-                            acc[3] += (double)rv * (double)arv;
-                            acc[4] += ap * (double)arv;
-                            acc[5] += ap * (double)a2p;
+                            // a product of two FP32 values is exact in FP64, so
+                            // the fused multiply-add rounds as acc + a * b does
+                            const double ap = (double)apv, rd = (double)rv, ard = (double)arv;
+                            acc[0] = __builtin_fma((double)pv, ap, acc[0]);
+                            acc[1] = __builtin_fma(rd, ap, acc[1]);
+                            acc[2] = __builtin_fma(ap, ap, acc[2]);
+                            acc[3] = __builtin_fma(rd, ard, acc[3]);
+                            acc[4] = __builtin_fma(ap, ard, acc[4]);
+                            acc[5] = __builtin_fma(ap, (double)a2p, acc[5]);
                             (void)o_r;
                             (void)o_p;
                             (void)o_x;
@@ -2307,7 +2304,7 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                             o_x[j] = __builtin_fmaf(fa1, p1, __builtin_fmaf(fa0, pv, B.x[0][j]));
                             o_r[j] = __builtin_fmaf(-fa1, ap1, r1);
                             o_p[j] = p1;
-                            acc[0] += (double)o_r[j] * (double)o_r[j];
+                            acc[0] = __builtin_fma((double)o_r[j], (double)o_r[j], acc[0]);
                         }
                     }
                     if constexpr (UPD) {
@@ -2320,7 +2317,8 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
             }
         };
         // rows y0-2 .. y1+1, the next row's loads issued before the current
-        // row is computed (the first one above)
+        // row is computed (the first one above; two rows ahead in the first
+        // sweep measured no faster: it is bound by its arithmetic)
         for (int L = y0 - 2; L < y1 + 2; L += 2) {
             const bool more = L + 1 < y1 + 2;
             if (more) load_batch(B1, L + 1);

@@ -104,7 +104,7 @@ def test_parser_mixed_precision_keys():
     assert s.stop_rule == T.TL_STOP_RELATIVE
     assert s._deck.params.fp32_kernels == _lib.TL_FP32_KERNELS_AUTO
     for v, code in (("two", _lib.TL_FP32_KERNELS_TWO), ("fused", _lib.TL_FP32_KERNELS_FUSED),
-                    ("auto", _lib.TL_FP32_KERNELS_AUTO)):
+                    ("pairs", _lib.TL_FP32_KERNELS_PAIRS), ("auto", _lib.TL_FP32_KERNELS_AUTO)):
         assert T.Settings.from_text(base + f"fp32_kernels={v}\n")._deck.params.fp32_kernels == code
     assert s._deck.params.warm_policy == _lib.TL_WARM_POLICY_BACKOFF
     assert T.Settings.from_text(base + "warm_policy=every\n")._deck.params.warm_policy == _lib.TL_WARM_POLICY_EVERY

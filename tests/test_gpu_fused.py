@@ -65,7 +65,7 @@ def test_fused_matches_oracle_and_two_kernel_form(nx, ny):
 
 
 def test_fused_auto_selection_and_validation():
-    print("case: fused form selection (auto) and its nx % 4 requirement", flush=True)
+    print("case: form selection (auto) and the fused form's nx % 4 requirement", flush=True)
     # nx % 4 != 0: auto takes the two-kernel form, an explicit fused request is refused
     text = _deck(66, 20, 3, 1)
     s = T.Settings.from_text(text).set(fp32_kernels=_lib.TL_FP32_KERNELS_FUSED)
@@ -76,11 +76,13 @@ def test_fused_auto_selection_and_validation():
     a, ast_, _ = _run(text, 1, _lib.TL_FP32_KERNELS_AUTO)
     b, bst, _ = _run(text, 1, _lib.TL_FP32_KERNELS_TWO)
     assert np.array_equal(a, b) and ast_[0]["iterations"] == bst[0]["iterations"]
-    # nx % 4 == 0: auto is the fused form, bit for bit
+    # nx % 4 == 0: auto is the pairs of iterations with stream launches and
+    # the fused form with hipGraph batches, bit for bit
     text = _deck(128, 96, 3, 1)
-    a, ast_, ua = _run(text, 1, _lib.TL_FP32_KERNELS_AUTO)
-    b, bst, ub = _run(text, 1, _lib.TL_FP32_KERNELS_FUSED)
-    assert np.array_equal(a, b) and np.array_equal(ua, ub) and ast_[0]["iterations"] == bst[0]["iterations"]
+    for launch, form in ((T.TL_LAUNCH_STREAM, _lib.TL_FP32_KERNELS_PAIRS), (T.TL_LAUNCH_GRAPH, _lib.TL_FP32_KERNELS_FUSED)):
+        a, ast_, ua = _run(text, 1, _lib.TL_FP32_KERNELS_AUTO, launch)
+        b, bst, ub = _run(text, 1, form, launch)
+        assert np.array_equal(a, b) and np.array_equal(ua, ub) and ast_[0]["iterations"] == bst[0]["iterations"]
 
 
 def test_fused_graph_replays_bit_identical_to_stream():

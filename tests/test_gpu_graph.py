@@ -5,7 +5,9 @@ stream launches; only the iteration number comes from device memory.  Every
 field and every solver statistic must therefore be bit-identical to the
 stream run, after every timestep: each solve replays the cached graph many
 times, with the stream's own work (FP64 refreshes, finalise, halo updates,
-field summary) and host synchronisations between the replays.
+field summary) and host synchronisations between the replays.  The FP32
+cases name their form: the automatic choice differs between the two launch
+modes (pairs of iterations run as stream launches only).
 """
 import json
 import os
@@ -49,7 +51,9 @@ REL = "stop_rule=relative\nrel_tol=1e-10\n"
 
 CASES = [
     ("fp64 relative 256^2 random rectangles", decks.random_rect_deck(3, n=256, steps=3), 3),
-    ("fp32_ir 256^2 random rectangles", decks.random_rect_deck(3, n=256, steps=3, precision="fp32_ir"), 3),
+    # the one-kernel FP32 iteration (the form graph batches run when nx % 4 == 0)
+    ("fp32_ir 256^2 random rectangles",
+     decks.random_rect_deck(3, n=256, steps=3, precision="fp32_ir", extra="fp32_kernels=fused\n"), 3),
     ("fp64 reference rule 128^2 reference deck", decks.reference_deck(128, 2, max_iters=700), 2),
     ("fp32_ir 131x77 reference deck",
      decks.reference_deck(131, 2, extra=REL + "precision=fp32_ir\n").replace("y_cells=131", "y_cells=77"), 2),
@@ -72,7 +76,7 @@ def test_graph_recaptured_when_coefficients_change():
     """A new timestep size changes the FP32 face scales held by the captured
     launches: the cached graph must not be replayed with the old ones."""
     print("case: graph vs stream, fp32_ir 192^2 with dt changing per step", flush=True)
-    text = decks.random_rect_deck(9, n=192, steps=3, precision="fp32_ir")
+    text = decks.random_rect_deck(9, n=192, steps=3, precision="fp32_ir", extra="fp32_kernels=fused\n")
     dts = [0.004, 0.01, 0.004]
     _same(_run(text, T.TL_LAUNCH_GRAPH, 3, dts), _run(text, T.TL_LAUNCH_STREAM, 3, dts))
 

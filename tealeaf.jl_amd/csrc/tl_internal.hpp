@@ -126,6 +126,11 @@ constexpr int TBV = TL_TBV;
 // Measured at 8192^2: 0.245 ms per iteration (first sweep 0.145, second
 // 0.345 per pair) against 0.287 for the fused kernel, at the same iteration
 // counts.
+// the pair's second step: A p1 by the stencil of p1 formed on three rows (1),
+// or as A r - alpha0 A^2 p + beta0 A p (0)
+#ifndef TL_CGP_AP1
+#define TL_CGP_AP1 1
+#endif
 #ifndef TL_CGP_AUTO
 #define TL_CGP_AUTO 1
 #endif

@@ -2156,6 +2156,7 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
         float R0[4], R1[4], R2[4], R1h = 0.f, R2h = 0.f;
         float N0[4], N1[4], N2[4], N1h = 0.f, N2h = 0.f;  // A p at L-3, L-2, L-1
         float AR1[4], AR2[4];                            // A r at L-2, L-1
+        float Q0[4], Q1[4], Q2[4], Q1h = 0.f, Q2h = 0.f; // second sweep: p1 at L-3, L-2, L-1
         float Gc[4], Gch = 0.f, Gcf = 0.f;
         unsigned Gw[2] = {0u, 0u};
         float KX1[5], KX2[5], KX1o = 0.f;
@@ -2164,6 +2165,7 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
         for (int j = 0; j < 4; ++j) {
             P0[j] = P1[j] = P2[j] = R0[j] = R1[j] = R2[j] = N0[j] = N1[j] = N2[j] = 0.f;
             AR1[j] = AR2[j] = Gc[j] = KY0[j] = KY1[j] = KY2[j] = 0.f;
+            Q0[j] = Q1[j] = Q2[j] = 0.f;
         }
 #pragma unroll
         for (int j = 0; j < 5; ++j) KX1[j] = KX2[j] = 0.f;
@@ -2193,6 +2195,8 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                 N0[j] = N1[j];
                 N1[j] = N2[j];
                 AR1[j] = AR2[j];
+                Q0[j] = Q1[j];
+                Q1[j] = Q2[j];
                 KY0[j] = KY1[j];
                 KY1[j] = KY2[j];
             }
@@ -2203,6 +2207,7 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
             P1f = P2f;
             R1h = R2h;
             N1h = N2h;
+            Q1h = Q2h;
             KY1h = KY2h;
             float gL[4];
             const unsigned wL[2] = {B.g[0][0], B.g[0][1]};
@@ -2255,9 +2260,11 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                     const float l = j > 0 ? P1[j - 1] : pl;
                     const float r = j < 3 ? P1[j + 1] : pr;
                     N2[j] = smvp5f(P1[j], l, r, P0[j], P2[j], KX1[j], KX1[j + 1], KY1[j], KY2[j]);
-                    const float lr = j > 0 ? R1[j - 1] : rl;
-                    const float rrr = j < 3 ? R1[j + 1] : rr;
-                    AR2[j] = smvp5f(R1[j], lr, rrr, R0[j], R2[j], KX1[j], KX1[j + 1], KY1[j], KY2[j]);
+                    if (!UPD || !TL_CGP_AP1) {
+                        const float lr = j > 0 ? R1[j - 1] : rl;
+                        const float rrr = j < 3 ? R1[j + 1] : rr;
+                        AR2[j] = smvp5f(R1[j], lr, rrr, R0[j], R2[j], KX1[j], KX1[j + 1], KY1[j], KY2[j]);
+                    }
                 }
                 {
                     // A p at the column next to the wave (edge lanes)
@@ -2266,6 +2273,12 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                     const float kxl = lane == 0 ? KX1o : KX1[4], kxr = lane == 0 ? KX1[0] : KX1o;
                     N2h = smvp5f(P1h, l, r, P0h, P2h, kxl, kxr, KY1h, KY2h);
                 }
+                if constexpr (UPD && TL_CGP_AP1) {
+                    // p1 = r1 + beta0 p on row L-1 (and the next column), r1 = r - alpha0 A p
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) Q2[j] = __builtin_fmaf(fb0, P1[j], __builtin_fmaf(-fa0, N2[j], R1[j]));
+                    Q2h = __builtin_fmaf(fb0, P1h, __builtin_fmaf(-fa0, N2h, R1h));
+                }
             }
             // A^2 p on row L-2 and the row's sums or updates
             if (L >= y0 + 2) {
@@ -2273,9 +2286,32 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                 if (y < y1 && cl.active) {
                     const bool ybot = y <= 0, ytop = y >= g.ny - 1;
                     const float nl = nbr_l(N1[3], N1[0], N1h), nr = nbr_r(N1[0], N1[3], N1h);
+                    float ql = 0.f, qr = 0.f;
+                    if constexpr (UPD && TL_CGP_AP1) {
+                        ql = nbr_l(Q1[3], Q1[0], Q1h);
+                        qr = nbr_r(Q1[0], Q1[3], Q1h);
+                    }
+                    (void)ql;
+                    (void)qr;
                     float o_r[4], o_p[4], o_x[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
+                        if constexpr (UPD && TL_CGP_AP1) {
+                            // the second step as the fused kernel takes a step:
+                            // A p1 by the stencil of p1
+                            const float c = Q1[j];
+                            const float l = j > 0 ? Q1[j - 1] : ql;
+                            const float r = j < 3 ? Q1[j + 1] : qr;
+                            const float d = ybot ? c : Q0[j];
+                            const float u = ytop ? c : Q2[j];
+                            const float ap1 = smvp5f(c, l, r, d, u, KX2[j], KX2[j + 1], KY0[j], KY1[j]);
+                            const float r1 = __builtin_fmaf(-fa0, N1[j], R0[j]);
+                            o_x[j] = __builtin_fmaf(fa1, c, __builtin_fmaf(fa0, P0[j], B.x[0][j]));
+                            o_r[j] = __builtin_fmaf(-fa1, ap1, r1);
+                            o_p[j] = c;
+                            acc[0] = __builtin_fma((double)o_r[j], (double)o_r[j], acc[0]);
+                            continue;
+                        }
                         const float c = N1[j];
                         const float l = j > 0 ? N1[j - 1] : nl;
                         const float r = j < 3 ? N1[j + 1] : nr;

@@ -2030,7 +2030,9 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, c
 // Stops and refreshes are decided between pairs: the first sweep of pair t
 // tests gamma(t-1) as k_cgf does; when the pair's second step cannot be
 // taken (gamma1 or (p1, A p1) not positive, or t + 1 beyond the iteration
-// limit) the second sweep applies the first step only.
+// limit) the second sweep applies the first step only; when gamma1 already
+// meets the stop or refresh test, the second step is a null step and the
+// pair stops at t + 1 with p(t) kept.
 template <int FMT, bool UPD>
 __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, const float *__restrict__ rin,
                                                          float *__restrict__ rout, const float *__restrict__ pin,
@@ -2085,8 +2087,10 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
     // sweep's partials
     FBatch<1> B0, B1;
     load_batch(B0, cl.y0 - 2);
-    // scalars of the pair, identical in every block
+    // scalars of the pair, identical in every block; keep: the second step is
+    // a null step that leaves p(t) as the direction
     double bd, a0 = 0.0, b0 = 0.0, a1 = 0.0;
+    bool keep = false;
     if constexpr (!UPD) {
         double gam;
         if (sin) {
@@ -2121,17 +2125,26 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
             (rar - 2.0 * a0 * apar + a0 * a0 * apa2p) + 2.0 * b0 * (rap - a0 * apap) + b0 * b0 * pap;
         a1 = g1 / p1ap1;
         const bool cut = t + 1 > ctrl->maxiters;
-        const bool two = ok && !cut && g1 > 0.0 && p1ap1 > 0.0 && isfinite(a1);
+        // the stop and refresh tests on gamma1 = (r(t), r(t)) as well: CG's
+        // residual norm is not monotone, so a refresh the first step reaches
+        // could be gone after the second.  Then the second step is a null
+        // step (alpha1 = 0) that keeps p(t) and gamma(t-1), so that after the
+        // refresh beta = gamma_new / gamma(t-1) as after a refresh at t, and
+        // the pair stops at t + 1.
+        const int why1 = ok && !cut && isfinite(g1) ? decide_why(ctrl, t, g1) : WHY_NONE;
+        const bool early = why1 == WHY_REFRESH || why1 == WHY_CONVERGED;
+        const bool two = ok && !cut && !early && g1 > 0.0 && p1ap1 > 0.0 && isfinite(a1);
         if (!two) {
             a1 = 0.0;
             b0 = 0.0;
         }
+        keep = early;
         if (block_id() == 0 && threadIdx.x == 0) {
             ctrl->pw = pap;
             ctrl->alpha = two ? a1 : a0;
             ctrl->alphas[t & 1] = a0;
             ctrl->alphas[(t + 1) & 1] = a1;
-            ctrl->rrs[t & 1] = g1 > 0.0 ? g1 : gam;
+            ctrl->rrs[t & 1] = early ? gam : g1 > 0.0 ? g1 : gam;
             ctrl->iters = cut ? t : t + 1;
             if (!ok) {  // non-positive curvature: stop before this pair's update
                 ctrl->status = TL_E_BREAKDOWN;
@@ -2140,6 +2153,8 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                 ctrl->stop_iter = t - 1;
             } else if (cut) {
                 record_stop(ctrl, t, WHY_MAXITERS);
+            } else if (early) {
+                record_stop(ctrl, t + 1, why1);
             }
         }
         if (!ok) return;
@@ -2308,7 +2323,7 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                             const float r1 = __builtin_fmaf(-fa0, N1[j], R0[j]);
                             o_x[j] = __builtin_fmaf(fa1, c, __builtin_fmaf(fa0, P0[j], B.x[0][j]));
                             o_r[j] = __builtin_fmaf(-fa1, ap1, r1);
-                            o_p[j] = c;
+                            o_p[j] = keep ? P0[j] : c;
                             acc[0] = __builtin_fma((double)o_r[j], (double)o_r[j], acc[0]);
                             continue;
                         }
@@ -2339,7 +2354,7 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                             const float ap1 = __builtin_fmaf(fb0, apv, ar1);
                             o_x[j] = __builtin_fmaf(fa1, p1, __builtin_fmaf(fa0, pv, B.x[0][j]));
                             o_r[j] = __builtin_fmaf(-fa1, ap1, r1);
-                            o_p[j] = p1;
+                            o_p[j] = keep ? pv : p1;
                             acc[0] = __builtin_fma((double)o_r[j], (double)o_r[j], acc[0]);
                         }
                     }

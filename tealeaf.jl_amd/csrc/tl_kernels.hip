@@ -2019,16 +2019,18 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, c
 //   x += alpha0 p + alpha1 p1,  r(t+1) = r1 - alpha1 A p1,  p(t+1) = p1.
 // The first sweep (UPD false) forms p, A p, A^2 p and A r and sums the six
 // products (p, Ap), (r, Ap), (Ap, Ap), (r, Ar), (Ap, Ar), (Ap, A^2 p); it
-// stores nothing.  The second (UPD true) forms the same vectors bit for bit,
-// derives the scalars from the six sums and writes r(t+1), p(t+1) and x,
-// with (r(t+1), r(t+1)) for the next pair.  Bytes per cell for the two
+// stores nothing.  The second (UPD true) forms p and A p bit for bit as the
+// first did, derives the scalars from the six sums, forms p1 on three rows
+// and applies the stencil to it (TL_CGP_AP1; else A p1 from the basis as
+// above), and writes r(t+1), p(t+1) and x, with (r(t+1), r(t+1)) for the
+// next pair.  Bytes per cell for the two
 // iterations: 10 (r, p', 16-bit gc) + 26 (also x; writes r, p, x) = 36, or
 // 18 per iteration against 22 for k_cgf.  The walk is k_cgf's: rows L-3 .. L
 // in registers (p at L-2..L, r at L-2..L, A p at L-3..L-1, A r at L-2, L-1),
 // A p and A r at row L-1, A^2 p and the row's results at L-2; the edge lanes
 // compute A p at the column next to the wave from the two outside columns.
-// Stops and refreshes are decided between pairs: the first sweep of pair t
-// tests gamma(t-1) as k_cgf does; when the pair's second step cannot be
+// The first sweep of pair t applies the stop and refresh tests to
+// gamma(t-1) as k_cgf does; when the pair's second step cannot be
 // taken (gamma1 or (p1, A p1) not positive, or t + 1 beyond the iteration
 // limit) the second sweep applies the first step only; when gamma1 already
 // meets the stop or refresh test, the second step is a null step and the

@@ -32,6 +32,7 @@
 // solve is expressed by Ctrl::stop_iter: launches for later iterations exit
 // at once.
 #include <algorithm>
+#include <type_traits>
 #include <climits>
 #include <cmath>
 #include <mutex>
@@ -2034,9 +2035,12 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgf(Geo g, int ty, int t, c
 // taken (gamma1 or (p1, A p1) not positive, or t + 1 beyond the iteration
 // limit) the second sweep applies the first step only; when gamma1 already
 // meets the stop or refresh test, the second step is a null step and the
-// pair stops at t + 1 with p(t) kept.
+// pair stops at t + 1 with p(t) kept.  Both sweeps run 3 waves per SIMD (768
+// blocks, one round): the second needs 172 VGPRs unconstrained (2 waves);
+// held to 168 it spills 4 and runs as fast as before its row windows were
+// indexed by row (the first sweep: 147 VGPRs, 4% faster).
 template <int FMT, bool UPD>
-__global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, const float *__restrict__ rin,
+__global__ __launch_bounds__(TBV) __attribute__((amdgpu_waves_per_eu(3))) void k_cgp(Geo g, int ty, int t, const float *__restrict__ rin,
                                                          float *__restrict__ rout, const float *__restrict__ pin,
                                                          float *__restrict__ pout, float *__restrict__ x,
                                                          const tl_gc_t *__restrict__ gc, float hx, float hy,
@@ -2087,8 +2091,8 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
 
     // the band's first row is requested before the block sums the previous
     // sweep's partials
-    FBatch<1> B0, B1;
-    load_batch(B0, cl.y0 - 2);
+    FBatch<1> Bs[2];
+    load_batch(Bs[0], cl.y0 - 2);
     // scalars of the pair, identical in every block; keep: the second step is
     // a null step that leaves p(t) as the direction
     double bd, a0 = 0.0, b0 = 0.0, a1 = 0.0;
@@ -2169,24 +2173,35 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
 
     {
         const int y0 = cl.y0, y1 = cl.y1;
-        float P0[4], P1[4], P2[4], P0h = 0.f, P1h = 0.f, P1f = 0.f, P2h = 0.f, P2f = 0.f;
-        float R0[4], R1[4], R2[4], R1h = 0.f, R2h = 0.f;
-        float N0[4], N1[4], N2[4], N1h = 0.f, N2h = 0.f;  // A p at L-3, L-2, L-1
-        float AR1[4], AR2[4];                            // A r at L-2, L-1
-        float Q0[4], Q1[4], Q2[4], Q1h = 0.f, Q2h = 0.f; // second sweep: p1 at L-3, L-2, L-1
-        float Gc[4], Gch = 0.f, Gcf = 0.f;
-        unsigned Gw[2] = {0u, 0u};
-        float KX1[5], KX2[5], KX1o = 0.f;
-        float KY0[4], KY1[4], KY2[4], KY1h = 0.f, KY2h = 0.f;
+        // Row windows indexed by row: row m keeps slot (m - y0 + 2) % 3 of the
+        // three-row windows and (m - y0 + 2) % 2 of the two-row ones while the
+        // walk moves on, and the walk is unrolled six rows at a time, so every
+        // index is a constant and no value moves between registers.  K = the
+        // step's row L minus (y0 - 2), modulo 6.
+        float P[3][4], Ph[3], Pf[2];     // p at L-2..L (own, hcol); fcol at L-1, L
+        float R[3][4], Rh[2];            // r at L-2..L; hcol at L-1, L
+        float N[3][4], Nh[2];            // A p at L-3..L-1; hcol at L-2, L-1
+        float AR[2][4];                  // A r at L-2, L-1
+        float Q[3][4], Qh[2];            // second sweep: p1 at L-3..L-1; hcol at L-2, L-1
+        float Gc[2][4], Gch[2], Gcf[2];  // coefficients of rows L-1, L (own, hcol, fcol)
+        unsigned Gw[2][2];               // their raw words (for the neighbour lanes)
+        float KX[2][5];                  // kx of rows L-2, L-1 (5 faces)
+        float KY[3][4], KYh[2];          // ky faces L-3|L-2, L-2|L-1, L-1|L; hcol of the last two
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            P0[j] = P1[j] = P2[j] = R0[j] = R1[j] = R2[j] = N0[j] = N1[j] = N2[j] = 0.f;
-            AR1[j] = AR2[j] = Gc[j] = KY0[j] = KY1[j] = KY2[j] = 0.f;
-            Q0[j] = Q1[j] = Q2[j] = 0.f;
+        for (int w = 0; w < 3; ++w) {
+            Ph[w] = 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) P[w][j] = R[w][j] = N[w][j] = Q[w][j] = KY[w][j] = 0.f;
         }
 #pragma unroll
-        for (int j = 0; j < 5; ++j) KX1[j] = KX2[j] = 0.f;
-        (void)P0h;
+        for (int w = 0; w < 2; ++w) {
+            Pf[w] = Rh[w] = Nh[w] = Qh[w] = Gch[w] = Gcf[w] = KYh[w] = 0.f;
+            Gw[w][0] = Gw[w][1] = 0u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) AR[w][j] = Gc[w][j] = 0.f;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) KX[w][j] = 0.f;
+        }
 
         auto nbr_l = [&](float src3, float c0, float hval) -> float {
             float v = dpp_up1f(src3);
@@ -2201,31 +2216,11 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
             return v;
         };
 
-        auto compute_row = [&](const FBatch<1> &B, int L) {
-            if (L > y1 + 1) return;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                P0[j] = P1[j];
-                P1[j] = P2[j];
-                R0[j] = R1[j];
-                R1[j] = R2[j];
-                N0[j] = N1[j];
-                N1[j] = N2[j];
-                AR1[j] = AR2[j];
-                Q0[j] = Q1[j];
-                Q1[j] = Q2[j];
-                KY0[j] = KY1[j];
-                KY1[j] = KY2[j];
-            }
-#pragma unroll
-            for (int j = 0; j < 5; ++j) KX2[j] = KX1[j];
-            P0h = P1h;
-            P1h = P2h;
-            P1f = P2f;
-            R1h = R2h;
-            N1h = N2h;
-            Q1h = Q2h;
-            KY1h = KY2h;
+        auto compute_row = [&](const FBatch<1> &B, int L, auto kc) {
+            constexpr int K = decltype(kc)::value;
+            // slots: three-row windows (row L: a, L-1: b, L-2: c), two-row
+            // windows (row L: e, L-1: f)
+            constexpr int a = K % 3, b = (K + 2) % 3, c = (K + 1) % 3, e = K % 2, f = (K + 1) % 2;
             float gL[4];
             const unsigned wL[2] = {B.g[0][0], B.g[0][1]};
 #pragma unroll
@@ -2234,79 +2229,84 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                 gL[2 * k + 1] = gc_word<FMT>(wL[k], 1);
             }
             const float gLh = gc_word<FMT>(B.eg[0], hk), gLf = gc_word<FMT>(B.eg[0], 1 - hk);
+            float KXo;  // fcol|hcol face of row L-1
             {
-                float gl = gc_word<FMT>(dpp_up1u(Gw[1]), 1), gr = gc_word<FMT>(dpp_down1u(Gw[0]), 0);
-                if (eleft) gl = Gch;
-                if (eright) gr = Gch;
-                KX1[0] = hx * (gl + Gc[0]);
+                // faces of row L-1 (slot f) from its coefficients and row L's
+                float gl = gc_word<FMT>(dpp_up1u(Gw[f][1]), 1), gr = gc_word<FMT>(dpp_down1u(Gw[f][0]), 0);
+                if (eleft) gl = Gch[f];
+                if (eright) gr = Gch[f];
+                KX[f][0] = hx * (gl + Gc[f][0]);
 #pragma unroll
-                for (int j = 1; j < 4; ++j) KX1[j] = hx * (Gc[j - 1] + Gc[j]);
-                KX1[4] = hx * (Gc[3] + gr);
-                KX1o = lane == 0 ? hx * (Gcf + Gch) : hx * (Gch + Gcf);
+                for (int j = 1; j < 4; ++j) KX[f][j] = hx * (Gc[f][j - 1] + Gc[f][j]);
+                KX[f][4] = hx * (Gc[f][3] + gr);
+                KXo = lane == 0 ? hx * (Gcf[f] + Gch[f]) : hx * (Gch[f] + Gcf[f]);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) KY2[j] = hy * (Gc[j] + gL[j]);
-                KY2h = hy * (Gch + gLh);
+                for (int j = 0; j < 4; ++j) KY[a][j] = hy * (Gc[f][j] + gL[j]);
+                KYh[e] = hy * (Gch[f] + gLh);
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) Gc[j] = gL[j];
-            Gch = gLh;
-            Gcf = gLf;
-            Gw[0] = wL[0];
-            Gw[1] = wL[1];
+            for (int j = 0; j < 4; ++j) Gc[e][j] = gL[j];
+            Gch[e] = gLh;
+            Gcf[e] = gLf;
+            Gw[e][0] = wL[0];
+            Gw[e][1] = wL[1];
             // row L: r and p = r + beta p'
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                R2[j] = B.r[0][j];
-                P2[j] = __builtin_fmaf(beta, B.p[0][j], B.r[0][j]);
+                R[a][j] = B.r[0][j];
+                P[a][j] = __builtin_fmaf(beta, B.p[0][j], B.r[0][j]);
             }
             {
                 const float rh = B.er[0][hk], rf = B.er[0][1 - hk];
-                P2h = __builtin_fmaf(beta, B.ep[0][hk], rh);
-                P2f = __builtin_fmaf(beta, B.ep[0][1 - hk], rf);
-                R2h = rh;
+                Ph[a] = __builtin_fmaf(beta, B.ep[0][hk], rh);
+                Pf[e] = __builtin_fmaf(beta, B.ep[0][1 - hk], rf);
+                Rh[e] = rh;
             }
             // A p and A r on row L-1.  The rows outside the grid were loaded
             // clamped, so p and r there equal the edge row's, bit for bit:
             // the zero-flux neighbours need no select here (A^2 p below, from
             // A p of rows outside the grid, does)
             if (L >= y0) {
-                const float pl = nbr_l(P1[3], P1[0], P1h), pr = nbr_r(P1[0], P1[3], P1h);
-                const float rl = nbr_l(R1[3], R1[0], R1h), rr = nbr_r(R1[0], R1[3], R1h);
+                const float pl = nbr_l(P[b][3], P[b][0], Ph[b]), pr = nbr_r(P[b][0], P[b][3], Ph[b]);
+                const float rl = nbr_l(R[b][3], R[b][0], Rh[f]), rr = nbr_r(R[b][0], R[b][3], Rh[f]);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const float l = j > 0 ? P1[j - 1] : pl;
-                    const float r = j < 3 ? P1[j + 1] : pr;
-                    N2[j] = smvp5f(P1[j], l, r, P0[j], P2[j], KX1[j], KX1[j + 1], KY1[j], KY2[j]);
+                    const float l = j > 0 ? P[b][j - 1] : pl;
+                    const float r = j < 3 ? P[b][j + 1] : pr;
+                    N[b][j] = smvp5f(P[b][j], l, r, P[c][j], P[a][j], KX[f][j], KX[f][j + 1], KY[b][j], KY[a][j]);
                     if (!UPD || !TL_CGP_AP1) {
-                        const float lr = j > 0 ? R1[j - 1] : rl;
-                        const float rrr = j < 3 ? R1[j + 1] : rr;
-                        AR2[j] = smvp5f(R1[j], lr, rrr, R0[j], R2[j], KX1[j], KX1[j + 1], KY1[j], KY2[j]);
+                        const float lr = j > 0 ? R[b][j - 1] : rl;
+                        const float rrr = j < 3 ? R[b][j + 1] : rr;
+                        AR[f][j] =
+                            smvp5f(R[b][j], lr, rrr, R[c][j], R[a][j], KX[f][j], KX[f][j + 1], KY[b][j], KY[a][j]);
                     }
                 }
                 {
                     // A p at the column next to the wave (edge lanes)
-                    const float own = lane == 0 ? P1[0] : P1[3];
-                    const float l = lane == 0 ? P1f : own, r = lane == 0 ? own : P1f;
-                    const float kxl = lane == 0 ? KX1o : KX1[4], kxr = lane == 0 ? KX1[0] : KX1o;
-                    N2h = smvp5f(P1h, l, r, P0h, P2h, kxl, kxr, KY1h, KY2h);
+                    const float own = lane == 0 ? P[b][0] : P[b][3];
+                    const float l = lane == 0 ? Pf[f] : own, r = lane == 0 ? own : Pf[f];
+                    const float kxl = lane == 0 ? KXo : KX[f][4], kxr = lane == 0 ? KX[f][0] : KXo;
+                    Nh[f] = smvp5f(Ph[b], l, r, Ph[c], Ph[a], kxl, kxr, KYh[f], KYh[e]);
                 }
                 if constexpr (UPD && TL_CGP_AP1) {
                     // p1 = r1 + beta0 p on row L-1 (and the next column), r1 = r - alpha0 A p
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) Q2[j] = __builtin_fmaf(fb0, P1[j], __builtin_fmaf(-fa0, N2[j], R1[j]));
-                    Q2h = __builtin_fmaf(fb0, P1h, __builtin_fmaf(-fa0, N2h, R1h));
+                    for (int j = 0; j < 4; ++j)
+                        Q[b][j] = __builtin_fmaf(fb0, P[b][j], __builtin_fmaf(-fa0, N[b][j], R[b][j]));
+                    Qh[f] = __builtin_fmaf(fb0, Ph[b], __builtin_fmaf(-fa0, Nh[f], Rh[f]));
                 }
             }
-            // A^2 p on row L-2 and the row's sums or updates
+            // A^2 p on row L-2 and the row's sums or updates (row L-3: slot a
+            // of the three-row windows that end at L-1)
             if (L >= y0 + 2) {
                 const int y = L - 2;
                 if (y < y1 && cl.active) {
                     const bool ybot = y <= 0, ytop = y >= g.ny - 1;
-                    const float nl = nbr_l(N1[3], N1[0], N1h), nr = nbr_r(N1[0], N1[3], N1h);
+                    const float nl = nbr_l(N[c][3], N[c][0], Nh[e]), nr = nbr_r(N[c][0], N[c][3], Nh[e]);
                     float ql = 0.f, qr = 0.f;
                     if constexpr (UPD && TL_CGP_AP1) {
-                        ql = nbr_l(Q1[3], Q1[0], Q1h);
-                        qr = nbr_r(Q1[0], Q1[3], Q1h);
+                        ql = nbr_l(Q[c][3], Q[c][0], Qh[e]);
+                        qr = nbr_r(Q[c][0], Q[c][3], Qh[e]);
                     }
                     (void)ql;
                     (void)qr;
@@ -2316,26 +2316,26 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
                         if constexpr (UPD && TL_CGP_AP1) {
                             // the second step as the fused kernel takes a step:
                             // A p1 by the stencil of p1
-                            const float c = Q1[j];
-                            const float l = j > 0 ? Q1[j - 1] : ql;
-                            const float r = j < 3 ? Q1[j + 1] : qr;
-                            const float d = ybot ? c : Q0[j];
-                            const float u = ytop ? c : Q2[j];
-                            const float ap1 = smvp5f(c, l, r, d, u, KX2[j], KX2[j + 1], KY0[j], KY1[j]);
-                            const float r1 = __builtin_fmaf(-fa0, N1[j], R0[j]);
-                            o_x[j] = __builtin_fmaf(fa1, c, __builtin_fmaf(fa0, P0[j], B.x[0][j]));
+                            const float cq = Q[c][j];
+                            const float l = j > 0 ? Q[c][j - 1] : ql;
+                            const float r = j < 3 ? Q[c][j + 1] : qr;
+                            const float d = ybot ? cq : Q[a][j];
+                            const float u = ytop ? cq : Q[b][j];
+                            const float ap1 = smvp5f(cq, l, r, d, u, KX[e][j], KX[e][j + 1], KY[c][j], KY[b][j]);
+                            const float r1 = __builtin_fmaf(-fa0, N[c][j], R[c][j]);
+                            o_x[j] = __builtin_fmaf(fa1, cq, __builtin_fmaf(fa0, P[c][j], B.x[0][j]));
                             o_r[j] = __builtin_fmaf(-fa1, ap1, r1);
-                            o_p[j] = keep ? P0[j] : c;
+                            o_p[j] = keep ? P[c][j] : cq;
                             acc[0] = __builtin_fma((double)o_r[j], (double)o_r[j], acc[0]);
                             continue;
                         }
-                        const float c = N1[j];
-                        const float l = j > 0 ? N1[j - 1] : nl;
-                        const float r = j < 3 ? N1[j + 1] : nr;
-                        const float d = ybot ? c : N0[j];
-                        const float u = ytop ? c : N2[j];
-                        const float a2p = smvp5f(c, l, r, d, u, KX2[j], KX2[j + 1], KY0[j], KY1[j]);
-                        const float pv = P0[j], apv = N1[j], rv = R0[j], arv = AR1[j];
+                        const float cn = N[c][j];
+                        const float l = j > 0 ? N[c][j - 1] : nl;
+                        const float r = j < 3 ? N[c][j + 1] : nr;
+                        const float d = ybot ? cn : N[a][j];
+                        const float u = ytop ? cn : N[b][j];
+                        const float a2p = smvp5f(cn, l, r, d, u, KX[e][j], KX[e][j + 1], KY[c][j], KY[b][j]);
+                        const float pv = P[c][j], apv = N[c][j], rv = R[c][j], arv = AR[e][j];
                         if constexpr (!UPD) {
                             // a product of two FP32 values is exact in FP64, so
                             // the fused multiply-add rounds as acc + a * b does
@@ -2372,13 +2372,20 @@ __global__ __launch_bounds__(TBV) TL_CGF_ATTR void k_cgp(Geo g, int ty, int t, c
         // rows y0-2 .. y1+1, the next row's loads issued before the current
         // row is computed (the first one above; two rows ahead in the first
         // sweep measured no faster: it is bound by its arithmetic)
-        for (int L = y0 - 2; L < y1 + 2; L += 2) {
-            const bool more = L + 1 < y1 + 2;
-            if (more) load_batch(B1, L + 1);
-            compute_row(B0, L);
-            if (!more) break;
-            if (L + 2 < y1 + 2) load_batch(B0, L + 2);
-            compute_row(B1, L + 1);
+        auto step = [&](auto kc, int L) -> bool {
+            constexpr int K = decltype(kc)::value;
+            if (L > y1 + 1) return false;
+            if (L + 1 <= y1 + 1) load_batch(Bs[(K + 1) % 2], L + 1);
+            compute_row(Bs[K % 2], L, kc);
+            return true;
+        };
+        for (int L = y0 - 2; L <= y1 + 1; L += 6) {
+            if (!step(std::integral_constant<int, 0>{}, L)) break;
+            if (!step(std::integral_constant<int, 1>{}, L + 1)) break;
+            if (!step(std::integral_constant<int, 2>{}, L + 2)) break;
+            if (!step(std::integral_constant<int, 3>{}, L + 3)) break;
+            if (!step(std::integral_constant<int, 4>{}, L + 4)) break;
+            if (!step(std::integral_constant<int, 5>{}, L + 5)) break;
         }
     }
     write_partials<NS, TBV>(acc, sout, g.nbmax);

@@ -64,6 +64,10 @@ def test_pairs_match_oracle_and_fused_form(nx, ny):
         print(f"  step {t + 1}: pairs {pi} ({pst[t]['refreshes']} refreshes), fused {fi} "
               f"({fst[t]['refreshes']})", flush=True)
         assert abs(pi - fi) <= max(6, 0.05 * fi), (t, pi, fi)
+        # the pair tests the refresh rule after its first step as well, so it
+        # refreshes as often as the one-kernel form (the second step of a pair
+        # that reaches a refresh is a null step)
+        assert pst[t]["refreshes"] == fst[t]["refreshes"], (t, pst[t]["refreshes"], fst[t]["refreshes"])
 
 
 def test_pairs_validation_and_iteration_limit():

@@ -30,6 +30,27 @@ def test_exports_match_header():
     assert declared <= set(_lib.EXPORTS) | {"tl_step_cb"}, declared - set(_lib.EXPORTS)
 
 
+def test_constants_match_header():
+    """Every integer #define of the header that the Python mirror names has the
+    same value there; the field ids follow the mirror's field order."""
+    hdr = open(os.path.join(ROOT, "include", "tealeaf_hip.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define\s+(TL_[A-Z0-9_]+)\s+(-?\d+)\b", hdr)}
+    assert len(defs) > 50
+    named = 0
+    for k, v in defs.items():
+        if k.startswith("TL_F_"):
+            assert _lib.FIELDS[v] == k[len("TL_F_"):].lower(), (k, v)
+            named += 1
+            continue
+        obj = getattr(_lib, k, getattr(T, k, None))
+        if obj is not None:
+            assert int(obj) == v, (k, v, obj)
+            named += 1
+    assert defs["TL_NFIELDS"] == len(_lib.FIELDS)
+    assert defs["TL_K_COUNT"] == _lib.TL_K_COUNT and set(_lib.KERNEL_NAMES) >= set(range(_lib.TL_K_COUNT))
+    assert named >= len(defs) - 3, named
+
+
 def test_struct_sizes_match_header():
     # the ctypes mirrors must agree with the C layout the library was built with
     p = _lib.SolveParams()

@@ -172,6 +172,24 @@ def roofline_from(prof, n, coef_format=None):
     }
 
 
+def pair_summary(prof, n):
+    """The FP32 CG pair as a whole (both sweeps per two iterations): 36 B/cell
+    of algorithmic bytes over the sum of the two sweeps' average live launch
+    durations; None when the pair form did not run."""
+    d = [prof.get(_lib.TL_K_CGP_G32), prof.get(_lib.TL_K_CGP_U32)]
+    if any(x is None or len(x) == 0 for x in d):
+        return None
+    ms = []
+    for x in d:
+        live = x[x > 0.25 * float(np.median(x))]
+        ms.append(float(live.mean()))
+    b = (KERNEL_BYTES[_lib.TL_K_CGP_G32] + KERNEL_BYTES[_lib.TL_K_CGP_U32]) * n * n
+    gbs = b / ((ms[0] + ms[1]) * 1e-3) / 1e9
+    return {"first_sweep_ms": round(ms[0], 5), "second_sweep_ms": round(ms[1], 5),
+            "ms_per_iteration": round((ms[0] + ms[1]) / 2, 5), "bytes_per_cell_per_iteration": b // (2 * n * n),
+            "achieved_GBps": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4)}
+
+
 def cpu_model():
     """CPU model name of this host (first 'model name' of /proc/cpuinfo)."""
     try:
@@ -515,6 +533,9 @@ def main():
         out["flagship"]["summaries"] = [list(map(float, x)) for x in flag["summaries"]]
     fmt = flag["stats"][0].get("coef_format") if flag["stats"] else None
     out["roofline"] = roofline_from(flag["prof"], a.n, fmt)
+    pair = pair_summary(flag["prof"], a.n)
+    if pair is not None:
+        out["fp32_pair"] = pair
     # every profiled kernel of the flagship's timed steps, and the share of the
     # timed wall time they cover (launches past a solve's stop excluded; the
     # totals scale the sampled launches by the sampling period)

@@ -37,12 +37,29 @@ def test_roofline_picks_dominant_kernel_and_excludes_stopped_launches():
 def test_traffic_record_covers_the_profiled_kernels():
     with open(bench.PMC_TRAFFIC) as f:
         rec = json.load(f)["kernels"]
-    for kid in (_lib.TL_K_CG_W64, _lib.TL_K_CG_UR64, _lib.TL_K_CG_W32, _lib.TL_K_CG_UR32):
+    for kid in (_lib.TL_K_CG_W64, _lib.TL_K_CG_UR64, _lib.TL_K_CG_W32, _lib.TL_K_CG_UR32, _lib.TL_K_CGF32,
+                _lib.TL_K_CGP_G32, _lib.TL_K_CGP_U32):
         name = _lib.kernel_name(kid)
         assert name in rec, name
         # counter traffic is at least the algorithmic bytes and within 15% of them
         algo = bench.KERNEL_BYTES[kid] * 8192 * 8192
         assert algo <= rec[name]["traffic_bytes"] < 1.15 * algo, (name, rec[name]["traffic_bytes"], algo)
+
+
+def test_pair_summary_and_roofline_of_the_pair_form():
+    """With the pair form the dominant kernel is its second sweep (26 B/cell);
+    the pair record adds both sweeps: 36 B/cell per two iterations."""
+    g = np.concatenate([np.full(40, 0.14), np.full(5, 0.002)])
+    u = np.concatenate([np.full(40, 0.34), np.full(5, 0.003)])
+    prof = {_lib.TL_K_CGP_G32: g, _lib.TL_K_CGP_U32: u}
+    r = bench.roofline_from(prof, 8192, _lib.TL_COEF_FP16)
+    assert r["kernel"] == "k_cgp<1, true>" and r["bytes_per_cell"] == 26 and r["timed_launches"] == 40
+    assert r["traffic"] is not None
+    pr = bench.pair_summary(prof, 8192)
+    assert pr["first_sweep_ms"] == 0.14 and pr["second_sweep_ms"] == 0.34
+    assert pr["bytes_per_cell_per_iteration"] == 18 and pr["ms_per_iteration"] == 0.24
+    assert abs(pr["achieved_GBps"] - 36 * 8192 * 8192 / 0.48e-3 / 1e9) < 0.1
+    assert bench.pair_summary({_lib.TL_K_CGF32: np.full(4, 0.28)}, 8192) is None
 
 
 def test_more_than_one_gpu_is_refused():
